@@ -1,0 +1,142 @@
+/* CPU oracle for the GANDALF particle hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C (OpenMP) restatement of the reference algorithms, used as the checker for the
+ * HIP path and as the CPU baseline timed beside it.  Only tests/, the smoke check and the
+ * benchmark's cpu_baseline leg may load it.  The product path never calls into it.
+ *
+ * Every function cites the reference file:line it restates.  Layout conventions: positions,
+ * velocities and accelerations are (N,3) row-major double arrays (r[3*i+k]); everything else
+ * is a length-N double array.  Particles are identified by their index in the input arrays.
+ */
+#ifndef GDF_ORACLE_H
+#define GDF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int kernel;              /* 0 = M4, 1 = quintic */
+  int tabulated;           /* 1 = 1000-entry nearest-lower tables (SmoothingKernel.h:548-756) */
+  double h_fac, h_converge;
+  int eos;                 /* 0 = energy_eqn (adiabatic), 1 = isothermal, 2 = barotropic */
+  double gamma, temp0, mu_bar, rho_bary;
+  int energy_integration;  /* 1 when u is integrated (energy_eqn) */
+  double alpha_visc, beta_visc;
+  int hydro_forces, self_gravity;
+  double thetamaxsqd;
+  int nleafmax;            /* leaf bucket size of the tree walks */
+  int periodic;            /* 1 = all three axes periodic */
+  double boxmin[3], boxmax[3];
+  double courant_mult, accel_mult, energy_mult;
+} orc_params;
+
+enum { ORC_W0 = 0, ORC_W1, ORC_WOMEGA, ORC_WZETA, ORC_WGRAV, ORC_WPOT,
+       ORC_W0_S2, ORC_WOMEGA_S2, ORC_WZETA_S2 };
+
+/* kernel functions (SmoothingKernel.h:132-408, tabulated :583-637) */
+double orc_kernel_eval(const orc_params* p, int which, double s);
+double orc_kernel_range(const orc_params* p);
+
+/* initial smoothing length guess (Sph.cpp:76-113, 3D branch) */
+double orc_h_guess(const orc_params* p, int N, const double* r);
+
+/* exact neighbour sets, ascending j per particle.  mode 0 = gather {j : r2 < k^2 h_i^2};
+ * mode 1 = scatter-gather {j : r2 < k^2 h_i^2 or r2 < k^2 h_j^2} (NeighbourManager.h:521);
+ * mode 2 = search {j : r2 <= k^2 h_i^2} with h the search length.  Returns total entries,
+ * or -(needed) if cap is too small.  offsets has N+1 entries. */
+int64_t orc_neighbour_sets(const orc_params* p, int N, const double* r, const double* h,
+                           int mode, int64_t* offsets, int32_t* ids, int64_t cap);
+
+/* grad-h h-rho iteration for every particle (GradhSph.cpp:142-326 driven as
+ * GradhSphTree.cpp:83-271), then the EOS (GradhSph.cpp:335-347).  h is in/out; u is in/out
+ * (set by isothermal/barotropic EOS).  nopbc=1 ignores periodicity (first setup pass).
+ * Returns 0, or -1 if an h iteration failed to converge. */
+int orc_compute_h(const orc_params* p, int N, const double* r, const double* m, int nopbc,
+                  double* h, double* rho, double* invomega, double* zeta, double* hfactor,
+                  double* u, double* sound, double* pressure);
+
+/* hydro forces over the scatter-gather sets; variant 0 = ComputeSphHydroForces
+ * (GradhSph.cpp:361-460), variant 1 = hydro part of ComputeSphHydroGravForces (:474-585).
+ * Overwrites a, dudt, div_v. */
+void orc_hydro_forces(const orc_params* p, int variant, int N, const double* r,
+                      const double* v, const double* m, const double* h, const double* rho,
+                      const double* invomega, const double* pressure, const double* sound,
+                      const double* hfactor, double* a, double* dudt, double* div_v);
+
+/* FP64 direct O(N^2) kernel-softened gravity, the theta -> 0 limit of the walk: neighbour
+ * pairs as GradhSph.cpp:555-565, others as :657-690, gpot seeded with the self term as
+ * GradhSphTree.cpp:512.  Overwrites atree and gpot. */
+void orc_direct_gravity(const orc_params* p, int N, const double* r, const double* m,
+                        const double* h, const double* zeta, const double* hfactor,
+                        double* atree, double* gpot);
+
+/* Plummer-softened point-mass direct sum (G=1): a_i = sum m_j dr/(r^2+eps^2)^1.5 */
+void orc_direct_plummer(int N, const double* r, const double* m, double eps,
+                        const int32_t* targets, int ntargets, double* a, double* pot);
+
+/* ---- CPU twin of the device tree (Morton keys, LBVH, bottom-up moments, groups) ----
+ * Arrays are sized: perm[N], parent[2N-1], left/right/first/last[N-1] (internal nodes),
+ * nodes' moments [2N-1] in the unified index (internal 0..N-2, leaf l -> N-1+l).
+ * group_first/group_count/group_node sized N (returns number of groups). */
+typedef struct {
+  int N;
+  int32_t *perm;                         /* sorted slot -> input index */
+  uint64_t *keys;                        /* sorted keys */
+  int32_t *left, *right, *first, *last;  /* internal nodes */
+  int32_t *parent;                       /* unified index */
+  double *mass, *com, *quad, *bbmin, *bbmax, *hmax, *cdistsqd;  /* com/bb x3, quad x5 */
+  int32_t *group_first, *group_count, *group_node;
+  int ngroups;
+} orc_tree;
+
+int orc_tree_alloc(orc_tree* t, int N);
+void orc_tree_free(orc_tree* t);
+/* r, m, h in input order; builds everything.  group_max = particles per walk group. */
+int orc_tree_build(const orc_params* p, orc_tree* t, const double* r, const double* m,
+                   const double* h, int group_max);
+/* refresh hmax bottom-up after h changed (KDTree.cpp:1128-1208 analogue); h in input order */
+void orc_tree_refit_h(const orc_params* p, orc_tree* t, const double* h);
+
+/* tree gravity with the device walk's acceptance rules; r,m,h,zeta,hfactor in input order.
+ * If ncell/nnear are non-null they receive per-group interaction counts. */
+void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
+                      const double* m, const double* h, const double* zeta,
+                      const double* hfactor, double* atree, double* gpot,
+                      int32_t* ncell, int32_t* nnear);
+
+/* ---- integrator (SphLeapfrogKDK.cpp:76-272) and timesteps (SphIntegration.cpp:81-134) ---- */
+void orc_advance(int N, double dt, int energy_integration, const double* r0,
+                 const double* v0, const double* a0, const double* u0, const double* dudt0,
+                 double* r, double* v, double* u);
+void orc_end_step(int N, double dtstep, int energy_integration, const double* a,
+                  const double* dudt, double* v, double* u, double* r0, double* v0,
+                  double* a0, double* u0, double* dudt0, const double* r);
+double orc_timesteps(const orc_params* p, int N, const double* h, const double* sound,
+                     const double* div_v, const double* a, const double* u,
+                     const double* dudt, double* dt_out);
+void orc_wrap_periodic(const orc_params* p, int N, double* r);
+
+/* diagnostics (SimAnalysis.hpp:52-185): out = {Etot, ketot, utot, gpetot, mom[3],
+ * angmom[3], mtot} */
+void orc_diagnostics(const orc_params* p, int N, const double* r, const double* v,
+                     const double* m, const double* u, const double* gpot, double* out);
+
+/* ---- full step driver (SphSimulation.cpp:204-880 with global timesteps) ---- */
+typedef struct orc_sim orc_sim;
+orc_sim* orc_sim_create(const orc_params* p, int N, const double* r, const double* v,
+                        const double* m, const double* u);
+void orc_sim_destroy(orc_sim* s);
+int orc_sim_setup(orc_sim* s);
+int orc_sim_step(orc_sim* s);
+/* field names: r v a atree m h rho u dudt div_v gpot invomega zeta sound pressure hfactor */
+int orc_sim_get(orc_sim* s, const char* field, double* out);
+double orc_sim_time(orc_sim* s);
+double orc_sim_timestep(orc_sim* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

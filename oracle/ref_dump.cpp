@@ -1,0 +1,110 @@
+// Fixture dump driver for the reference GANDALF build (test infrastructure only).
+//
+// Links against the reference's own objects (oracle/Makefile.ref) and drives them through
+// the public SimulationBase API exactly as src/Common/gandalf.cpp:40-140 does:
+// Parameters::ReadParamsFile -> SimulationBase::SimulationFactory -> SetupSimulation ->
+// MainLoop.  After setup and after each requested step it writes every per-particle field
+// the GPU path must reproduce, at full precision, in a flat binary format:
+//
+//   int32 N, int32 nfields, then nfields arrays of N doubles, in FIELD_NAMES order.
+//
+// usage: ref_dump <paramfile> <nsteps> <outprefix> [dump_every]
+//   writes <outprefix>_s<step>.bin (step 0 = after setup) and <outprefix>_diag.txt
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+#include "Parameters.h"
+#include "Simulation.h"
+#include "Exception.h"
+#include "Particle.h"
+
+static const char* FIELD_NAMES[] = {
+  "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "atx", "aty", "atz",
+  "m", "h", "rho", "u", "dudt", "div_v", "gpot", "invomega", "zeta", "sound",
+  "pressure", "hfactor", "iorig", "levelneib", "dt"
+};
+static const int NFIELDS = sizeof(FIELD_NAMES)/sizeof(FIELD_NAMES[0]);
+
+static void dump(Simulation<3>* sim, const std::string& fname)
+{
+  Hydrodynamics<3>* hydro = sim->hydro;
+  const int N = hydro->Nhydro;
+  std::vector<double> buf((size_t) N*NFIELDS);
+  for (int i=0; i<N; i++) {
+    GradhSphParticle<3>& p = static_cast<GradhSphParticle<3>&>(hydro->GetParticlePointer(i));
+    double* f = &buf[0];
+    int k = 0;
+    for (int d=0; d<3; d++) f[(size_t)(k++)*N + i] = p.r[d];
+    for (int d=0; d<3; d++) f[(size_t)(k++)*N + i] = p.v[d];
+    for (int d=0; d<3; d++) f[(size_t)(k++)*N + i] = p.a[d];
+    for (int d=0; d<3; d++) f[(size_t)(k++)*N + i] = p.atree[d];
+    f[(size_t)(k++)*N + i] = p.m;
+    f[(size_t)(k++)*N + i] = p.h;
+    f[(size_t)(k++)*N + i] = p.rho;
+    f[(size_t)(k++)*N + i] = p.u;
+    f[(size_t)(k++)*N + i] = p.dudt;
+    f[(size_t)(k++)*N + i] = p.div_v;
+    f[(size_t)(k++)*N + i] = p.gpot;
+    f[(size_t)(k++)*N + i] = p.invomega;
+    f[(size_t)(k++)*N + i] = p.zeta;
+    f[(size_t)(k++)*N + i] = p.sound;
+    f[(size_t)(k++)*N + i] = p.pressure;
+    f[(size_t)(k++)*N + i] = p.hfactor;
+    f[(size_t)(k++)*N + i] = p.iorig;
+    f[(size_t)(k++)*N + i] = p.levelneib;
+    f[(size_t)(k++)*N + i] = p.dt;
+  }
+  FILE* fp = fopen(fname.c_str(), "wb");
+  if (!fp) { perror(fname.c_str()); exit(2); }
+  int hdr[2] = {N, NFIELDS};
+  fwrite(hdr, sizeof(int), 2, fp);
+  fwrite(&buf[0], sizeof(double), buf.size(), fp);
+  fclose(fp);
+}
+
+static void diag_line(Simulation<3>* sim, FILE* fp)
+{
+  sim->CalculateDiagnostics();
+  const Diagnostics<3>& d = sim->diag;
+  fprintf(fp, "%d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+          sim->Nsteps, (double) sim->t, (double) sim->timestep, d.Etot, d.ketot, d.utot, d.gpetot,
+          d.mom[0], d.mom[1], d.mom[2], d.angmom[0], d.angmom[1], d.angmom[2], d.mtot);
+  fflush(fp);
+}
+
+int main(int argc, char** argv)
+{
+  if (argc < 4) {
+    fprintf(stderr, "usage: %s <paramfile> <nsteps> <outprefix> [dump_every]\n", argv[0]);
+    return 1;
+  }
+  const std::string paramfile = argv[1];
+  const int nsteps = atoi(argv[2]);
+  const std::string prefix = argv[3];
+  const int every = argc > 4 ? atoi(argv[4]) : nsteps;
+
+  ExceptionHandler::makeExceptionHandler(cplusplus);
+  Parameters* params = new Parameters();
+  params->ReadParamsFile(paramfile);
+  SimulationBase* base = SimulationBase::SimulationFactory(params->intparams["ndim"],
+                                                           params->stringparams["sim"], params);
+  base->SetupSimulation();
+  Simulation<3>* sim = static_cast<Simulation<3>*>(base);
+
+  FILE* dfp = fopen((prefix + "_diag.txt").c_str(), "w");
+  fprintf(dfp, "# Nsteps t timestep Etot ketot utot gpetot momx momy momz angx angy angz mtot\n");
+  diag_line(sim, dfp);
+  dump(sim, prefix + "_s0.bin");
+  for (int s=1; s<=nsteps; s++) {
+    sim->MainLoop();
+    diag_line(sim, dfp);
+    if (every > 0 && (s % every == 0 || s == nsteps)) {
+      char name[64];
+      snprintf(name, sizeof(name), "_s%d.bin", s);
+      dump(sim, prefix + name);
+    }
+  }
+  fclose(dfp);
+  return 0;
+}

@@ -1,0 +1,100 @@
+"""Regenerate the golden fixtures from the compiled reference (oracle/_ref/ref_dump).
+
+Run in the build container (where /root/reference exists):
+    make -f oracle/Makefile.ref -j8 && python tests/golden/make_fixtures.py
+
+Each case writes seeded inputs with gandalf_amd.ic, runs the reference through
+SimulationFactory -> SetupSimulation -> MainLoop (oracle/ref_dump.cpp) and stores the inputs,
+the per-particle fields after setup (s0) and after `nsteps` steps, and the reference's
+diagnostics per step, in tests/golden/<case>.npz.  Provenance (parameters, seed, build mode)
+is stored in the same file.
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from gandalf_amd import ic  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "atx", "aty", "atz", "m", "h",
+          "rho", "u", "dudt", "div_v", "gpot", "invomega", "zeta", "sound", "pressure",
+          "hfactor", "iorig", "levelneib", "dt"]
+KEEP = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "atx", "aty", "atz", "h", "rho",
+        "u", "dudt", "div_v", "gpot", "invomega", "zeta", "sound", "pressure", "hfactor"]
+
+
+def _load(fn):
+    b = open(fn, "rb").read()
+    N, nf = np.frombuffer(b[:8], np.int32)
+    a = np.frombuffer(b[8:], np.float64).reshape(nf, N)
+    return {FIELDS[k]: a[k] for k in range(nf)}
+
+
+CASES = {
+    # hydro only, periodic jittered lattice, analytic M4
+    "box16_m4": (lambda: ic.jittered_lattice_box(n_side=16, seed=11), {}, 10),
+    # same, reference default tabulated kernel
+    "box16_m4_tab": (lambda: ic.jittered_lattice_box(n_side=16, seed=12),
+                     {"tabulated_kernel": 1}, 10),
+    # self-gravitating cold sphere, brute-force "tree" = exact kernel-softened direct sum
+    "sphere4k_m4_bf": (lambda: ic.cold_sphere(N=4096, seed=13),
+                       {"neib_search": "bruteforce"}, 3),
+    # same inputs through the reference KD-tree (tree-error statistics)
+    "sphere4k_m4_kd": (lambda: ic.cold_sphere(N=4096, seed=13), {}, 0),
+    # rotating m=2 cloud, quintic kernel, barotropic EOS, brute-force gravity
+    "cloud4k_quintic_bf": (lambda: ic.rotating_m2_cloud(N=4096, seed=14, rho_bary=1.0),
+                           {"neib_search": "bruteforce"}, 3),
+}
+
+
+def run_case(name, gen, over, nsteps, outdir):
+    g = gen()
+    prm = dict(g["params"])
+    prm.update(over)
+    with tempfile.TemporaryDirectory() as td:
+        col = os.path.join(td, "in.col")
+        dat = os.path.join(td, "in.dat")
+        ic.write_column_file(col, g)
+        ic.write_param_file(dat, prm, col, "FIX")
+        steps = max(nsteps, 0)
+        subprocess.run([REF, dat, str(steps), os.path.join(td, "out"), str(max(steps, 1))],
+                       cwd=td, check=True, stdout=subprocess.DEVNULL)
+        s0 = _load(os.path.join(td, "out_s0.bin"))
+        arrs = {"in_r": g["r"], "in_v": g["v"], "in_m": g["m"], "in_u": g["u"]}
+        for k in KEEP:
+            arrs[f"s0_{k}"] = s0[k]
+        if nsteps > 0:
+            sN = _load(os.path.join(td, f"out_s{nsteps}.bin"))
+            for k in KEEP:
+                arrs[f"s{nsteps}_{k}"] = sN[k]
+        diag = np.loadtxt(os.path.join(td, "out_diag.txt"))
+        arrs["diag"] = np.atleast_2d(diag)
+    meta = {"params": prm, "nsteps": nsteps, "build": "serial, COMPILER_MODE=STANDARD, "
+            "PRECISION=DOUBLE, DEBUG_LEVEL=1 (oracle/Makefile.ref)",
+            "diag_columns": "Nsteps t timestep Etot ketot utot gpetot momx momy momz angx "
+                            "angy angz mtot"}
+    arrs["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(outdir, f"{name}.npz"), **arrs)
+    print(name, "ok", g["r"].shape[0], "particles")
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("reference dump driver not built: make -f oracle/Makefile.ref -j8")
+    which = sys.argv[1:] or list(CASES)
+    for name in which:
+        gen, over, nsteps = CASES[name]
+        run_case(name, gen, over, nsteps, HERE)
+
+
+if __name__ == "__main__":
+    main()

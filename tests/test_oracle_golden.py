@@ -1,0 +1,91 @@
+"""The CPU oracle pinned against the compiled reference's own outputs (golden fixtures),
+closed forms of the kernels and brute-force neighbour sets.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import load_golden, vec
+
+CASES_HYDRO = ["box16_m4", "box16_m4_tab"]
+
+
+def _setup_and_run(d):
+    p = O.params_from_dict(d["meta"]["params"])
+    s = O.Sim(p, d["in_r"], d["in_v"], d["in_m"], d["in_u"])
+    s.setup()
+    return p, s
+
+
+def relmax(a, b):
+    scale = np.max(np.abs(b))
+    return np.max(np.abs(a - b)) / scale
+
+
+@pytest.mark.parametrize("case", CASES_HYDRO + ["sphere4k_m4_bf", "cloud4k_quintic_bf"])
+def test_setup_h_rho_matches_reference(case):
+    """grad-h h-rho after the reference's three setup passes (SphSimulation.cpp:204-565)."""
+    d = load_golden(case)
+    p, s = _setup_and_run(d)
+    for f in ("h", "rho", "invomega", "zeta", "sound", "pressure"):
+        assert relmax(s.get(f), d[f"s0_{f}"]) < 1e-12, f
+
+
+@pytest.mark.parametrize("case", CASES_HYDRO)
+def test_hydro_forces_and_trajectory_match_reference(case):
+    d = load_golden(case)
+    p, s = _setup_and_run(d)
+    assert relmax(s.get("a"), vec(d, "s0", ("ax", "ay", "az"))) < 1e-12
+    assert relmax(s.get("dudt"), d["s0_dudt"]) < 1e-10
+    assert s.timestep == pytest.approx(d["diag"][0, 2], rel=1e-13)
+    n = d["meta"]["nsteps"]
+    for _ in range(n):
+        s.step()
+    assert np.max(np.abs(s.get("r") - vec(d, f"s{n}"))) < 1e-12
+    assert relmax(s.get("v"), vec(d, f"s{n}", ("vx", "vy", "vz"))) < 1e-10
+    assert relmax(s.get("u"), d[f"s{n}_u"]) < 1e-12
+    assert s.t == pytest.approx(d["diag"][n, 1], rel=1e-13)
+
+
+@pytest.mark.parametrize("case", ["sphere4k_m4_bf", "cloud4k_quintic_bf"])
+def test_direct_gravity_matches_reference_bruteforce(case):
+    """neib_search=bruteforce makes the reference's walk an exact direct sum
+    (BruteForceTree.cpp:214), pinning the kernel-softened pair terms and self potential."""
+    d = load_golden(case)
+    p = O.params_from_dict(d["meta"]["params"])
+    st = {k: d[f"s0_{k}"] for k in ("h", "zeta", "hfactor")}
+    at, gp = O.direct_gravity(p, vec(d, "s0"), d["in_m"], st)
+    assert relmax(at, vec(d, "s0", ("atx", "aty", "atz"))) < 1e-12
+    assert relmax(gp, d["s0_gpot"]) < 1e-12
+    a, dudt, div_v = O.hydro_forces(p, 1, vec(d, "s0"), vec(d, "s0", ("vx", "vy", "vz")),
+                                    d["in_m"], {k: d[f"s0_{k}"] for k in
+                                                ("h", "rho", "invomega", "pressure", "sound",
+                                                 "hfactor")})
+    assert relmax(a + at, vec(d, "s0", ("ax", "ay", "az"))) < 1e-12
+
+
+def test_tree_twin_error_below_reference_tree_error():
+    """Tree gravity of the device-walk twin vs the exact sum, against the reference
+    KD-tree's own error at the same opening angle (treeerror.py:21-34 metric)."""
+    bf = load_golden("sphere4k_m4_bf")
+    kd = load_golden("sphere4k_m4_kd")
+    p = O.params_from_dict(bf["meta"]["params"])
+    r = vec(bf, "s0")
+    st = {k: bf[f"s0_{k}"] for k in ("h", "zeta", "hfactor")}
+    exact = vec(bf, "s0", ("atx", "aty", "atz"))
+    tw = O.TreeTwin(p, r, bf["in_m"], st["h"])
+    at, gp, nc, nn = tw.gravity(r, bf["in_m"], st)
+    e_twin = np.linalg.norm(at - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    e_ref = np.linalg.norm(vec(kd, "s0", ("atx", "aty", "atz")) - exact, axis=1) / \
+        np.linalg.norm(exact, axis=1)
+    assert np.sqrt(np.mean(e_twin ** 2)) < np.sqrt(np.mean(e_ref ** 2))
+    assert np.max(e_twin) < np.max(e_ref)
+
+
+def test_reference_energy_bookkeeping():
+    """Diagnostics restated from SimAnalysis.hpp:52-185 reproduce the reference's Etot."""
+    d = load_golden("sphere4k_m4_bf")
+    p = O.params_from_dict(d["meta"]["params"])
+    out = O.diagnostics(p, vec(d, "s0"), vec(d, "s0", ("vx", "vy", "vz")), d["in_m"],
+                        d["s0_u"], d["s0_gpot"])
+    assert out[0] == pytest.approx(d["diag"][0, 3], rel=1e-12)
+    assert out[3] == pytest.approx(d["diag"][0, 6], rel=1e-12)
