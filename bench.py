@@ -1,0 +1,153 @@
+"""Benchmark: SPH + self-gravity particle-steps/s on one MI355X.
+
+Workload: the rotating m=2-perturbed cloud of 1,048,576 SPH particles (uniform radius-1 sphere,
+rho = rho0 (1 + 0.1 cos 2 phi), solid-body rotation E_rot/|W| = 0.16, thermal ratio 0.26,
+barotropic EOS, quintic kernel, self-gravity with quadrupole cells, opening angle 0.5), seeded
+synthetic initial conditions, global timesteps.  A "step" is one full MainLoop: kick-drift,
+tree build, h-rho iteration, hydro forces, gravity walk, timestep reduction, end-of-step kick.
+
+Both precision paths are timed: the FP32 mixed path (the headline value) and the plain FP64
+path.  Inputs are resident on the device before the timed region; the timed region brackets
+exactly --steps steps with device synchronisation on both sides.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}   # MI355X dense vector peaks (FLOP/s)
+# algorithmic work per interaction (frozen accounting): density pair 45, hydro pair 75 (the
+# gravity part of a neighbour pair is charged as a direct term), direct/monopole 22,
+# quadrupole particle-cell 70
+W_DENS, W_HYDRO, W_PP, W_PC = 45, 75, 22, 70
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_path(ic, precision, steps, warmup):
+    from gandalf_amd.gpu import GpuSim
+    s = GpuSim(ic["params"], precision=precision)
+    s.set_particles(ic["r"], ic["v"], ic["m"], ic["u"])
+    t0 = time.perf_counter()
+    s.setup()
+    log(f"[{precision}] setup {time.perf_counter() - t0:.2f} s")
+    for w in range(warmup):
+        s.step(1)
+    s.set_timing(True)
+    passes = {k: 0.0 for k in ("tree", "density", "hydro", "gravity", "integrate")}
+    counts = None
+    per_step = []
+    t_start = time.perf_counter()
+    for k in range(steps):
+        ts = time.perf_counter()
+        s.step(1)                      # returns after the step's final device sync
+        per_step.append(time.perf_counter() - ts)
+        pt = s.pass_times()
+        for key in passes:
+            passes[key] += pt[key]
+        c = s.interaction_counts()
+        counts = c if counts is None else {k2: counts[k2] + c[k2] for k2 in c}
+    wall = time.perf_counter() - t_start
+    t, dt, n = s.time()
+    diag = s.diagnostics()
+    s.close()
+    N = ic["r"].shape[0]
+    return dict(N=N, wall=wall, steps=steps, per_step=per_step, passes=passes, counts=counts,
+                t=t, dt=dt, E=float(diag[0]), precision=precision)
+
+
+def roofline(res, dtype):
+    c, p = res["counts"], res["passes"]
+    flops = W_PP * c["grav_pp"] + W_PC * c["grav_pc"]
+    ms = p["gravity"]
+    achieved = flops / (ms * 1e-3) if ms > 0 else 0.0
+    peak = PEAK[dtype]
+    return {"kernel": "k_gravity", "bound": "valu", "achieved": achieved / 1e12,
+            "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": None,
+            "flop_per_step": flops / res["steps"],
+            "kernel_ms_per_step": ms / res["steps"]}
+
+
+def cpu_baseline(nsample=16384, steps=1):
+    """The CPU oracle (OpenMP restatement of the reference path) on a bounded sample of the
+    same workload: the same generator at nsample particles, one step after setup."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from gandalf_amd import ic as ICS
+    g = ICS.rotating_m2_cloud(N=nsample, seed=3, rho_bary=10.0)
+    p = O.params_from_dict(g["params"])
+    s = O.Sim(p, g["r"], g["v"], g["m"], g["u"])
+    s.setup()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nsample * steps / dt, "unit": "particle-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"rotating m=2 cloud at N={nsample}, {steps} step(s) after setup, "
+                      f"oracle/gdf_oracle.c OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU path is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    from gandalf_amd import ic as ICS
+    log(f"generating rotating m=2 cloud N={args.n}")
+    g = ICS.rotating_m2_cloud(N=args.n, seed=3, rho_bary=10.0)
+    mixed = run_path(g, "mixed", args.steps, args.warmup)
+    log(f"[mixed] {mixed['wall']:.3f} s for {args.steps} steps; passes(ms) {mixed['passes']}")
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = run_path(g, "fp64", args.steps, args.warmup)
+        log(f"[fp64] {fp64['wall']:.3f} s for {args.steps} steps; passes(ms) {fp64['passes']}")
+    N = args.n
+    value = N * args.steps / mixed["wall"]
+    out = {
+        "metric": "SPH+grav particle-steps/s at 1M (FP32 vs FP64); ensemble p50/p99 step latency",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * mixed["wall"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "rotating m=2 perturbed cloud, quintic kernel, barotropic EOS, "
+                               "self-gravity theta=0.5 quadrupole, global timesteps",
+                   "n_particles": N, "precision": "fp32 pair arithmetic + fp64 integration"},
+        "roofline": roofline(mixed, "fp32"),
+        "passes_ms_per_step": {k: v / args.steps for k, v in mixed["passes"].items()},
+        "interactions_per_step": {k: v // args.steps for k, v in mixed["counts"].items()},
+    }
+    if fp64 is not None:
+        out["fp64"] = {"value": N * args.steps / fp64["wall"],
+                       "ms_per_step": 1e3 * fp64["wall"] / args.steps,
+                       "roofline": roofline(fp64, "fp64"),
+                       "passes_ms_per_step": {k: v / args.steps for k, v in fp64["passes"].items()}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,533 @@
+// C ABI of libgandalf_gpu: context lifetime, particle import/export and the hot-path entry
+// points declared in include/gandalf_gpu.h.  No C++ exception crosses the boundary; errors are
+// status codes plus gdf_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "gdf_internal.h"
+#include "gdf_device.h"
+
+namespace gdf {
+
+int fail(gdf_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int check_hip(gdf_ctx* c, hipError_t e, const char* what) {
+  if (c) c->err = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+  return e == hipErrorOutOfMemory ? GDF_ERR_NOMEM : GDF_ERR_HIP;
+}
+
+namespace {
+
+constexpr int BS = 256;
+
+__global__ void k_scatter_out(int N, int w, const double* f0, const double* f1, const double* f2,
+                              const int* __restrict__ iorig, double* __restrict__ out) {
+  const int s = blockIdx.x*BS + threadIdx.x;
+  if (s >= N) return;
+  const int o = iorig[s];
+  out[(size_t) o*w] = f0[s];
+  if (w == 3) { out[(size_t) o*3 + 1] = f1[s]; out[(size_t) o*3 + 2] = f2[s]; }
+}
+
+__global__ void k_gather_in(int N, int w, double* f0, double* f1, double* f2,
+                            const int* __restrict__ iorig, const double* __restrict__ in) {
+  const int s = blockIdx.x*BS + threadIdx.x;
+  if (s >= N) return;
+  const int o = iorig[s];
+  f0[s] = in[(size_t) o*w];
+  if (w == 3) { f1[s] = in[(size_t) o*3 + 1]; f2[s] = in[(size_t) o*3 + 2]; }
+}
+
+__global__ void k_iota(int N, int* p) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i < N) p[i] = i;
+}
+
+// resolve a field name to up to three device arrays
+bool field_ptrs(gdf_ctx* c, const char* name, double** p, int* w) {
+  struct E { const char* n; int src; int f; int w; };
+  static const E tab[] = {
+    {"r", 0, F_X, 3}, {"v", 0, F_VX, 3}, {"r0", 0, F_X0, 3}, {"v0", 0, F_VX0, 3},
+    {"a0", 0, F_AX0, 3}, {"m", 0, F_M, 1}, {"h", 0, F_H, 1}, {"u", 0, F_U, 1},
+    {"u0", 0, F_U0, 1}, {"dudt0", 0, F_DUDT0, 1},
+    {"a", 1, D_AX, 3}, {"atree", 1, D_ATX, 3}, {"rho", 1, D_RHO, 1}, {"dudt", 1, D_DUDT, 1},
+    {"div_v", 1, D_DIVV, 1}, {"gpot", 1, D_GPOT, 1}, {"invomega", 1, D_INVOMEGA, 1},
+    {"zeta", 1, D_ZETA, 1}, {"sound", 1, D_SOUND, 1}, {"pressure", 1, D_PRESSURE, 1},
+    {"hfactor", 1, D_HFACTOR, 1}, {"dt", 1, D_DT, 1}};
+  for (const E& e : tab) {
+    if (strcmp(e.n, name)) continue;
+    for (int k = 0; k < e.w; k++) p[k] = e.src == 0 ? c->pf[e.f + k] : c->df[e.f + k];
+    *w = e.w;
+    return true;
+  }
+  return false;
+}
+
+void free_particles(gdf_ctx* c) {
+  for (int f = 0; f < F_NPERSIST; f++) { hipFree(c->pf[f]); hipFree(c->pf_alt[f]); c->pf[f] = c->pf_alt[f] = nullptr; }
+  for (int f = 0; f < D_NDERIVED; f++) { hipFree(c->df[f]); c->df[f] = nullptr; }
+  void* ptrs[] = {c->iorig, c->iorig_alt, c->posm, c->hrecA, c->hrecB, c->grec, c->status,
+                  c->keys, c->keys_alt, c->vals, c->vals_alt, c->sort_tmp, c->left, c->right,
+                  c->first, c->last, c->parent, c->visit, c->bbmin, c->bbmax, c->geo_c,
+                  c->geo_h, c->mp, c->gstart, c->grp_first, c->grp_count, c->grp_node, c->nsel,
+                  c->sel_tmp, c->list_g, c->nn_g, c->list_s, c->nn_s, c->stacks, c->grav_nc,
+                  c->grav_nn, c->tmpd};
+  for (void* p : ptrs) if (p) hipFree(p);
+  c->iorig = c->iorig_alt = nullptr; c->posm = c->hrecA = c->hrecB = c->grec = nullptr;
+  c->status = nullptr; c->keys = c->keys_alt = nullptr; c->vals = c->vals_alt = nullptr;
+  c->sort_tmp = nullptr; c->left = c->right = c->first = c->last = c->parent = c->visit = nullptr;
+  c->bbmin = c->bbmax = nullptr; c->geo_c = c->geo_h = nullptr; c->mp = nullptr;
+  c->gstart = c->grp_first = c->grp_count = c->grp_node = c->nsel = nullptr; c->sel_tmp = nullptr;
+  c->list_g = c->nn_g = c->list_s = c->nn_s = c->stacks = nullptr;
+  c->grav_nc = c->grav_nn = nullptr; c->tmpd = nullptr;
+  c->Ncap = 0; c->N = 0;
+}
+
+int alloc_particles(gdf_ctx* c, int N) {
+  free_particles(c);
+  c->Ncap = N;
+  const size_t n = (size_t) N;
+  for (int f = 0; f < F_NPERSIST; f++) {
+    GDF_HIP(c, hipMalloc(&c->pf[f], sizeof(double)*n));
+    GDF_HIP(c, hipMalloc(&c->pf_alt[f], sizeof(double)*n));
+  }
+  for (int f = 0; f < D_NDERIVED; f++) GDF_HIP(c, hipMalloc(&c->df[f], sizeof(double)*n));
+  GDF_HIP(c, hipMalloc(&c->iorig, sizeof(int)*n));
+  GDF_HIP(c, hipMalloc(&c->iorig_alt, sizeof(int)*n));
+  GDF_HIP(c, hipMalloc(&c->posm, sizeof(double4)*n));
+  GDF_HIP(c, hipMalloc(&c->grec, sizeof(double4)*n));
+  GDF_HIP(c, hipMalloc(&c->grav_nc, sizeof(int32_t)*n));
+  GDF_HIP(c, hipMalloc(&c->grav_nn, sizeof(int32_t)*n));
+  GDF_HIP(c, hipMalloc(&c->tmpd, sizeof(double)*3*n));
+  GDF_TRY(tree_alloc(c));
+  GDF_TRY(sph_alloc_lists(c));
+  return GDF_OK;
+}
+
+int upload_kernel_tables(gdf_ctx* c) {
+  // SmoothingKernel.h:583-604 tables: 6 linear in s, 3 in s^2; values from the oracle-free
+  // host evaluation of the same polynomials (evaluated here in double)
+  const int res = 1000;
+  KernelDesc& k = c->phys.kern;
+  std::vector<double> t(9*res);
+  auto eval = [&](int w, double s) -> double {
+    const bool m4 = k.type == 0;
+    const double nrm = k.norm;
+    // host copies of the device polynomials
+    if (m4) {
+      switch (w) {
+        case 0: return s < 1 ? nrm*(1.0 - 1.5*s*s + 0.75*s*s*s) : (s < 2 ? 0.25*nrm*pow(2.0 - s, 3) : 0.0);
+        case 1: return s < 1 ? nrm*(-3.0*s + 2.25*s*s) : (s < 2 ? -0.75*nrm*(2.0 - s)*(2.0 - s) : 0.0);
+        case 2: return s < 1 ? nrm*(-3 + 1.5*5.0*s*s - 0.75*6.0*pow(s, 3))
+                             : (s < 2 ? nrm*(-6.0 + 3.0*4.0*s - 1.5*5.0*s*s + 0.25*6.0*pow(s, 3)) : 0.0);
+        case 3: return s < 1 ? 1.4 - 2.0*s*s + 1.5*pow(s, 4) - 0.6*pow(s, 5)
+                             : (s < 2 ? 1.6 - 4.0*s*s + 4.0*pow(s, 3) - 1.5*pow(s, 4) + 0.2*pow(s, 5) : 0.0);
+        case 4: return s < 1 ? 1.333333333333333333333*s - 1.2*pow(s, 3) + 0.5*pow(s, 4)
+                             : (s < 2 ? 2.6666666666666666667*s - 3.0*s*s + 1.2*pow(s, 3) -
+                                        0.166666666666666666667*pow(s, 4) - 0.06666666666666666667/(s*s)
+                                      : 1.0/(s*s));
+        default: return s < 1 ? 1.4 - 0.666666666666666666666666*s*s + 0.3*pow(s, 4) - 0.1*pow(s, 5)
+                              : (s < 2 ? -1.0/(15.0*s) + 1.6 - 1.33333333333333333333333333*s*s + pow(s, 3) -
+                                         0.3*pow(s, 4) + (1.0/30.0)*pow(s, 5)
+                                       : 1.0/s);
+      }
+    }
+    const double c12 = 12.0/359.0;
+    switch (w) {
+      case 0: return s < 1 ? nrm*(66.0 - 60.0*s*s + 30.0*pow(s, 4) - 10.0*pow(s, 5))
+                   : s < 2 ? nrm*(51.0 + 75.0*s - 210.0*s*s + 150.0*pow(s, 3) - 45.0*pow(s, 4) + 5.0*pow(s, 5))
+                   : s < 3 ? nrm*(243.0 - 405*s + 270.0*s*s - 90.0*pow(s, 3) + 15.0*pow(s, 4) - pow(s, 5)) : 0.0;
+      case 1: return s < 1 ? nrm*(-120.0*s + 120.0*pow(s, 3) - 50.0*pow(s, 4))
+                   : s < 2 ? nrm*(75.0 - 420.0*s + 450.0*s*s - 180.0*pow(s, 3) + 25.0*pow(s, 4))
+                   : s < 3 ? nrm*(-405.0 + 540.0*s - 270.0*s*s + 60.0*pow(s, 3) - 5.0*pow(s, 4)) : 0.0;
+      case 2: return s < 1 ? nrm*(-198.0 + 60.0*5.0*s*s - 30.0*7.0*pow(s, 4) + 10.0*8.0*pow(s, 5))
+                   : s < 2 ? nrm*(-153.0 - 75.0*4.0*s + 210.0*5.0*s*s - 150.0*6.0*pow(s, 3) + 45.0*7.0*pow(s, 4) - 5.0*8.0*pow(s, 5))
+                   : s < 3 ? nrm*(-729.0 + 405.0*4.0*s - 270.0*5.0*s*s + 90.0*6.0*pow(s, 3) - 15.0*7.0*pow(s, 4) + 8.0*pow(s, 5)) : 0.0;
+      case 3: return s < 1 ? 33.0*s*s - 15.0*pow(s, 4) + 5.0*pow(s, 6) - 1.42857142857*pow(s, 7) - 34.14285714
+                   : s < 2 ? 25.5*s*s + 25.0*pow(s, 3) - 52.5*pow(s, 4) + 30.0*pow(s, 5) - 7.5*pow(s, 6) + 0.7142857143*pow(s, 7) - 33.785714286
+                   : s < 3 ? 121.5*s*s - 135.0*pow(s, 3) + 67.5*pow(s, 4) - 18.0*pow(s, 5) + 2.5*pow(s, 6) - 0.142857143*pow(s, 7) - 52.07142857 : 0.0;
+      case 4: return s < 1 ? c12*(22.0*s - 12.0*pow(s, 3) + (30.0/7.0)*pow(s, 5) - (5.0/4.0)*pow(s, 6))
+                   : s < 2 ? c12*(17.0*s + (75.0/4.0)*s*s - 42.0*pow(s, 3) + 25.0*pow(s, 4) - (45.0/7.0)*pow(s, 5) + (5.0/8.0)*pow(s, 6) + (5.0/56.0)/(s*s))
+                   : s < 3 ? c12*(81.0*s - (405.0/4.0)*pow(s, 2) + 54.0*pow(s, 3) - 15.0*pow(s, 4) + (15.0/7.0)*pow(s, 5) - (1.0/8.0)*pow(s, 6) - (507.0/56.0)/(s*s))
+                   : 1.0/(s*s);
+      default: return s < 1 ? c12*(-11.0*s*s + 3.0*pow(s, 4) - (5.0/7.0)*pow(s, 6) + (5.0/28.0)*pow(s, 7) + (478.0/14.0))
+                    : s < 2 ? c12*(-(17.0/2.0)*s*s - (25.0/4.0)*pow(s, 3) + (21.0/2.0)*pow(s, 4) - 5.0*pow(s, 5) + (15.0/14.0)*pow(s, 6) - (5.0/56.0)*pow(s, 7) + (473.0/14.0) + (5.0/56.0)/s)
+                    : s < 3 ? c12*(-(81.0/2.0)*s*s + (135.0/4.0)*pow(s, 3) - (27.0/2.0)*pow(s, 4) + 3.0*pow(s, 5) - (5.0/14.0)*pow(s, 6) + (1.0/56.0)*pow(s, 7) + (729.0/14.0) - (507.0/56.0)/s)
+                    : 1.0/s;
+    }
+  };
+  const double step = k.range/res, stepsq = k.rangesqd/res;
+  for (int w = 0; w < 6; w++) for (int i = 0; i < res; i++) t[w*res + i] = eval(w, step*i);
+  const int sqw[3] = {0, 2, 3};
+  for (int w = 0; w < 3; w++) for (int i = 0; i < res; i++) t[(6 + w)*res + i] = eval(sqw[w], sqrt(stepsq*i));
+  std::vector<float> tf(t.begin(), t.end());
+  GDF_HIP(c, hipMalloc(&c->ktab, sizeof(double)*9*res));
+  GDF_HIP(c, hipMalloc(&c->ktabf, sizeof(float)*9*res));
+  GDF_HIP(c, hipMemcpy(c->ktab, t.data(), sizeof(double)*9*res, hipMemcpyHostToDevice));
+  GDF_HIP(c, hipMemcpy(c->ktabf, tf.data(), sizeof(float)*9*res, hipMemcpyHostToDevice));
+  k.tab = c->ktab; k.tabf = c->ktabf; k.res = res;
+  return GDF_OK;
+}
+
+}  // namespace
+}  // namespace gdf
+
+using namespace gdf;
+
+extern "C" {
+
+int gdf_config_default(gdf_config* cfg) {
+  if (!cfg) return GDF_ERR_INVALID;
+  memset(cfg, 0, sizeof(*cfg));
+  cfg->kernel = GDF_KERNEL_M4;
+  cfg->tabulated_kernel = 1;
+  cfg->h_fac = 1.2; cfg->h_converge = 0.01;
+  cfg->gas_eos = GDF_EOS_ENERGY_EQN;
+  cfg->gamma_eos = 1.66666666666666; cfg->temp0 = 1.0; cfg->mu_bar = 1.0; cfg->rho_bary = 1.0e-14;
+  cfg->alpha_visc = 1.0; cfg->beta_visc = 2.0;
+  cfg->hydro_forces = 1; cfg->self_gravity = 0;
+  cfg->thetamaxsqd = 0.1; cfg->Nleafmax = 6; cfg->periodic = 0;
+  for (int k = 0; k < 3; k++) { cfg->boxmin[k] = -9.9e30; cfg->boxmax[k] = 9.9e30; }
+  cfg->courant_mult = 0.15; cfg->accel_mult = 0.3; cfg->energy_mult = 0.4;
+  cfg->precision = GDF_PREC_FP64;
+  cfg->group_max = 64;
+  return GDF_OK;
+}
+
+int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
+  if (!cfg || !out) return GDF_ERR_INVALID;
+  *out = nullptr;
+  if (cfg->kernel != GDF_KERNEL_M4 && cfg->kernel != GDF_KERNEL_QUINTIC) return GDF_ERR_INVALID;
+  if (cfg->gas_eos < 0 || cfg->gas_eos > 2) return GDF_ERR_INVALID;
+  if (cfg->group_max < 1 || cfg->group_max > kMaxGroup) return GDF_ERR_INVALID;
+  if (cfg->Nleafmax < 1 || cfg->Nleafmax > 14) return GDF_ERR_INVALID;
+  if (!(cfg->thetamaxsqd > 0.0) || !(cfg->h_fac > 0.0)) return GDF_ERR_INVALID;
+  if (cfg->periodic)
+    for (int k = 0; k < 3; k++)
+      if (!(cfg->boxmax[k] > cfg->boxmin[k]) || cfg->boxmax[k] - cfg->boxmin[k] > 1e20)
+        return GDF_ERR_INVALID;
+  gdf_ctx* c = new (std::nothrow) gdf_ctx();
+  if (!c) return GDF_ERR_NOMEM;
+  c->cfg = *cfg;
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return GDF_ERR_HIP;
+  }
+  PhysParams& p = c->phys;
+  KernelDesc& k = p.kern;
+  k.type = cfg->kernel; k.tabulated = cfg->tabulated_kernel;
+  if (k.type == 0) { k.range = 2.0; k.rangesqd = 4.0; k.norm = 0.31830988618379; }
+  else { k.range = 3.0; k.rangesqd = 9.0; k.norm = 0.31830988618379*(1/120.); }
+  k.res = 1000; k.resinv = 1000/k.range; k.resinvsqd = 1000/k.rangesqd;
+  p.h_fac = cfg->h_fac; p.h_converge = cfg->h_converge;
+  p.gamma = cfg->gamma_eos; p.gammam1 = cfg->gamma_eos - 1.0;
+  p.temp0 = cfg->temp0; p.mu_bar = cfg->mu_bar; p.invrho_bary = 1.0/cfg->rho_bary;
+  p.eos = cfg->gas_eos; p.energy_integration = cfg->gas_eos == GDF_EOS_ENERGY_EQN;
+  p.alpha_visc = cfg->alpha_visc; p.beta_visc = cfg->beta_visc;
+  p.periodic = cfg->periodic;
+  for (int d = 0; d < 3; d++) {
+    p.boxmin[d] = cfg->boxmin[d]; p.boxmax[d] = cfg->boxmax[d];
+    p.boxsize[d] = cfg->boxmax[d] - cfg->boxmin[d];
+    p.boxhalf[d] = 0.5*p.boxsize[d];
+  }
+  p.thetamaxsqd = cfg->thetamaxsqd; p.nleaf = cfg->Nleafmax;
+  p.courant_mult = cfg->courant_mult; p.accel_mult = cfg->accel_mult;
+  p.energy_mult = cfg->energy_mult;
+  p.hydro_forces = cfg->hydro_forces; p.self_gravity = cfg->self_gravity;
+  int rc = upload_kernel_tables(c);
+  if (rc == GDF_OK) {
+    if (hipMalloc(&c->counters, sizeof(int)*16) != hipSuccess ||
+        hipMalloc(&c->icount, sizeof(long long)*8) != hipSuccess ||
+        hipMalloc(&c->red, sizeof(double)*16384) != hipSuccess ||
+        hipHostMalloc(&c->h_counters, sizeof(int)*16) != hipSuccess ||
+        hipHostMalloc(&c->h_red, sizeof(double)*16384) != hipSuccess)
+      rc = GDF_ERR_NOMEM;
+  }
+  if (rc == GDF_OK) {
+    hipMemset(c->counters, 0, sizeof(int)*16);
+    hipMemset(c->icount, 0, sizeof(long long)*8);
+    for (int e = 0; e < 12; e++) hipEventCreate(&c->ev[e]);
+  }
+  if (rc != GDF_OK) { gdf_destroy(c); return rc; }
+  *out = c;
+  return GDF_OK;
+}
+
+void gdf_destroy(gdf_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_particles(c);
+  if (c->counters) hipFree(c->counters);
+  if (c->icount) hipFree(c->icount);
+  if (c->red) hipFree(c->red);
+  if (c->ktab) hipFree(c->ktab);
+  if (c->ktabf) hipFree(c->ktabf);
+  if (c->h_counters) hipHostFree(c->h_counters);
+  if (c->h_red) hipHostFree(c->h_red);
+  for (int e = 0; e < 12; e++) if (c->ev[e]) hipEventDestroy(c->ev[e]);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* gdf_last_error(const gdf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gdf_num_particles(const gdf_ctx* c) { return c ? c->N : 0; }
+
+int gdf_set_particles(gdf_ctx* c, int N, const double* r, const double* v, const double* m,
+                      const double* h, const double* u) {
+  if (!c) return GDF_ERR_INVALID;
+  if (N <= 0 || !r || !v || !m || !u) return fail(c, GDF_ERR_INVALID, "gdf_set_particles: bad arguments");
+  if ((long long) N > (1LL << 30)) return fail(c, GDF_ERR_INVALID, "too many particles");
+  for (int i = 0; i < N; i++)
+    if (!(m[i] > 0.0)) return fail(c, GDF_ERR_INVALID, "particle masses must be positive");
+  if (N > c->Ncap) GDF_TRY(alloc_particles(c, N));
+  c->N = N;
+  const size_t n = (size_t) N;
+  std::vector<double> col(n);
+  auto up = [&](double* dst, const double* src, int stride, int off) -> int {
+    for (size_t i = 0; i < n; i++) col[i] = src ? src[i*stride + off] : 0.0;
+    GDF_HIP(c, hipMemcpy(dst, col.data(), sizeof(double)*n, hipMemcpyHostToDevice));
+    return GDF_OK;
+  };
+  for (int k = 0; k < 3; k++) {
+    GDF_TRY(up(c->pf[F_X + k], r, 3, k)); GDF_TRY(up(c->pf[F_X0 + k], r, 3, k));
+    GDF_TRY(up(c->pf[F_VX + k], v, 3, k)); GDF_TRY(up(c->pf[F_VX0 + k], v, 3, k));
+    GDF_TRY(up(c->pf[F_AX0 + k], nullptr, 1, 0));
+  }
+  GDF_TRY(up(c->pf[F_M], m, 1, 0));
+  GDF_TRY(up(c->pf[F_H], h, 1, 0));
+  GDF_TRY(up(c->pf[F_U], u, 1, 0));
+  GDF_TRY(up(c->pf[F_U0], u, 1, 0));
+  GDF_TRY(up(c->pf[F_DUDT0], nullptr, 1, 0));
+  for (int f = 0; f < D_NDERIVED; f++) GDF_HIP(c, hipMemset(c->df[f], 0, sizeof(double)*n));
+  k_iota<<<nblocks(N, BS), BS, 0, c->stream>>>(N, c->iorig);
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  c->tree_valid = false; c->sg_valid = false; c->setup_done = false;
+  c->t = c->timestep = c->tlast = 0.0; c->nsteps = 0;
+  return GDF_OK;
+}
+
+int gdf_get_field(gdf_ctx* c, const char* name, double* out) {
+  if (!c || !name || !out) return GDF_ERR_INVALID;
+  if (c->N <= 0) return fail(c, GDF_ERR_INVALID, "no particles");
+  double* p[3] = {nullptr, nullptr, nullptr};
+  int w = 0;
+  if (!field_ptrs(c, name, p, &w)) return fail(c, GDF_ERR_INVALID, std::string("unknown field ") + name);
+  k_scatter_out<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, w, p[0], p[1], p[2], c->iorig, c->tmpd);
+  GDF_HIP(c, hipGetLastError());
+  GDF_HIP(c, hipMemcpyAsync(out, c->tmpd, sizeof(double)*w*(size_t) c->N, hipMemcpyDeviceToHost, c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  return GDF_OK;
+}
+
+int gdf_set_field(gdf_ctx* c, const char* name, const double* in) {
+  if (!c || !name || !in) return GDF_ERR_INVALID;
+  if (c->N <= 0) return fail(c, GDF_ERR_INVALID, "no particles");
+  double* p[3] = {nullptr, nullptr, nullptr};
+  int w = 0;
+  if (!field_ptrs(c, name, p, &w)) return fail(c, GDF_ERR_INVALID, std::string("unknown field ") + name);
+  GDF_HIP(c, hipMemcpyAsync(c->tmpd, in, sizeof(double)*w*(size_t) c->N, hipMemcpyHostToDevice, c->stream));
+  k_gather_in<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, w, p[0], p[1], p[2], c->iorig, c->tmpd);
+  GDF_HIP(c, hipGetLastError());
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  if (!strcmp(name, "r") || !strcmp(name, "m") || !strcmp(name, "h")) c->tree_valid = false;
+  c->sg_valid = false;
+  return GDF_OK;
+}
+
+int gdf_build_tree(gdf_ctx* c) {
+  if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  return tree_build(c);
+}
+
+int gdf_update_all_sph_properties(gdf_ctx* c, int nopbc) {
+  if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  if (!c->tree_valid) GDF_TRY(tree_build(c));
+  return sph_density(c, nopbc);
+}
+
+int gdf_update_all_sph_hydro_forces(gdf_ctx* c) {
+  if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "hydro forces before the density pass");
+  GDF_TRY(integ_zero_forces(c));
+  GDF_TRY(sph_build_sg(c));
+  return sph_hydro(c, 0);
+}
+
+int gdf_update_all_sph_forces(gdf_ctx* c) {
+  if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "forces before the density pass");
+  GDF_TRY(integ_zero_forces(c));
+  if (c->phys.hydro_forces) {
+    GDF_TRY(sph_build_sg(c));
+    GDF_TRY(sph_hydro(c, 1));
+  }
+  GDF_TRY(gravity_walk(c));
+  return integ_add_atree(c);
+}
+
+int gdf_maximum_smoothing_length(gdf_ctx* c, double* hmax) {
+  if (!c || !hmax || c->N <= 0) return GDF_ERR_INVALID;
+  return integ_max_h(c, hmax);
+}
+
+int gdf_advance_particles(gdf_ctx* c, double dt) {
+  if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  return integ_advance(c, dt);
+}
+
+int gdf_check_boundaries(gdf_ctx* c) {
+  if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  return integ_wrap(c);
+}
+
+int gdf_end_timestep(gdf_ctx* c, double dtstep) {
+  if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  GDF_TRY(integ_end_step(c, dtstep));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  return GDF_OK;
+}
+
+int gdf_compute_global_timestep(gdf_ctx* c, double* dt_min) {
+  if (!c || !dt_min || c->N <= 0) return GDF_ERR_INVALID;
+  return integ_timestep(c, dt_min);
+}
+
+int gdf_calculate_diagnostics(gdf_ctx* c, double* out) {
+  if (!c || !out || c->N <= 0) return GDF_ERR_INVALID;
+  return integ_diagnostics(c, out);
+}
+
+int gdf_time(const gdf_ctx* c, double* t, double* timestep, int64_t* nsteps) {
+  if (!c) return GDF_ERR_INVALID;
+  if (t) *t = c->t;
+  if (timestep) *timestep = c->timestep;
+  if (nsteps) *nsteps = c->nsteps;
+  return GDF_OK;
+}
+
+int gdf_interaction_counts(gdf_ctx* c, int64_t* out6) {
+  if (!c || !out6) return GDF_ERR_INVALID;
+  for (int k = 0; k < 6; k++) out6[k] = c->icounts_last[k];
+  return GDF_OK;
+}
+
+int gdf_gravity_group_counts(gdf_ctx* c, int32_t* ncell, int32_t* nnear) {
+  if (!c || c->ngroups <= 0) return GDF_ERR_INVALID;
+  if (ncell) GDF_HIP(c, hipMemcpy(ncell, c->grav_nc, sizeof(int32_t)*c->ngroups, hipMemcpyDeviceToHost));
+  if (nnear) GDF_HIP(c, hipMemcpy(nnear, c->grav_nn, sizeof(int32_t)*c->ngroups, hipMemcpyDeviceToHost));
+  return GDF_OK;
+}
+
+int gdf_pass_times(gdf_ctx* c, double* out5) {
+  if (!c || !out5) return GDF_ERR_INVALID;
+  for (int k = 0; k < 5; k++) out5[k] = c->pass_ms[k];
+  return GDF_OK;
+}
+
+int gdf_set_timing(gdf_ctx* c, int enable) {
+  if (!c) return GDF_ERR_INVALID;
+  c->timing = enable != 0;
+  return GDF_OK;
+}
+
+int gdf_tree_arrays(gdf_ctx* c, int32_t* perm, uint64_t* keys, int32_t* left, int32_t* right,
+                    int32_t* first, int32_t* last, double* mass, double* com, double* quad,
+                    double* bbmin, double* bbmax, double* hmax, int32_t* group_first,
+                    int32_t* group_count, int32_t* group_node, int* ngroups) {
+  if (!c || c->N <= 0 || !c->tree_valid) return GDF_ERR_INVALID;
+  const size_t N = (size_t) c->N, nn = 2*N - 1, ni = N > 1 ? N - 1 : 0;
+  hipStream_t s = c->stream;
+  GDF_HIP(c, hipStreamSynchronize(s));
+  if (perm) GDF_HIP(c, hipMemcpy(perm, c->iorig, sizeof(int)*N, hipMemcpyDeviceToHost));
+  if (keys) GDF_HIP(c, hipMemcpy(keys, c->keys, sizeof(uint64_t)*N, hipMemcpyDeviceToHost));
+  if (ni) {
+    if (left) GDF_HIP(c, hipMemcpy(left, c->left, sizeof(int)*ni, hipMemcpyDeviceToHost));
+    if (right) GDF_HIP(c, hipMemcpy(right, c->right, sizeof(int)*ni, hipMemcpyDeviceToHost));
+    if (first) GDF_HIP(c, hipMemcpy(first, c->first, sizeof(int)*ni, hipMemcpyDeviceToHost));
+    if (last) GDF_HIP(c, hipMemcpy(last, c->last, sizeof(int)*ni, hipMemcpyDeviceToHost));
+  }
+  std::vector<double> mpv(10*nn);
+  if (mass || com || quad) {
+    GDF_HIP(c, hipMemcpy(mpv.data(), c->mp, sizeof(double)*10*nn, hipMemcpyDeviceToHost));
+    for (size_t n = 0; n < nn; n++) {
+      if (mass) mass[n] = mpv[10*n + 3];
+      if (com) for (int k = 0; k < 3; k++) com[3*n + k] = mpv[10*n + k];
+      if (quad) for (int k = 0; k < 5; k++) quad[5*n + k] = mpv[10*n + 4 + k];
+    }
+  }
+  if (bbmin) GDF_HIP(c, hipMemcpy(bbmin, c->bbmin, sizeof(double)*3*nn, hipMemcpyDeviceToHost));
+  if (bbmax) GDF_HIP(c, hipMemcpy(bbmax, c->bbmax, sizeof(double)*3*nn, hipMemcpyDeviceToHost));
+  if (hmax) {
+    std::vector<double4> gc(nn);
+    GDF_HIP(c, hipMemcpy(gc.data(), c->geo_c, sizeof(double4)*nn, hipMemcpyDeviceToHost));
+    for (size_t n = 0; n < nn; n++) hmax[n] = gc[n].w;
+  }
+  const size_t ng = (size_t) c->ngroups;
+  if (group_first) GDF_HIP(c, hipMemcpy(group_first, c->grp_first, sizeof(int)*ng, hipMemcpyDeviceToHost));
+  if (group_count) GDF_HIP(c, hipMemcpy(group_count, c->grp_count, sizeof(int)*ng, hipMemcpyDeviceToHost));
+  if (group_node) GDF_HIP(c, hipMemcpy(group_node, c->grp_node, sizeof(int)*ng, hipMemcpyDeviceToHost));
+  if (ngroups) *ngroups = c->ngroups;
+  return GDF_OK;
+}
+
+int64_t gdf_neighbour_sets(gdf_ctx* c, int mode, int64_t* offsets, int32_t* ids, int64_t cap) {
+  if (!c || !offsets || c->N <= 0 || (mode != 0 && mode != 1)) return GDF_ERR_INVALID;
+  if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "no tree");
+  if (!c->sg_valid) GDF_TRY(sph_build_sg(c));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  const int N = c->N;
+  const int capl = c->cap_s;
+  std::vector<int> nn(N), lst((size_t) capl*N), io(N);
+  std::vector<double> hh(N), x(N), y(N), z(N);
+  GDF_HIP(c, hipMemcpy(nn.data(), c->nn_s, sizeof(int)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(lst.data(), c->list_s, sizeof(int)*(size_t) capl*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(io.data(), c->iorig, sizeof(int)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(hh.data(), c->pf[F_H], sizeof(double)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(x.data(), c->pf[F_X], sizeof(double)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(y.data(), c->pf[F_Y], sizeof(double)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(z.data(), c->pf[F_Z], sizeof(double)*N, hipMemcpyDeviceToHost));
+  // the scatter-gather list holds exactly the mode-1 set; the gather set is its subset with
+  // r2 < k^2 h_i^2, decided with the same arithmetic as the device tests
+  const double krsqd = c->phys.kern.rangesqd;
+  std::vector<std::vector<int>> sets(N);
+  for (int s = 0; s < N; s++) {
+    const int n = std::min(nn[s], capl);
+    const double r2i = (krsqd*hh[s])*hh[s];
+    auto& out = sets[io[s]];
+    for (int k = 0; k < n; k++) {
+      const int j = lst[(size_t) k*N + s];
+      if (mode == 0) {
+        double dx = x[j] - x[s], dy = y[j] - y[s], dz = z[j] - z[s];
+        if (c->phys.periodic) {
+          const double* hb = c->phys.boxhalf; const double* bs = c->phys.boxsize;
+          if (dx > hb[0]) dx -= bs[0]; else if (dx < -hb[0]) dx += bs[0];
+          if (dy > hb[1]) dy -= bs[1]; else if (dy < -hb[1]) dy += bs[1];
+          if (dz > hb[2]) dz -= bs[2]; else if (dz < -hb[2]) dz += bs[2];
+        }
+        const double d2 = (dx*dx + dy*dy) + dz*dz;
+        if (!(d2 < r2i)) continue;
+      }
+      out.push_back(io[j]);
+    }
+    std::sort(out.begin(), out.end());
+  }
+  int64_t total = 0;
+  offsets[0] = 0;
+  for (int i = 0; i < N; i++) { total += (int64_t) sets[i].size(); offsets[i + 1] = total; }
+  if (!ids) return total;
+  if (total > cap) return fail(c, GDF_ERR_CAPACITY, "neighbour set output capacity too small");
+  for (int i = 0; i < N; i++)
+    std::copy(sets[i].begin(), sets[i].end(), ids + offsets[i]);
+  return total;
+}
+
+}  // extern "C"

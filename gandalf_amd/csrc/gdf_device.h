@@ -1,0 +1,233 @@
+// Device-side helpers: smoothing kernels, periodic images, wave primitives.
+//
+// Kernel polynomials restate src/Headers/SmoothingKernel.h:132-408 (M4 and quintic, 3D) with
+// the reference's own constants; powers are formed by multiplication.  Tabulated lookups follow
+// SmoothingKernel.h:611-657 (nearest-lower entry of a 1000-entry table).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "gdf_internal.h"
+
+namespace gdf {
+
+enum { KW0 = 0, KW1, KWOMEGA, KWZETA, KWGRAV, KWPOT, KW0S2, KWOMEGAS2, KWZETAS2 };
+
+template <typename T> __device__ __forceinline__ const T* ktab(const KernelDesc& k);
+template <> __device__ __forceinline__ const double* ktab<double>(const KernelDesc& k) { return k.tab; }
+template <> __device__ __forceinline__ const float* ktab<float>(const KernelDesc& k) { return k.tabf; }
+
+// ---- M4 (SmoothingKernel.h:132-240) ----
+template <typename T> __device__ __forceinline__ T m4_w0(T norm, T s) {
+  if (s < T(1)) return norm*(T(1) - T(1.5)*s*s + T(0.75)*s*s*s);
+  if (s < T(2)) { T q = T(2) - s; return T(0.25)*norm*(q*q*q); }
+  return T(0);
+}
+template <typename T> __device__ __forceinline__ T m4_w1(T norm, T s) {
+  if (s < T(1)) return norm*(T(-3)*s + T(2.25)*s*s);
+  if (s < T(2)) return T(-0.75)*norm*(T(2) - s)*(T(2) - s);
+  return T(0);
+}
+template <typename T> __device__ __forceinline__ T m4_womega(T norm, T s) {
+  const T s3 = s*s*s;
+  if (s < T(1)) return norm*(T(-3) + T(1.5)*T(5)*s*s - T(0.75)*T(6)*s3);
+  if (s < T(2)) return norm*(T(-6) + T(3)*T(4)*s - T(1.5)*T(5)*s*s + T(0.25)*T(6)*s3);
+  return T(0);
+}
+template <typename T> __device__ __forceinline__ T m4_wzeta(T s) {
+  const T s2 = s*s, s4 = s2*s2, s5 = s4*s;
+  if (s < T(1)) return T(1.4) - T(2)*s2 + T(1.5)*s4 - T(0.6)*s5;
+  if (s < T(2)) return T(1.6) - T(4)*s2 + T(4)*s2*s - T(1.5)*s4 + T(0.2)*s5;
+  return T(0);
+}
+template <typename T> __device__ __forceinline__ T m4_wgrav(T s) {
+  const T s2 = s*s, s3 = s2*s, s4 = s2*s2;
+  if (s < T(1)) return T(1.333333333333333333333)*s - T(1.2)*s3 + T(0.5)*s4;
+  if (s < T(2))
+    return T(2.6666666666666666667)*s - T(3)*s2 + T(1.2)*s3 - T(0.166666666666666666667)*s4 -
+           T(0.06666666666666666667)/s2;
+  return T(1)/s2;
+}
+template <typename T> __device__ __forceinline__ T m4_wpot(T s) {
+  const T s2 = s*s, s3 = s2*s, s4 = s2*s2, s5 = s4*s;
+  if (s < T(1)) return T(1.4) - T(0.666666666666666666666666)*s2 + T(0.3)*s4 - T(0.1)*s5;
+  if (s < T(2))
+    return T(-1)/(T(15)*s) + T(1.6) - T(1.33333333333333333333333333)*s2 + s3 - T(0.3)*s4 +
+           T(1.0/30.0)*s5;
+  return T(1)/s;
+}
+
+// ---- quintic (SmoothingKernel.h:282-408) ----
+template <typename T> __device__ __forceinline__ T q_w0(T norm, T s) {
+  const T s2 = s*s, s3 = s2*s, s4 = s2*s2, s5 = s4*s;
+  if (s < T(1)) return norm*(T(66) - T(60)*s2 + T(30)*s4 - T(10)*s5);
+  if (s < T(2)) return norm*(T(51) + T(75)*s - T(210)*s2 + T(150)*s3 - T(45)*s4 + T(5)*s5);
+  if (s < T(3)) return norm*(T(243) - T(405)*s + T(270)*s2 - T(90)*s3 + T(15)*s4 - s5);
+  return T(0);
+}
+template <typename T> __device__ __forceinline__ T q_w1(T norm, T s) {
+  const T s2 = s*s, s3 = s2*s, s4 = s2*s2;
+  if (s < T(1)) return norm*(T(-120)*s + T(120)*s3 - T(50)*s4);
+  if (s < T(2)) return norm*(T(75) - T(420)*s + T(450)*s2 - T(180)*s3 + T(25)*s4);
+  if (s < T(3)) return norm*(T(-405) + T(540)*s - T(270)*s2 + T(60)*s3 - T(5)*s4);
+  return T(0);
+}
+template <typename T> __device__ __forceinline__ T q_womega(T norm, T s) {
+  const T s2 = s*s, s3 = s2*s, s4 = s2*s2, s5 = s4*s;
+  if (s < T(1)) return norm*(T(-198) + T(60)*T(5)*s2 - T(30)*T(7)*s4 + T(10)*T(8)*s5);
+  if (s < T(2))
+    return norm*(T(-153) - T(75)*T(4)*s + T(210)*T(5)*s2 - T(150)*T(6)*s3 + T(45)*T(7)*s4 -
+                 T(5)*T(8)*s5);
+  if (s < T(3))
+    return norm*(T(-729) + T(405)*T(4)*s - T(270)*T(5)*s2 + T(90)*T(6)*s3 - T(15)*T(7)*s4 +
+                 T(8)*s5);
+  return T(0);
+}
+template <typename T> __device__ __forceinline__ T q_wzeta(T s) {
+  const T s2 = s*s, s3 = s2*s, s4 = s2*s2, s5 = s4*s, s6 = s3*s3, s7 = s6*s;
+  if (s < T(1)) return T(33)*s2 - T(15)*s4 + T(5)*s6 - T(1.42857142857)*s7 - T(34.14285714);
+  if (s < T(2))
+    return T(25.5)*s2 + T(25)*s3 - T(52.5)*s4 + T(30)*s5 - T(7.5)*s6 + T(0.7142857143)*s7 -
+           T(33.785714286);
+  if (s < T(3))
+    return T(121.5)*s2 - T(135)*s3 + T(67.5)*s4 - T(18)*s5 + T(2.5)*s6 - T(0.142857143)*s7 -
+           T(52.07142857);
+  return T(0);
+}
+template <typename T> __device__ __forceinline__ T q_wgrav(T s) {
+  const T s2 = s*s, s3 = s2*s, s4 = s2*s2, s5 = s4*s, s6 = s3*s3;
+  const T c = T(12.0/359.0);
+  if (s < T(1)) return c*(T(22)*s - T(12)*s3 + T(30.0/7.0)*s5 - T(5.0/4.0)*s6);
+  if (s < T(2))
+    return c*(T(17)*s + T(75.0/4.0)*s2 - T(42)*s3 + T(25)*s4 - T(45.0/7.0)*s5 +
+              T(5.0/8.0)*s6 + T(5.0/56.0)/s2);
+  if (s < T(3))
+    return c*(T(81)*s - T(405.0/4.0)*s2 + T(54)*s3 - T(15)*s4 + T(15.0/7.0)*s5 -
+              T(1.0/8.0)*s6 - T(507.0/56.0)/s2);
+  return T(1)/s2;
+}
+template <typename T> __device__ __forceinline__ T q_wpot(T s) {
+  const T s2 = s*s, s3 = s2*s, s4 = s2*s2, s5 = s4*s, s6 = s3*s3, s7 = s6*s;
+  const T c = T(12.0/359.0);
+  if (s < T(1))
+    return c*(T(-11)*s2 + T(3)*s4 - T(5.0/7.0)*s6 + T(5.0/28.0)*s7 + T(478.0/14.0));
+  if (s < T(2))
+    return c*(T(-17.0/2.0)*s2 - T(25.0/4.0)*s3 + T(21.0/2.0)*s4 - T(5)*s5 + T(15.0/14.0)*s6 -
+              T(5.0/56.0)*s7 + T(473.0/14.0) + T(5.0/56.0)/s);
+  if (s < T(3))
+    return c*(T(-81.0/2.0)*s2 + T(135.0/4.0)*s3 - T(27.0/2.0)*s4 + T(3)*s5 - T(5.0/14.0)*s6 +
+              T(1.0/56.0)*s7 + T(729.0/14.0) - T(507.0/56.0)/s);
+  return T(1)/s;
+}
+
+// ---- dispatch; s is r/h (or (r/h)^2 for the _s2 forms) ----
+template <typename T> __device__ __forceinline__ T tab_lin(const KernelDesc& k, int w, T s) {
+  if (s >= T(k.range)) return T(0);
+  return ktab<T>(k)[w*k.res + (int) (s*T(k.resinv))];
+}
+template <typename T> __device__ __forceinline__ T tab_sq(const KernelDesc& k, int w, T s2) {
+  if (s2 >= T(k.rangesqd)) return T(0);
+  return ktab<T>(k)[w*k.res + (int) (s2*T(k.resinvsqd))];
+}
+
+template <typename T> __device__ __forceinline__ T kw1(const KernelDesc& k, T s) {
+  if (k.tabulated) return tab_lin<T>(k, KW1, s);
+  return k.type == 0 ? m4_w1<T>(T(k.norm), s) : q_w1<T>(T(k.norm), s);
+}
+template <typename T> __device__ __forceinline__ T kwgrav(const KernelDesc& k, T s) {
+  if (k.tabulated) {
+    if (s >= T(k.range)) return T(1)/(s*s);
+    return ktab<T>(k)[KWGRAV*k.res + (int) (s*T(k.resinv))];
+  }
+  return k.type == 0 ? m4_wgrav<T>(s) : q_wgrav<T>(s);
+}
+template <typename T> __device__ __forceinline__ T kwpot(const KernelDesc& k, T s) {
+  if (k.tabulated) {
+    if (s >= T(k.range)) return T(1)/s;
+    return ktab<T>(k)[KWPOT*k.res + (int) (s*T(k.resinv))];
+  }
+  return k.type == 0 ? m4_wpot<T>(s) : q_wpot<T>(s);
+}
+// density triple w0, womega, wzeta at s^2 (GradhSph.cpp:197-204)
+template <typename T>
+__device__ __forceinline__ void kdens(const KernelDesc& k, T s2, T& w0, T& wom, T& wz) {
+  if (k.tabulated) {
+    if (s2 >= T(k.rangesqd)) { w0 = wom = wz = T(0); return; }
+    const int idx = (int) (s2*T(k.resinvsqd));
+    const T* t = ktab<T>(k);
+    w0 = t[KW0S2*k.res + idx]; wom = t[KWOMEGAS2*k.res + idx]; wz = t[KWZETAS2*k.res + idx];
+    return;
+  }
+  const T s = sqrt(s2);
+  if (k.type == 0) { w0 = m4_w0<T>(T(k.norm), s); wom = m4_womega<T>(T(k.norm), s); wz = m4_wzeta<T>(s); }
+  else { w0 = q_w0<T>(T(k.norm), s); wom = q_womega<T>(T(k.norm), s); wz = q_wzeta<T>(s); }
+}
+
+// ---- periodic nearest image (GhostNeighbours.hpp:119-138) ----
+template <typename T>
+__device__ __forceinline__ void nearest_image(const PhysParams& p, T& dx, T& dy, T& dz) {
+  if (!p.periodic) return;
+  if (dx > T(p.boxhalf[0])) dx -= T(p.boxsize[0]); else if (dx < -T(p.boxhalf[0])) dx += T(p.boxsize[0]);
+  if (dy > T(p.boxhalf[1])) dy -= T(p.boxsize[1]); else if (dy < -T(p.boxhalf[1])) dy += T(p.boxsize[1]);
+  if (dz > T(p.boxhalf[2])) dz -= T(p.boxsize[2]); else if (dz < -T(p.boxhalf[2])) dz += T(p.boxsize[2]);
+}
+
+// squared distance with the reference's operation order (no fused multiply-add), so
+// neighbour-set membership is decided exactly as the oracle decides it
+__device__ __forceinline__ double dot3_rn(double a, double b, double c) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
+}
+__device__ __forceinline__ double range2_rn(double krsqd, double h) {
+  return __dmul_rn(__dmul_rn(krsqd, h), h);
+}
+
+// ---- wave primitives (wave64) ----
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ int lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned) (mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned) mask, 0));
+}
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = __lane_id();
+  #pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_max_int(int v) {
+  #pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+  #pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_bcast(int v, int src) { return __shfl(v, src, 64); }
+
+// Next work item for a persistent wavefront.  Every active lane takes part in the atomic (the
+// first active lane adds one, the others zero) and the result is broadcast with readfirstlane,
+// so the index is a scalar and the loop that consumes it has uniform control flow.  Fetching
+// behind `if (lane == 0)` and broadcasting with a shuffle instead lets the compiler treat the
+// loop exit as divergent and re-enter the loop body with lane 0 split off, where the shuffle
+// then reads a stale index.
+__device__ __forceinline__ int wave_next_item(int* counter) {
+  const int first = lanes_below(__ballot(1)) == 0;
+  const int old = atomicAdd(counter, first ? 1 : 0);
+  return __builtin_amdgcn_readfirstlane(old);
+}
+
+// Kahan-compensated accumulator for the mixed-precision pair sums
+struct KahanF {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+}  // namespace gdf

@@ -1,0 +1,349 @@
+// Tree-gravity walk with per-wavefront interaction lists.
+//
+// Replaces the gravity half of GradhSphTree::UpdateAllSphForces
+// (src/GradhSph/GradhSphTree.cpp:444-657): the walk of Tree::ComputeGravityInteractionAndGhostList
+// (src/Tree/Tree.cpp:628-735), the kernel-softened neighbour gravity of
+// GradhSph::ComputeSphHydroGravForces (src/GradhSph/GradhSph.cpp:555-565), the direct sum of
+// GradhSphBase::ComputeDirectGravForces (:657-690) and the quadrupole cell terms of
+// ComputeQuadropole (src/Headers/NeighbourSearch.h:423-449).
+//
+// One wavefront per walk group.  Acceptance per node, with the group's bounding box G:
+//   * kernel-range test (as the reference's neighbour-cell test, Tree.cpp:662-663): if the
+//     minimum box-box distance is below k*max(hmax_G, hmax_cell) the node is resolved to
+//     particles (bucket of <= Nleafmax particles) or opened;
+//   * geometric MAC (Tree.h:413-432) evaluated from the nearest point of G: accept the
+//     multipole iff dist(G, centre_cell)^2 >= cdistsqd = max(rmax^2, hmax^2)/theta^2, a
+//     conservative form of the reference's centre-to-centre test;
+//   * otherwise open, or send the bucket's particles to the direct list.
+// Particles always use the unified pair term: kernel-softened with the zeta corrections when
+// inside either smoothing range, Newtonian otherwise (the two agree beyond the kernel range).
+// Accepted cells and direct particles are staged 64 at a time in LDS and every lane (one target
+// particle) sweeps the batch.
+#include <hip/hip_runtime.h>
+
+#include "gdf_internal.h"
+#include "gdf_device.h"
+
+namespace gdf {
+namespace {
+
+constexpr int PCAP = 1024;   // particle ring buffer (power of two); Nleafmax <= 14
+constexpr int CCAP = 256;    // cell ring buffer (power of two)
+
+struct GravArgs {
+  int N, ngroups, nleaf;
+  const int* grp_node; const int* grp_first; const int* grp_count;
+  const int* first; const int* last; const int* left; const int* right;
+  const double4* geo_c; const double4* geo_h; const double* mp;
+  const double4* posm; const double4* grec;     // grec: (h, zeta, hfactor, 0)
+  double* atx; double* aty; double* atz; double* gpot;
+  int* stacks; int* counter; int* flags; long long* icount;
+  int32_t* gnc; int32_t* gnn;
+  PhysParams p;
+};
+
+// FP64 pair term, operation order of GradhSph.cpp:498-565 / :671-686
+__device__ __forceinline__ void pair_fp64(const KernelDesc& k, double krsqd, double dx, double dy,
+                                          double dz, double mj, double hi, double hj, double zi,
+                                          double zj, double hfi, double hfj, double& ax,
+                                          double& ay, double& az, double& pot) {
+  const double d2 = dx*dx + dy*dy + dz*dz;
+  if (d2 < krsqd*hi*hi || d2 < krsqd*hj*hj) {
+    const double invh_i = 1/hi, invh_j = 1/hj;
+    const double drmag = sqrt(d2 + 1.0e-20);
+    const double inv = 1.0/drmag;
+    dx *= inv; dy *= inv; dz *= inv;
+    const double si = drmag*invh_i, sj = drmag*invh_j;
+    const double wkerni = hfi*kw1<double>(k, si);
+    const double wkernj = hfj*kw1<double>(k, sj);
+    const double paux = 0.5*(invh_i*invh_i*kwgrav<double>(k, si) + zi*wkerni +
+                             invh_j*invh_j*kwgrav<double>(k, sj) + zj*wkernj);
+    ax += mj*dx*paux; ay += mj*dy*paux; az += mj*dz*paux;
+    pot += 0.5*mj*(invh_i*kwpot<double>(k, si) + invh_j*kwpot<double>(k, sj));
+  } else {
+    const double dd = d2 + 1.0e-20;
+    const double inv = 1.0/sqrt(dd);
+    const double inv3 = inv*inv*inv;
+    ax += mj*dx*inv3; ay += mj*dy*inv3; az += mj*dz*inv3;
+    pot += mj*inv;
+  }
+}
+
+__device__ __forceinline__ void pair_fp32(const KernelDesc& k, float krsqd, float dx, float dy,
+                                          float dz, float mj, float hi, float hj, float zi,
+                                          float zj, float hfi, float hfj, KahanF& ax, KahanF& ay,
+                                          KahanF& az, KahanF& pot) {
+  const float d2 = dx*dx + dy*dy + dz*dz;
+  if (d2 < krsqd*hi*hi || d2 < krsqd*hj*hj) {
+    const float invh_i = 1.0f/hi, invh_j = 1.0f/hj;
+    const float drmag = sqrtf(d2 + 1.0e-20f);
+    const float inv = 1.0f/drmag;
+    dx *= inv; dy *= inv; dz *= inv;
+    const float si = drmag*invh_i, sj = drmag*invh_j;
+    const float wkerni = hfi*kw1<float>(k, si);
+    const float wkernj = hfj*kw1<float>(k, sj);
+    const float paux = 0.5f*(invh_i*invh_i*kwgrav<float>(k, si) + zi*wkerni +
+                             invh_j*invh_j*kwgrav<float>(k, sj) + zj*wkernj);
+    ax.add(mj*dx*paux); ay.add(mj*dy*paux); az.add(mj*dz*paux);
+    pot.add(0.5f*mj*(invh_i*kwpot<float>(k, si) + invh_j*kwpot<float>(k, sj)));
+  } else {
+    const float inv = rsqrtf(d2 + 1.0e-20f);
+    const float inv3 = inv*inv*inv;
+    ax.add(mj*dx*inv3); ay.add(mj*dy*inv3); az.add(mj*dz*inv3);
+    pot.add(mj*inv);
+  }
+}
+
+// quadrupole cell term with dr = rp - com (NeighbourSearch.h:423-449)
+template <typename T>
+__device__ __forceinline__ void quad_term(T dx, T dy, T dz, T mc, const T* q, T& ax, T& ay,
+                                          T& az, T& pot) {
+  const T drsqd = dx*dx + dy*dy + dz*dz + T(1.0e-20);
+  const T invdrsqd = T(1)/drsqd;
+  const T invdrmag = sqrt(invdrsqd);
+  const T invdr5 = invdrsqd*invdrsqd*invdrmag;
+  const T mono = mc*invdrsqd*invdrmag;
+  const T qscalar = q[0]*dx*dx + q[2]*dy*dy - (q[0] + q[2])*dz*dz +
+                    T(2)*(q[1]*dx*dy + q[3]*dx*dz + q[4]*dy*dz);
+  const T qfactor = T(2.5)*qscalar*invdr5*invdrsqd;
+  ax += -mono*dx + (q[0]*dx + q[1]*dy + q[3]*dz)*invdr5 - qfactor*dx;
+  ay += -mono*dy + (q[1]*dx + q[2]*dy + q[4]*dz)*invdr5 - qfactor*dy;
+  az += -mono*dz + (q[3]*dx + q[4]*dy - (q[0] + q[2])*dz)*invdr5 - qfactor*dz;
+  pot += mc*invdrmag + T(0.5)*qscalar*invdr5;
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
+  __shared__ int pbuf[PCAP];
+  __shared__ int cbuf[CCAP];
+  // staged batch: particles (x, y, z, m, h, zeta, hfactor) or cells (com x3, m, q x5)
+  __shared__ double stage[9*kWave];
+  double (*sd)[kWave] = reinterpret_cast<double (*)[kWave]>(stage);
+  float (*sf)[kWave] = reinterpret_cast<float (*)[kWave]>(stage);
+  const int lane = threadIdx.x;
+  int* stk = a.stacks + (size_t) blockIdx.x*kStackCap;
+  const int N = a.N;
+  const KernelDesc& K = a.p.kern;
+  const double kr = K.range, krsqd = K.rangesqd;
+  for (;;) {
+    const int g = wave_next_item(a.counter);
+    if (g >= a.ngroups) break;
+    const int G = a.grp_node[g], gfirst = a.grp_first[g], gcnt = a.grp_count[g];
+    const bool has = lane < gcnt;
+    const int i = gfirst + lane;
+    const double4 gc = a.geo_c[G], gh = a.geo_h[G];
+    const double hmaxG = gc.w;
+    double4 me = make_double4(0, 0, 0, 0), mr = make_double4(1, 0, 0, 0);
+    if (has) { me = a.posm[i]; mr = a.grec[i]; }
+    // accumulators; gpot starts at the self term (m/h) wpot(0) (GradhSphTree.cpp:512)
+    double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
+    KahanF fx, fy, fz, fp;
+    const double pot0 = has ? (me.w/mr.x)*kwpot<double>(K, 0.0) : 0.0;
+    // target position relative to the group centre (cell-relative frame of the FP32 path)
+    const float rxi = (float) (me.x - gc.x), ryi = (float) (me.y - gc.y), rzi = (float) (me.z - gc.z);
+    int sp = 1, phead = 0, ptail = 0, chead = 0, ctail = 0;
+    int nnear = 0, ncell = 0;
+    bool overflow = false;
+    if (lane == 0) __hip_atomic_store(&stk[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence_block();
+    while (sp > 0) {
+      const int b = min(sp, kWave);
+      int node = -1;
+      if (lane < b) node = __hip_atomic_load(&stk[sp - 1 - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sp -= b;
+      bool open = false, near = false, cell = false;
+      int ef = 0, ec = 0, cl = 0, cr = 0;
+      if (node >= 0) {
+#pragma clang fp contract(off)
+        const double4 nc = a.geo_c[node], nh = a.geo_h[node];
+        double dx = nc.x - gc.x, dy = nc.y - gc.y, dz = nc.z - gc.z;
+        nearest_image<double>(a.p, dx, dy, dz);
+        const double gx = fmax(fabs(dx) - nh.x - gh.x, 0.0);
+        const double gy = fmax(fabs(dy) - nh.y - gh.y, 0.0);
+        const double gz = fmax(fabs(dz) - nh.z - gh.z, 0.0);
+        const double dbox2 = gx*gx + gy*gy + gz*gz;
+        const double hr = kr*fmax(hmaxG, nc.w);
+        if (node >= N - 1) { ef = node - (N - 1); ec = 1; }
+        else { ef = a.first[node]; ec = a.last[node] - a.first[node] + 1; }
+        const bool bucket = ec <= a.nleaf || node >= N - 1;
+        if (dbox2 < hr*hr) {
+          if (bucket) near = true; else open = true;
+        } else {
+          const double ex = fmax(fabs(dx) - gh.x, 0.0);
+          const double ey = fmax(fabs(dy) - gh.y, 0.0);
+          const double ez = fmax(fabs(dz) - gh.z, 0.0);
+          const double dmac2 = ex*ex + ey*ey + ez*ez;
+          if (dmac2 >= nh.w) {
+            if (ec == 1) near = true; else cell = true;
+          } else {
+            if (bucket) near = true; else open = true;
+          }
+        }
+        if (open) { cl = a.left[node]; cr = a.right[node]; }
+      }
+      // children onto the stack
+      const unsigned long long om = __ballot(open);
+      const int nopen = __popcll(om);
+      if (sp + 2*nopen > kStackCap) { overflow = true; break; }
+      if (open) {
+        const int o = sp + 2*lanes_below(om);
+        __hip_atomic_store(&stk[o], cr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&stk[o + 1], cl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      sp += 2*nopen;
+      // accepted cells
+      const unsigned long long cm = __ballot(cell);
+      if (cell) cbuf[(ctail + lanes_below(cm)) & (CCAP - 1)] = node;
+      ctail += __popcll(cm);
+      ncell += __popcll(cm);
+      // direct particles
+      const int ecount = near ? ec : 0;
+      const int incl = wave_incl_scan(ecount);
+      const int tot = __shfl(incl, 63, 64);
+      for (int k = 0; k < ecount; k++) pbuf[(ptail + incl - ecount + k) & (PCAP - 1)] = ef + k;
+      ptail += tot;
+      nnear += tot;
+      __syncthreads();
+      // evaluate full batches (and everything left once the walk is done)
+      while (ctail - chead >= kWave || (sp == 0 && ctail > chead)) {
+        const int nb = min(ctail - chead, kWave);
+        if (lane < nb) {
+          const int c = cbuf[(chead + lane) & (CCAP - 1)];
+          const double* m = a.mp + 10*(size_t) c;
+          double cx = m[0], cy = m[1], cz = m[2];
+          if (a.p.periodic) {
+            // nearest periodic replica of the cell relative to the group centre
+            double dx = cx - gc.x, dy = cy - gc.y, dz = cz - gc.z;
+            nearest_image<double>(a.p, dx, dy, dz);
+            cx = gc.x + dx; cy = gc.y + dy; cz = gc.z + dz;
+          }
+          if (MIXED) {
+            sf[0][lane] = (float) (cx - gc.x); sf[1][lane] = (float) (cy - gc.y);
+            sf[2][lane] = (float) (cz - gc.z);
+            for (int k = 3; k < 9; k++) sf[k][lane] = (float) m[k];
+          } else {
+            sd[0][lane] = cx; sd[1][lane] = cy; sd[2][lane] = cz;
+            for (int k = 3; k < 9; k++) sd[k][lane] = m[k];
+          }
+        }
+        __syncthreads();
+        if (has) {
+          if (MIXED) {
+            for (int q = 0; q < nb; q++) {
+              const float qq[5] = {sf[4][q], sf[5][q], sf[6][q], sf[7][q], sf[8][q]};
+              float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
+              quad_term<float>(rxi - sf[0][q], ryi - sf[1][q], rzi - sf[2][q], sf[3][q], qq, bx,
+                               by, bz, bp);
+              fx.add(bx); fy.add(by); fz.add(bz); fp.add(bp);
+            }
+          } else {
+            for (int q = 0; q < nb; q++) {
+              const double qq[5] = {sd[4][q], sd[5][q], sd[6][q], sd[7][q], sd[8][q]};
+              quad_term<double>(me.x - sd[0][q], me.y - sd[1][q], me.z - sd[2][q], sd[3][q], qq,
+                                ax, ay, az, pot);
+            }
+          }
+        }
+        chead += nb;
+        __syncthreads();
+      }
+      while (ptail - phead >= kWave || (sp == 0 && ptail > phead)) {
+        const int nb = min(ptail - phead, kWave);
+        if (lane < nb) {
+          const int j = pbuf[(phead + lane) & (PCAP - 1)];
+          const double4 pj = a.posm[j];
+          const double4 rj = a.grec[j];
+          double dx = pj.x - gc.x, dy = pj.y - gc.y, dz = pj.z - gc.z;
+          nearest_image<double>(a.p, dx, dy, dz);
+          if (MIXED) {
+            sf[0][lane] = (float) dx; sf[1][lane] = (float) dy; sf[2][lane] = (float) dz;
+            sf[3][lane] = (float) pj.w; sf[4][lane] = (float) rj.x; sf[5][lane] = (float) rj.y;
+            sf[6][lane] = (float) rj.z;
+          } else {
+            sd[0][lane] = gc.x + dx; sd[1][lane] = gc.y + dy; sd[2][lane] = gc.z + dz;
+            sd[3][lane] = pj.w; sd[4][lane] = rj.x; sd[5][lane] = rj.y; sd[6][lane] = rj.z;
+          }
+        }
+        __syncthreads();
+        if (has) {
+          if (MIXED) {
+            const float hi = (float) mr.x, zi = (float) mr.y, hfi = (float) mr.z;
+            const float krs = (float) krsqd;
+            for (int q = 0; q < nb; q++)
+              pair_fp32(K, krs, sf[0][q] - rxi, sf[1][q] - ryi, sf[2][q] - rzi, sf[3][q], hi,
+                        sf[4][q], zi, sf[5][q], hfi, sf[6][q], fx, fy, fz, fp);
+          } else {
+            for (int q = 0; q < nb; q++) {
+              double dx = sd[0][q] - me.x, dy = sd[1][q] - me.y, dz = sd[2][q] - me.z;
+              if (a.p.periodic) nearest_image<double>(a.p, dx, dy, dz);
+              pair_fp64(K, krsqd, dx, dy, dz, sd[3][q], mr.x, sd[4][q], mr.y, sd[5][q], mr.z,
+                        sd[6][q], ax, ay, az, pot);
+            }
+          }
+        }
+        phead += nb;
+        __syncthreads();
+      }
+      __threadfence_block();
+    }
+    if (overflow) {
+      if (lane == 0) atomicOr(&a.flags[2], 1);
+      continue;
+    }
+    if (has) {
+      if (MIXED) {
+        a.atx[i] = (double) fx.s; a.aty[i] = (double) fy.s; a.atz[i] = (double) fz.s;
+        a.gpot[i] = pot0 + (double) fp.s;
+      } else {
+        a.atx[i] = ax; a.aty[i] = ay; a.atz[i] = az;
+        a.gpot[i] = pot0 + pot;
+      }
+    }
+    if (lane == 0) {
+      a.gnc[g] = ncell;
+      a.gnn[g] = nnear;
+      atomicAdd((unsigned long long*) &a.icount[2], (unsigned long long) nnear*gcnt);
+      atomicAdd((unsigned long long*) &a.icount[3], (unsigned long long) ncell*gcnt);
+    }
+  }
+}
+
+__global__ void k_pack_grav(int N, const double* __restrict__ h, const double* __restrict__ zeta,
+                            const double* __restrict__ hf, double4* __restrict__ g) {
+  const int i = blockIdx.x*256 + threadIdx.x;
+  if (i < N) g[i] = make_double4(h[i], zeta[i], hf[i], 0.0);
+}
+
+__global__ void k_zero8(int* p) { if (threadIdx.x < 8) p[threadIdx.x] = 0; }
+
+}  // namespace
+
+int gravity_walk(gdf_ctx* c) {
+  const int N = c->N;
+  hipStream_t s = c->stream;
+  if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "gravity pass before tree build");
+  k_pack_grav<<<nblocks(N, 256), 256, 0, s>>>(N, c->pf[F_H], c->df[D_ZETA], c->df[D_HFACTOR],
+                                                c->grec);
+  GravArgs a;
+  a.N = N; a.ngroups = c->ngroups; a.nleaf = c->phys.nleaf;
+  a.grp_node = c->grp_node; a.grp_first = c->grp_first; a.grp_count = c->grp_count;
+  a.first = c->first; a.last = c->last; a.left = c->left; a.right = c->right;
+  a.geo_c = c->geo_c; a.geo_h = c->geo_h; a.mp = c->mp;
+  a.posm = c->posm; a.grec = c->grec;
+  a.atx = c->df[D_ATX]; a.aty = c->df[D_ATY]; a.atz = c->df[D_ATZ]; a.gpot = c->df[D_GPOT];
+  a.stacks = c->stacks; a.counter = c->counters + 4; a.flags = c->counters;
+  a.icount = c->icount; a.gnc = c->grav_nc; a.gnn = c->grav_nn;
+  a.p = c->phys;
+  k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
+  k_zero8<<<1, 64, 0, s>>>(c->counters);
+  const int grid = std::min(c->ngroups, kWalkSlots);
+  if (c->cfg.precision == GDF_PREC_MIXED) k_gravity<true><<<grid, kWave, 0, s>>>(a);
+  else k_gravity<false><<<grid, kWave, 0, s>>>(a);
+  GDF_HIP(c, hipGetLastError());
+  GDF_HIP(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int)*8, hipMemcpyDeviceToHost, s));
+  GDF_HIP(c, hipStreamSynchronize(s));
+  if (c->h_counters[2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in gravity pass");
+  return GDF_OK;
+}
+
+}  // namespace gdf

@@ -1,0 +1,316 @@
+// Leapfrog KDK integrator, periodic wrap, timestep and diagnostics reductions.
+//
+// SphLeapfrogKDK::AdvanceParticles / EndTimestep (src/Hydrodynamics/SphLeapfrogKDK.cpp:76-127,
+// :219-272), TimeIntegration::CheckBoundaries (src/Common/Integration.cpp:11-80),
+// SphIntegration::Timestep (src/Hydrodynamics/SphIntegration.cpp:81-134) with the global minimum
+// of Simulation::ComputeGlobalTimestep (src/Common/Simulation.cpp:1669-1754), and
+// Simulation::CalculateDiagnostics (src/Common/SimAnalysis.hpp:52-185).  These streaming kernels
+// are HBM-bound; integration arithmetic keeps the reference's operation order (no contraction)
+// so positions and velocities are reproduced bit for bit from identical inputs.
+#include <hip/hip_runtime.h>
+
+#include "gdf_internal.h"
+#include "gdf_device.h"
+
+namespace gdf {
+namespace {
+
+constexpr int BS = 256;
+constexpr int RED_BLOCKS = 1024;
+
+struct AdvArgs {
+  int N, energy;
+  double dt;
+  const double* r0[3]; const double* v0[3]; const double* a0[3];
+  double* r[3]; double* v[3];
+  const double* u0; const double* dudt0; double* u;
+};
+
+__global__ void k_advance(AdvArgs a) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= a.N) return;
+  const double dt = a.dt;
+  for (int k = 0; k < 3; k++) a.r[k][i] = a.r0[k][i] + a.v0[k][i]*dt + 0.5*a.a0[k][i]*dt*dt;
+  for (int k = 0; k < 3; k++) a.v[k][i] = a.v0[k][i] + a.a0[k][i]*dt;
+  if (a.energy) a.u[i] = a.u0[i] + a.dudt0[i]*dt;
+}
+
+struct WrapArgs { int N; double lo[3], hi[3], size[3]; double* r[3]; double* r0[3]; };
+
+__global__ void k_wrap(WrapArgs a) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= a.N) return;
+  for (int k = 0; k < 3; k++) {
+    double r = a.r[k][i];
+    if (r < a.lo[k]) { a.r[k][i] = r + a.size[k]; a.r0[k][i] += a.size[k]; }
+    r = a.r[k][i];
+    if (r > a.hi[k]) { a.r[k][i] = r - a.size[k]; a.r0[k][i] -= a.size[k]; }
+  }
+}
+
+struct EndArgs {
+  int N, energy;
+  double dt;
+  const double* a[3]; double* v[3]; double* r0[3]; double* v0[3]; double* a0[3];
+  const double* r[3];
+  const double* dudt; double* u; double* u0; double* dudt0;
+};
+
+__global__ void k_end_step(EndArgs e) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= e.N) return;
+  const double dt = e.dt;
+  for (int k = 0; k < 3; k++) {
+    const double a = e.a[k][i], a0 = e.a0[k][i];
+    const double v = e.v[k][i] + 0.5*dt*(a - a0);
+    e.v[k][i] = v;
+    e.r0[k][i] = e.r[k][i];
+    e.v0[k][i] = v;
+    e.a0[k][i] = a;
+  }
+  if (e.energy) {
+    const double dudt = e.dudt[i], dudt0 = e.dudt0[i];
+    double u = e.u[i] + 0.5*(dudt - dudt0)*dt;
+    if (u <= 0.0) u = e.u0[i] + dudt0*dt;
+    e.u[i] = u;
+    e.u0[i] = u;
+    e.dudt0[i] = dudt;
+  }
+}
+
+struct TsArgs {
+  int N, hydro, energy;
+  double courant, accel, emult;
+  const double* h; const double* sound; const double* div_v; const double* a[3];
+  const double* u; const double* dudt;
+  double* dt_out; double* part;
+};
+
+__global__ void k_timestep(TsArgs t) {
+#pragma clang fp contract(off)
+  __shared__ double s[BS];
+  double best = 9.9e50;
+  for (int i = blockIdx.x*BS + threadIdx.x; i < t.N; i += gridDim.x*BS) {
+    const double h = t.h[i];
+    double ts;
+    if (t.hydro) ts = t.courant*h/(t.sound[i] + h*fabs(t.div_v[i]) + 1.0e-50);
+    else ts = t.courant*h/(h*fabs(t.div_v[i]) + 1.0e-50);
+    const double ax = t.a[0][i], ay = t.a[1][i], az = t.a[2][i];
+    const double amag = sqrt(ax*ax + ay*ay + az*az);
+    ts = fmin(ts, t.accel*sqrt(h/(amag + 1.0e-50)));
+    if (t.energy) ts = fmin(ts, t.emult*(t.u[i]/(fabs(t.dudt[i]) + 1.0e-20)));
+    if (t.dt_out) t.dt_out[i] = ts;
+    best = fmin(best, ts);
+  }
+  s[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = BS/2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) t.part[blockIdx.x] = s[0];
+}
+
+__global__ void k_min_final(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double s[BS];
+  double best = 9.9e50;
+  for (int b = threadIdx.x; b < nb; b += BS) best = fmin(best, part[b]);
+  s[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = BS/2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+__global__ void k_max_h(int N, const double* __restrict__ h, double* __restrict__ part) {
+  __shared__ double s[BS];
+  double best = 0.0;
+  for (int i = blockIdx.x*BS + threadIdx.x; i < N; i += gridDim.x*BS) best = fmax(best, h[i]);
+  s[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = BS/2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+constexpr int ND = 11;  // mtot ke u gpe mom[3] ang[3] (+pad)
+
+struct DiagArgs {
+  int N;
+  const double* r[3]; const double* v[3]; const double* m; const double* u; const double* gpot;
+  double* part;
+};
+
+__global__ void k_diag(DiagArgs d) {
+  __shared__ double s[ND][BS];
+  double acc[ND];
+  for (int k = 0; k < ND; k++) acc[k] = 0.0;
+  for (int i = blockIdx.x*BS + threadIdx.x; i < d.N; i += gridDim.x*BS) {
+    const double m = d.m[i];
+    const double rx = d.r[0][i], ry = d.r[1][i], rz = d.r[2][i];
+    const double vx = d.v[0][i], vy = d.v[1][i], vz = d.v[2][i];
+    acc[0] += m;
+    acc[1] += m*(vx*vx + vy*vy + vz*vz);
+    acc[2] += m*d.u[i];
+    acc[3] -= m*d.gpot[i];
+    acc[4] += m*vx; acc[5] += m*vy; acc[6] += m*vz;
+    acc[7] += m*(ry*vz - rz*vy);
+    acc[8] += m*(rz*vx - rx*vz);
+    acc[9] += m*(rx*vy - ry*vx);
+  }
+  for (int k = 0; k < ND; k++) s[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = BS/2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < ND; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < ND) d.part[blockIdx.x*ND + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void k_zero_d(int n, double* __restrict__ p) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_add_atree(int N, double* __restrict__ ax, double* __restrict__ ay,
+                            double* __restrict__ az, const double* __restrict__ tx,
+                            const double* __restrict__ ty, const double* __restrict__ tz) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= N) return;
+  ax[i] += tx[i]; ay[i] += ty[i]; az[i] += tz[i];
+}
+
+__global__ void k_pack_posm(int N, const double* __restrict__ x, const double* __restrict__ y,
+                            const double* __restrict__ z, const double* __restrict__ m,
+                            double4* __restrict__ out) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i < N) out[i] = make_double4(x[i], y[i], z[i], m[i]);
+}
+
+}  // namespace
+
+int integ_advance(gdf_ctx* c, double dt) {
+  AdvArgs a;
+  a.N = c->N; a.energy = c->phys.energy_integration; a.dt = dt;
+  for (int k = 0; k < 3; k++) {
+    a.r0[k] = c->pf[F_X0 + k]; a.v0[k] = c->pf[F_VX0 + k]; a.a0[k] = c->pf[F_AX0 + k];
+    a.r[k] = c->pf[F_X + k]; a.v[k] = c->pf[F_VX + k];
+  }
+  a.u0 = c->pf[F_U0]; a.dudt0 = c->pf[F_DUDT0]; a.u = c->pf[F_U];
+  k_advance<<<nblocks(c->N, BS), BS, 0, c->stream>>>(a);
+  GDF_HIP(c, hipGetLastError());
+  c->tree_valid = false;
+  return GDF_OK;
+}
+
+int integ_wrap(gdf_ctx* c) {
+  if (!c->phys.periodic) return GDF_OK;
+  WrapArgs w;
+  w.N = c->N;
+  for (int k = 0; k < 3; k++) {
+    w.lo[k] = c->phys.boxmin[k]; w.hi[k] = c->phys.boxmax[k]; w.size[k] = c->phys.boxsize[k];
+    w.r[k] = c->pf[F_X + k]; w.r0[k] = c->pf[F_X0 + k];
+  }
+  k_wrap<<<nblocks(c->N, BS), BS, 0, c->stream>>>(w);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int integ_end_step(gdf_ctx* c, double dtstep) {
+  EndArgs e;
+  e.N = c->N; e.energy = c->phys.energy_integration; e.dt = dtstep;
+  for (int k = 0; k < 3; k++) {
+    e.a[k] = c->df[D_AX + k]; e.v[k] = c->pf[F_VX + k]; e.r0[k] = c->pf[F_X0 + k];
+    e.v0[k] = c->pf[F_VX0 + k]; e.a0[k] = c->pf[F_AX0 + k]; e.r[k] = c->pf[F_X + k];
+  }
+  e.dudt = c->df[D_DUDT]; e.u = c->pf[F_U]; e.u0 = c->pf[F_U0]; e.dudt0 = c->pf[F_DUDT0];
+  k_end_step<<<nblocks(c->N, BS), BS, 0, c->stream>>>(e);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int integ_timestep(gdf_ctx* c, double* dtmin) {
+  TsArgs t;
+  t.N = c->N; t.hydro = c->phys.hydro_forces; t.energy = c->phys.energy_integration;
+  t.courant = c->phys.courant_mult; t.accel = c->phys.accel_mult; t.emult = c->phys.energy_mult;
+  t.h = c->pf[F_H]; t.sound = c->df[D_SOUND]; t.div_v = c->df[D_DIVV];
+  for (int k = 0; k < 3; k++) t.a[k] = c->df[D_AX + k];
+  t.u = c->pf[F_U]; t.dudt = c->df[D_DUDT]; t.dt_out = c->df[D_DT]; t.part = c->red;
+  const int nb = std::min(nblocks(c->N, BS), RED_BLOCKS);
+  k_timestep<<<nb, BS, 0, c->stream>>>(t);
+  k_min_final<<<1, BS, 0, c->stream>>>(nb, c->red, c->red + RED_BLOCKS*ND);
+  GDF_HIP(c, hipMemcpyAsync(c->h_red, c->red + RED_BLOCKS*ND, sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  *dtmin = c->h_red[0];
+  if (!(*dtmin > 0.0) || *dtmin > 1.0e20)
+    return fail(c, GDF_ERR_TIMESTEP, "Error : Hydro timestep too large");
+  return GDF_OK;
+}
+
+int integ_max_h(gdf_ctx* c, double* hmax) {
+  const int nb = std::min(nblocks(c->N, BS), RED_BLOCKS);
+  k_max_h<<<nb, BS, 0, c->stream>>>(c->N, c->pf[F_H], c->red);
+  GDF_HIP(c, hipMemcpyAsync(c->h_red, c->red, sizeof(double)*nb, hipMemcpyDeviceToHost,
+                            c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  double best = 0.0;
+  for (int b = 0; b < nb; b++) best = std::max(best, c->h_red[b]);
+  *hmax = best;
+  return GDF_OK;
+}
+
+int integ_diagnostics(gdf_ctx* c, double* out) {
+  DiagArgs d;
+  d.N = c->N;
+  for (int k = 0; k < 3; k++) { d.r[k] = c->pf[F_X + k]; d.v[k] = c->pf[F_VX + k]; }
+  d.m = c->pf[F_M]; d.u = c->pf[F_U]; d.gpot = c->df[D_GPOT]; d.part = c->red;
+  const int nb = std::min(nblocks(c->N, BS), RED_BLOCKS);
+  k_diag<<<nb, BS, 0, c->stream>>>(d);
+  GDF_HIP(c, hipMemcpyAsync(c->h_red, c->red, sizeof(double)*ND*nb, hipMemcpyDeviceToHost,
+                            c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  double acc[ND] = {};
+  for (int b = 0; b < nb; b++) for (int k = 0; k < ND; k++) acc[k] += c->h_red[b*ND + k];
+  const double ketot = 0.5*acc[1], gpetot = 0.5*acc[3];
+  double Etot = ketot;
+  if (c->phys.hydro_forces) Etot += acc[2];
+  if (c->phys.self_gravity) Etot += gpetot;
+  out[0] = Etot; out[1] = ketot; out[2] = acc[2]; out[3] = gpetot;
+  for (int k = 0; k < 3; k++) { out[4 + k] = acc[4 + k]; out[7 + k] = acc[7 + k]; }
+  out[10] = acc[0];
+  return GDF_OK;
+}
+
+int integ_zero_forces(gdf_ctx* c) {
+  const int n = c->N;
+  const int fields[] = {D_AX, D_AY, D_AZ, D_ATX, D_ATY, D_ATZ, D_DUDT, D_DIVV, D_GPOT};
+  for (int f : fields) k_zero_d<<<nblocks(n, BS), BS, 0, c->stream>>>(n, c->df[f]);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int integ_add_atree(gdf_ctx* c) {
+  k_add_atree<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, c->df[D_AX], c->df[D_AY],
+                                                        c->df[D_AZ], c->df[D_ATX], c->df[D_ATY],
+                                                        c->df[D_ATZ]);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int integ_pack_posm(gdf_ctx* c) {
+  k_pack_posm<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, c->pf[F_X], c->pf[F_Y],
+                                                        c->pf[F_Z], c->pf[F_M], c->posm);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+}  // namespace gdf

@@ -1,0 +1,166 @@
+// Internal declarations shared by the HIP translation units of libgandalf_gpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "../../include/gandalf_gpu.h"
+
+namespace gdf {
+
+// Walk groups hold at most one wavefront of targets.
+constexpr int kWave = 64;
+constexpr int kMaxGroup = 64;
+// Resident walk slots (one wavefront each); walk kernels grab groups from a counter.
+constexpr int kWalkSlots = 4096;
+// Per-slot stack capacity of the tree walks (node indices).
+constexpr int kStackCap = 8192;
+
+// Per-particle persistent fields, permuted with the particles at every tree build.
+enum PField {
+  F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_X0, F_Y0, F_Z0, F_VX0, F_VY0, F_VZ0,
+  F_AX0, F_AY0, F_AZ0, F_M, F_H, F_U, F_U0, F_DUDT0, F_NPERSIST
+};
+// Derived per-particle fields, recomputed every step after the permutation.
+enum DField {
+  D_AX, D_AY, D_AZ, D_ATX, D_ATY, D_ATZ, D_RHO, D_DUDT, D_DIVV, D_GPOT, D_INVOMEGA,
+  D_ZETA, D_SOUND, D_PRESSURE, D_HFACTOR, D_HSEARCH, D_DT, D_NDERIVED
+};
+
+// Smoothing-kernel description passed by value to device code.
+struct KernelDesc {
+  int type;            // 0 M4, 1 quintic
+  int tabulated;
+  double range, rangesqd, norm, resinv, resinvsqd;
+  int res;
+  const double* tab;   // 9*res doubles (device), ORC order: w0 w1 womega wzeta wgrav wpot w0_s2 womega_s2 wzeta_s2
+  const float* tabf;   // float copy for the mixed path
+};
+
+struct PhysParams {
+  KernelDesc kern;
+  double h_fac, h_converge, gamma, gammam1, temp0, mu_bar, invrho_bary;
+  int eos, energy_integration;
+  double alpha_visc, beta_visc;
+  int periodic;
+  double boxsize[3], boxhalf[3], boxmin[3], boxmax[3];
+  double thetamaxsqd;
+  int nleaf;
+  double courant_mult, accel_mult, energy_mult;
+  int hydro_forces, self_gravity;
+};
+
+// Tree node arrays in the unified index (internal 0..N-2, leaf l -> N-1+l).
+struct TreeView {
+  int N;
+  const int* left; const int* right; const int* first; const int* last;
+  const double4* geo_c;   // (cx, cy, cz, hmax)
+  const double4* geo_h;   // (hx, hy, hz, cdistsqd)
+  const double* mp;       // 10 doubles per node: comx comy comz m q0..q4 pad
+};
+
+}  // namespace gdf
+
+struct gdf_ctx {
+  gdf_config cfg;
+  gdf::PhysParams phys;
+  std::string err;
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  int N = 0, Ncap = 0;
+  double* pf[gdf::F_NPERSIST] = {};      // persistent fields (sorted order)
+  double* pf_alt[gdf::F_NPERSIST] = {};  // permutation targets
+  double* df[gdf::D_NDERIVED] = {};      // derived fields
+  int* iorig = nullptr; int* iorig_alt = nullptr;
+  double4* posm = nullptr;               // packed (x, y, z, m) for neighbour gathers
+  double4* hrecA = nullptr;              // hydro records: (vx, vy, vz, h)
+  double4* hrecB = nullptr;              //                (rho, P*invomega/rho^2, sound, hfactor)
+  double4* grec = nullptr;               // gravity records: (h, zeta, hfactor, 0)
+  int* status = nullptr;
+
+  // tree
+  uint64_t* keys = nullptr; uint64_t* keys_alt = nullptr;
+  int* vals = nullptr; int* vals_alt = nullptr;
+  void* sort_tmp = nullptr; size_t sort_tmp_bytes = 0;
+  int* left = nullptr; int* right = nullptr; int* first = nullptr; int* last = nullptr;
+  int* parent = nullptr; int* visit = nullptr;
+  double* bbmin = nullptr; double* bbmax = nullptr;     // 3 per node
+  double4* geo_c = nullptr; double4* geo_h = nullptr;
+  double* mp = nullptr;                                // 10 per node
+  int* gstart = nullptr;                               // per-slot group-start marker
+  int* grp_first = nullptr; int* grp_count = nullptr; int* grp_node = nullptr;
+  int* nsel = nullptr; void* sel_tmp = nullptr; size_t sel_tmp_bytes = 0;
+  int ngroups = 0;
+  bool tree_valid = false;
+
+  // neighbour lists, interleaved: entry k of particle i at k*N + i
+  int cap_g = 0, cap_s = 0;
+  int* list_g = nullptr; int* nn_g = nullptr;
+  int* list_s = nullptr; int* nn_s = nullptr;
+  bool sg_valid = false;
+
+  // walk scratch
+  int* stacks = nullptr;
+  int* counters = nullptr;     // [0..7] device counters/flags
+  int* h_counters = nullptr;   // pinned mirror
+  long long* icount = nullptr; // [8] interaction counters
+  int32_t* grav_nc = nullptr; int32_t* grav_nn = nullptr;
+  double* red = nullptr;       // reduction partials
+  double* h_red = nullptr;     // pinned
+  double* tmpd = nullptr;      // scratch N*3 doubles for field export
+
+  // kernel tables
+  double* ktab = nullptr; float* ktabf = nullptr;
+
+  // simulation driver state (SphSimulation)
+  double t = 0.0, timestep = 0.0, tlast = 0.0;
+  int64_t nsteps = 0;
+  bool setup_done = false;
+
+  // timing
+  bool timing = false;
+  hipEvent_t ev[12] = {};
+  double pass_ms[5] = {};
+  long long icounts_last[6] = {};
+};
+
+namespace gdf {
+
+int fail(gdf_ctx* c, int code, const std::string& msg);
+int check_hip(gdf_ctx* c, hipError_t e, const char* what);
+#define GDF_HIP(c, call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return gdf::check_hip((c), e_, #call); } while (0)
+#define GDF_TRY(call) do { int rc_ = (call); if (rc_ != GDF_OK) return rc_; } while (0)
+
+// tree (gdf_tree.hip)
+int tree_alloc(gdf_ctx* c);
+int tree_build(gdf_ctx* c);
+int tree_refit_h(gdf_ctx* c);
+TreeView tree_view(const gdf_ctx* c);
+
+// SPH passes (gdf_sph.hip)
+int sph_alloc_lists(gdf_ctx* c);
+int sph_density(gdf_ctx* c, int nopbc);
+int sph_build_sg(gdf_ctx* c);
+int sph_hydro(gdf_ctx* c, int variant);
+
+// gravity (gdf_gravity.hip)
+int gravity_walk(gdf_ctx* c);
+
+// integration (gdf_integrate.hip)
+int integ_advance(gdf_ctx* c, double dt);
+int integ_wrap(gdf_ctx* c);
+int integ_end_step(gdf_ctx* c, double dtstep);
+int integ_timestep(gdf_ctx* c, double* dtmin);
+int integ_diagnostics(gdf_ctx* c, double* out);
+int integ_max_h(gdf_ctx* c, double* hmax);
+int integ_zero_forces(gdf_ctx* c);
+int integ_add_atree(gdf_ctx* c);
+int integ_pack_posm(gdf_ctx* c);
+
+inline int nblocks(long long n, int bs) { return (int) ((n + bs - 1) / bs); }
+
+}  // namespace gdf

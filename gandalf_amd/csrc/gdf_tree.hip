@@ -1,0 +1,391 @@
+// Device tree: Morton keys, radix sort, LBVH topology, bottom-up cell moments, walk groups.
+//
+// Replaces KDTree::BuildTree / StockTree (src/Tree/KDTree.cpp:220-313, :760-1083) behind
+// HydroTree::BuildTree (src/Tree/HydroTree.cpp:310-372).  Instead of median QuickSelect splits
+// the particles are sorted along a 63-bit Morton curve and the binary radix tree over the sorted
+// keys (Karras 2012) supplies the cells.  Cell moments follow the reference's bottom-up stock:
+// mass, centre of mass, traceless quadrupole with the parallel-axis merge (KDTree.cpp:950-1051),
+// bounding box, hmax and the geometric-MAC distance cdistsqd = max(rmax^2, hmax^2)/theta^2
+// (KDTree.cpp:917,997).  The particle arrays are physically permuted into key order so every
+// later pass streams coalesced SoA data.
+#include <hip/hip_runtime.h>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include "gdf_internal.h"
+#include "gdf_device.h"
+
+namespace gdf {
+namespace {
+
+constexpr int BS = 256;
+
+__global__ void k_bounds_partial(int N, const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, double* __restrict__ part) {
+  __shared__ double s[6][BS];
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = blockIdx.x*BS + threadIdx.x; i < N; i += gridDim.x*BS) {
+    const double v[3] = {x[i], y[i], z[i]};
+    for (int k = 0; k < 3; k++) { lo[k] = fmin(lo[k], v[k]); hi[k] = fmax(hi[k], v[k]); }
+  }
+  for (int k = 0; k < 3; k++) { s[k][threadIdx.x] = lo[k]; s[3+k][threadIdx.x] = hi[k]; }
+  __syncthreads();
+  for (int w = BS/2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+        s[3+k][threadIdx.x] = fmax(s[3+k][threadIdx.x], s[3+k][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x*6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// out: lo[3], scale
+__global__ void k_bounds_final(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fmin(lo[k], part[b*6 + k]);
+      hi[k] = fmax(hi[k], part[b*6 + 3 + k]);
+    }
+  double span = hi[0] - lo[0];
+  if (hi[1] - lo[1] > span) span = hi[1] - lo[1];
+  if (hi[2] - lo[2] > span) span = hi[2] - lo[2];
+  if (!(span > 0.0)) span = 1.0;
+  out[0] = lo[0]; out[1] = lo[1]; out[2] = lo[2];
+  out[3] = 2097152.0/span;
+}
+
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {
+  x &= 0x1fffffULL;
+  x = (x | (x << 32)) & 0x1f00000000ffffULL;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+  x = (x | (x << 8))  & 0x100f00f00f00f00fULL;
+  x = (x | (x << 4))  & 0x10c30c30c30c30c3ULL;
+  x = (x | (x << 2))  & 0x1249249249249249ULL;
+  return x;
+}
+
+__global__ void k_morton(int N, const double* __restrict__ x, const double* __restrict__ y,
+                         const double* __restrict__ z, const double* __restrict__ bnd,
+                         uint64_t* __restrict__ keys, int* __restrict__ vals) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= N) return;
+  const double scale = bnd[3];
+  const double v[3] = {x[i], y[i], z[i]};
+  uint64_t q[3];
+  for (int k = 0; k < 3; k++) {
+    const double f = __dmul_rn(__dadd_rn(v[k], -bnd[k]), scale);
+    q[k] = f >= 2097151.0 ? 2097151ULL : (uint64_t) f;
+  }
+  keys[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+  vals[i] = i;
+}
+
+struct PermArgs { const double* src[F_NPERSIST]; double* dst[F_NPERSIST]; };
+
+__global__ void k_permute(int N, const int* __restrict__ vals, PermArgs a,
+                          const int* __restrict__ io, int* __restrict__ io_alt) {
+  const int s = blockIdx.x*BS + threadIdx.x;
+  if (s >= N) return;
+  const int o = vals[s];
+  #pragma unroll
+  for (int f = 0; f < F_NPERSIST; f++) a.dst[f][s] = a.src[f][o];
+  io_alt[s] = io[o];
+}
+
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ key, int N, int i, long long j) {
+  if (j < 0 || j > N - 1) return -1;
+  const uint64_t a = key[i], b = key[j];
+  if (a != b) return __clzll((long long) (a ^ b));
+  return 64 + __clz((int) (i ^ (int) j));
+}
+
+// Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees, and k-d trees"
+__global__ void k_lbvh(int N, const uint64_t* __restrict__ key, int* __restrict__ left,
+                       int* __restrict__ right, int* __restrict__ first, int* __restrict__ last,
+                       int* __restrict__ parent) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= N - 1) return;
+  const int d = (delta(key, N, i, (long long) i + 1) - delta(key, N, i, (long long) i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(key, N, i, (long long) i - d);
+  long long lmax = 2;
+  while (delta(key, N, i, (long long) i + lmax*d) > dmin) lmax *= 2;
+  long long l = 0;
+  for (long long t = lmax/2; t >= 1; t /= 2)
+    if (delta(key, N, i, (long long) i + (l + t)*d) > dmin) l += t;
+  const long long j = (long long) i + l*d;
+  const int dnode = delta(key, N, i, j);
+  long long s = 0, div = 2, step;
+  do {
+    step = (l + div - 1)/div;
+    if (delta(key, N, i, (long long) i + (s + step)*d) > dnode) s += step;
+    div *= 2;
+  } while (step > 1);
+  const int gamma = (int) ((long long) i + s*d + (d < 0 ? d : 0));
+  const int f = (int) (i < j ? i : j), la = (int) (i < j ? j : i);
+  first[i] = f; last[i] = la;
+  const int L = (f == gamma) ? (N - 1 + gamma) : gamma;
+  const int R = (la == gamma + 1) ? (N - 1 + gamma + 1) : gamma + 1;
+  left[i] = L; right[i] = R;
+  parent[L] = i; parent[R] = i;
+  if (i == 0) parent[0] = -1;
+}
+
+struct NodeArgs {
+  double* bbmin; double* bbmax; double4* geo_c; double4* geo_h; double* mp;
+};
+
+__global__ void k_leaves(int N, const double* __restrict__ x, const double* __restrict__ y,
+                         const double* __restrict__ z, const double* __restrict__ m,
+                         const double* __restrict__ h, double theta2, NodeArgs a,
+                         int* __restrict__ visit, int* __restrict__ parent) {
+  const int l = blockIdx.x*BS + threadIdx.x;
+  if (l >= N) return;
+  if (l < N - 1) visit[l] = 0;
+  if (N == 1) parent[0] = -1;
+  const int n = N - 1 + l;
+  const double r[3] = {x[l], y[l], z[l]};
+  for (int k = 0; k < 3; k++) { a.bbmin[3*n+k] = r[k]; a.bbmax[3*n+k] = r[k]; a.mp[10*n+k] = r[k]; }
+  a.mp[10*n+3] = m[l];
+  for (int k = 4; k < 10; k++) a.mp[10*n+k] = 0.0;
+  const double hh = h[l];
+  a.geo_c[n] = make_double4(r[0], r[1], r[2], hh);
+  a.geo_h[n] = make_double4(0.0, 0.0, 0.0, __ddiv_rn(__dmul_rn(hh, hh), theta2));
+}
+
+// merge children (left, right) into internal node n; operation order of KDTree.cpp:950-1051
+__device__ void merge_node(int n, int L, int R, double theta2, const NodeArgs& a) {
+#pragma clang fp contract(off)
+  const double mL = a.mp[10*L+3], mR = a.mp[10*R+3];
+  const double m = mL + mR;
+  double com[3], lo[3], hi[3];
+  for (int k = 0; k < 3; k++) {
+    com[k] = m > 0 ? (mL*a.mp[10*L+k] + mR*a.mp[10*R+k])/m : 0.0;
+    lo[k] = fmin(a.bbmin[3*L+k], a.bbmin[3*R+k]);
+    hi[k] = fmax(a.bbmax[3*L+k], a.bbmax[3*R+k]);
+  }
+  const double hmax = fmax(a.geo_c[L].w, a.geo_c[R].w);
+  double q[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const int ch[2] = {L, R};
+  for (int c = 0; c < 2; c++) {
+    const int C = ch[c];
+    const double mi = a.mp[10*C+3];
+    if (!(mi > 0)) continue;
+    double dr[3];
+    for (int k = 0; k < 3; k++) dr[k] = a.mp[10*C+k] - com[k];
+    const double drsqd = dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2];
+    for (int k = 0; k < 5; k++) q[k] += a.mp[10*C+4+k];
+    q[0] += mi*(3.0*dr[0]*dr[0] - drsqd);
+    q[1] += mi*3.0*dr[0]*dr[1];
+    q[2] += mi*(3.0*dr[1]*dr[1] - drsqd);
+    q[3] += mi*3.0*dr[2]*dr[0];
+    q[4] += mi*3.0*dr[2]*dr[1];
+  }
+  for (int k = 0; k < 3; k++) { a.mp[10*n+k] = com[k]; a.bbmin[3*n+k] = lo[k]; a.bbmax[3*n+k] = hi[k]; }
+  a.mp[10*n+3] = m;
+  for (int k = 0; k < 5; k++) a.mp[10*n+4+k] = q[k];
+  double c[3], hf[3];
+  for (int k = 0; k < 3; k++) { c[k] = 0.5*(lo[k] + hi[k]); hf[k] = 0.5*(hi[k] - lo[k]); }
+  const double rmaxsqd = hf[0]*hf[0] + hf[1]*hf[1] + hf[2]*hf[2];
+  a.geo_c[n] = make_double4(c[0], c[1], c[2], hmax);
+  a.geo_h[n] = make_double4(hf[0], hf[1], hf[2], fmax(rmaxsqd, hmax*hmax)/theta2);
+}
+
+__device__ void merge_h(int n, int L, int R, double theta2, const NodeArgs& a) {
+#pragma clang fp contract(off)
+  const double hmax = fmax(a.geo_c[L].w, a.geo_c[R].w);
+  double4 gc = a.geo_c[n];
+  double4 gh = a.geo_h[n];
+  const double rmaxsqd = gh.x*gh.x + gh.y*gh.y + gh.z*gh.z;
+  gc.w = hmax;
+  gh.w = fmax(rmaxsqd, hmax*hmax)/theta2;
+  a.geo_c[n] = gc;
+  a.geo_h[n] = gh;
+}
+
+template <bool HONLY>
+__global__ void k_refit(int N, const int* __restrict__ parent, const int* __restrict__ left,
+                        const int* __restrict__ right, int* __restrict__ visit, double theta2,
+                        NodeArgs a) {
+  const int l = blockIdx.x*BS + threadIdx.x;
+  if (l >= N || N < 2) return;
+  int p = parent[N - 1 + l];
+  while (p >= 0) {
+    __threadfence();
+    const int old = __hip_atomic_fetch_add(&visit[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;          // first child to arrive; the sibling finishes the parent
+    if (HONLY) merge_h(p, left[p], right[p], theta2, a);
+    else merge_node(p, left[p], right[p], theta2, a);
+    __threadfence();
+    p = parent[p];
+  }
+}
+
+__global__ void k_zero_visit(int n, int* __restrict__ visit) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i < n) visit[i] = 0;
+}
+
+__global__ void k_hleaves(int N, const double* __restrict__ h, double theta2, NodeArgs a,
+                          int* __restrict__ visit) {
+  const int l = blockIdx.x*BS + threadIdx.x;
+  if (l >= N) return;
+  if (l < N - 1) visit[l] = 0;
+  const int n = N - 1 + l;
+  const double hh = h[l];
+  double4 gc = a.geo_c[n]; gc.w = hh; a.geo_c[n] = gc;
+  double4 gh = a.geo_h[n]; gh.w = __ddiv_rn(__dmul_rn(hh, hh), theta2); a.geo_h[n] = gh;
+}
+
+// a node is a walk group iff it holds <= gmax particles and its parent holds more
+__global__ void k_groups_mark(int N, int gmax, const int* __restrict__ parent,
+                              const int* __restrict__ first, const int* __restrict__ last,
+                              int* __restrict__ gstart, unsigned char* __restrict__ gflag) {
+  const int n = blockIdx.x*BS + threadIdx.x;
+  if (n >= 2*N - 1) return;
+  const int cnt = n >= N - 1 ? 1 : last[n] - first[n] + 1;
+  const int fst = n >= N - 1 ? n - (N - 1) : first[n];
+  if (cnt > gmax) return;
+  const int p = parent[n];
+  const int pcnt = p < 0 ? 0x7fffffff : last[p] - first[p] + 1;
+  if (p >= 0 && pcnt <= gmax) return;
+  gstart[fst] = n;
+  gflag[fst] = 1;
+}
+
+__global__ void k_groups_fill(int N, const int* __restrict__ nsel, const int* __restrict__ slots,
+                              const int* __restrict__ gstart, const int* __restrict__ first,
+                              const int* __restrict__ last, int* __restrict__ gnode,
+                              int* __restrict__ gcount) {
+  const int g = blockIdx.x*BS + threadIdx.x;
+  if (g >= *nsel) return;
+  const int s = slots[g];
+  const int n = gstart[s];
+  gnode[g] = n;
+  gcount[g] = n >= N - 1 ? 1 : last[n] - first[n] + 1;
+}
+
+}  // namespace
+
+TreeView tree_view(const gdf_ctx* c) {
+  TreeView v;
+  v.N = c->N; v.left = c->left; v.right = c->right; v.first = c->first; v.last = c->last;
+  v.geo_c = c->geo_c; v.geo_h = c->geo_h; v.mp = c->mp;
+  return v;
+}
+
+static NodeArgs node_args(gdf_ctx* c) {
+  NodeArgs a; a.bbmin = c->bbmin; a.bbmax = c->bbmax; a.geo_c = c->geo_c; a.geo_h = c->geo_h;
+  a.mp = c->mp;
+  return a;
+}
+
+int tree_alloc(gdf_ctx* c) {
+  const size_t N = (size_t) c->Ncap;
+  const size_t nn = 2*N;  // >= 2N-1
+  GDF_HIP(c, hipMalloc(&c->keys, sizeof(uint64_t)*N));
+  GDF_HIP(c, hipMalloc(&c->keys_alt, sizeof(uint64_t)*N));
+  GDF_HIP(c, hipMalloc(&c->vals, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->vals_alt, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->left, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->right, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->first, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->last, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->parent, sizeof(int)*nn));
+  GDF_HIP(c, hipMalloc(&c->visit, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->bbmin, sizeof(double)*3*nn));
+  GDF_HIP(c, hipMalloc(&c->bbmax, sizeof(double)*3*nn));
+  GDF_HIP(c, hipMalloc(&c->geo_c, sizeof(double4)*nn));
+  GDF_HIP(c, hipMalloc(&c->geo_h, sizeof(double4)*nn));
+  GDF_HIP(c, hipMalloc(&c->mp, sizeof(double)*10*nn));
+  GDF_HIP(c, hipMalloc(&c->gstart, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->grp_first, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->grp_count, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->grp_node, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->nsel, sizeof(int)*4));
+  // temporary storage of the sort and the group compaction
+  size_t b1 = 0, b2 = 0;
+  GDF_HIP(c, rocprim::radix_sort_pairs(nullptr, b1, c->keys, c->keys_alt, c->vals, c->vals_alt,
+                                       (unsigned int) N, 0, 63, c->stream));
+  GDF_HIP(c, rocprim::select(nullptr, b2, rocprim::counting_iterator<int>(0),
+                             (unsigned char*) c->visit, c->grp_first, c->nsel, N, c->stream));
+  c->sort_tmp_bytes = b1;
+  c->sel_tmp_bytes = b2;
+  GDF_HIP(c, hipMalloc(&c->sort_tmp, b1 > 0 ? b1 : 16));
+  GDF_HIP(c, hipMalloc(&c->sel_tmp, b2 > 0 ? b2 : 16));
+  return GDF_OK;
+}
+
+int tree_build(gdf_ctx* c) {
+  const int N = c->N;
+  hipStream_t s = c->stream;
+  if (N <= 0) return fail(c, GDF_ERR_INVALID, "tree build without particles");
+  const int nbp = std::min(nblocks(N, BS), 1024);
+  // 1. bounds and Morton keys
+  k_bounds_partial<<<nbp, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->red);
+  k_bounds_final<<<1, 64, 0, s>>>(nbp, c->red, c->red + 6*1024);
+  k_morton<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->red + 6*1024,
+                                         c->keys, c->vals);
+  // 2. stable radix sort of (key, index)
+  size_t bytes = c->sort_tmp_bytes;
+  GDF_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp, bytes, c->keys, c->keys_alt, c->vals,
+                                       c->vals_alt, (unsigned int) N, 0, 63, s));
+  std::swap(c->keys, c->keys_alt);
+  std::swap(c->vals, c->vals_alt);
+  // 3. permute persistent particle state into key order
+  PermArgs pa;
+  for (int f = 0; f < F_NPERSIST; f++) { pa.src[f] = c->pf[f]; pa.dst[f] = c->pf_alt[f]; }
+  k_permute<<<nblocks(N, BS), BS, 0, s>>>(N, c->vals, pa, c->iorig, c->iorig_alt);
+  for (int f = 0; f < F_NPERSIST; f++) std::swap(c->pf[f], c->pf_alt[f]);
+  std::swap(c->iorig, c->iorig_alt);
+  // 4. topology
+  if (N > 1)
+    k_lbvh<<<nblocks(N - 1, BS), BS, 0, s>>>(N, c->keys, c->left, c->right, c->first, c->last,
+                                             c->parent);
+  // 5. moments, bottom-up
+  NodeArgs na = node_args(c);
+  const double theta2 = c->phys.thetamaxsqd;
+  k_leaves<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M],
+                                         c->pf[F_H], theta2, na, c->visit, c->parent);
+  if (N > 1)
+    k_refit<false><<<nblocks(N, BS), BS, 0, s>>>(N, c->parent, c->left, c->right, c->visit,
+                                                 theta2, na);
+  // 6. walk groups in ascending first-particle order
+  GDF_HIP(c, hipMemsetAsync(c->visit, 0, sizeof(int)*((size_t) N), s));
+  k_groups_mark<<<nblocks(2*N - 1, BS), BS, 0, s>>>(N, c->cfg.group_max, c->parent, c->first,
+                                                    c->last, c->gstart,
+                                                    (unsigned char*) c->visit);
+  bytes = c->sel_tmp_bytes;
+  GDF_HIP(c, rocprim::select(c->sel_tmp, bytes, rocprim::counting_iterator<int>(0),
+                             (const unsigned char*) c->visit, c->grp_first, c->nsel, (size_t) N, s));
+  k_groups_fill<<<nblocks(N, BS), BS, 0, s>>>(N, c->nsel, c->grp_first, c->gstart, c->first,
+                                              c->last, c->grp_node, c->grp_count);
+  GDF_HIP(c, hipMemcpyAsync(c->h_counters, c->nsel, sizeof(int), hipMemcpyDeviceToHost, s));
+  GDF_HIP(c, hipStreamSynchronize(s));
+  c->ngroups = c->h_counters[0];
+  if (c->ngroups <= 0 || c->ngroups > N) return fail(c, GDF_ERR_INVALID, "walk group build failed");
+  c->tree_valid = true;
+  c->sg_valid = false;
+  GDF_HIP(c, hipGetLastError());
+  return integ_pack_posm(c);
+}
+
+int tree_refit_h(gdf_ctx* c) {
+  const int N = c->N;
+  hipStream_t s = c->stream;
+  NodeArgs na = node_args(c);
+  const double theta2 = c->phys.thetamaxsqd;
+  k_hleaves<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_H], theta2, na, c->visit);
+  if (N > 1)
+    k_refit<true><<<nblocks(N, BS), BS, 0, s>>>(N, c->parent, c->left, c->right, c->visit,
+                                                theta2, na);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+}  // namespace gdf

@@ -1,0 +1,192 @@
+// Host driver of the particle hot path; see gradhsph_simulation_gpu.h.
+#include "gradhsph_simulation_gpu.h"
+
+#include <cmath>
+#include <vector>
+
+namespace gandalf_gpu {
+
+namespace {
+
+// Sph::InitialSmoothingLengthGuess, 3D branch (src/Hydrodynamics/Sph.cpp:76-113): a uniform
+// medium filling the particles' bounding box; powf as in the reference
+double initial_h_guess(const std::vector<double>& r, double kernrange, double h_fac) {
+  const size_t N = r.size()/3;
+  double rmin[3] = {9.9e20, 9.9e20, 9.9e20}, rmax[3] = {-9.9e20, -9.9e20, -9.9e20};
+  for (size_t i = 0; i < N; i++)
+    for (int k = 0; k < 3; k++) {
+      if (r[3*i + k] < rmin[k]) rmin[k] = r[3*i + k];
+      if (r[3*i + k] > rmax[k]) rmax[k] = r[3*i + k];
+    }
+  const double pi = 3.14159265358979;
+  const int Ngather = (int) (4.0*pi*std::pow(kernrange*h_fac, 3)/3.0);
+  const double volume = (rmax[0] - rmin[0])*(rmax[1] - rmin[1])*(rmax[2] - rmin[2]);
+  return powf((3.0*volume*(double) Ngather)/(32.0*pi*(double) N), 0.33333333333333333333333);
+}
+
+void record(gdf_ctx* c, int e) {
+  if (c->timing) hipEventRecord(c->ev[e], c->stream);
+}
+
+}  // namespace
+
+template <int ndim>
+int GradhSphTreeGpu<ndim>::BuildTree(bool, int, int, int, double) {
+  return gdf::tree_build(ctx);
+}
+
+template <int ndim>
+int GradhSphTreeGpu<ndim>::UpdateAllSphProperties(bool nopbc) {
+  return gdf::sph_density(ctx, nopbc ? 1 : 0);
+}
+
+template <int ndim>
+int GradhSphTreeGpu<ndim>::UpdateAllSphHydroForces() {
+  GDF_TRY(gdf::integ_zero_forces(ctx));
+  GDF_TRY(gdf::sph_build_sg(ctx));
+  return gdf::sph_hydro(ctx, 0);
+}
+
+template <int ndim>
+int GradhSphTreeGpu<ndim>::UpdateAllSphForces() {
+  GDF_TRY(gdf::integ_zero_forces(ctx));
+  if (ctx->phys.hydro_forces) {
+    GDF_TRY(gdf::sph_build_sg(ctx));
+    GDF_TRY(gdf::sph_hydro(ctx, 1));
+  }
+  record(ctx, 3);
+  GDF_TRY(gdf::gravity_walk(ctx));
+  return gdf::integ_add_atree(ctx);
+}
+
+template <int ndim>
+int GradhSphTreeGpu<ndim>::GetMaximumSmoothingLength(double* hmax) {
+  return gdf::integ_max_h(ctx, hmax);
+}
+
+template <int ndim>
+int SphLeapfrogKDKGpu<ndim>::AdvanceParticles(int, double t, double) {
+  return gdf::integ_advance(ctx, t - ctx->tlast);
+}
+
+template <int ndim>
+int SphLeapfrogKDKGpu<ndim>::CheckBoundaries() {
+  return gdf::integ_wrap(ctx);
+}
+
+template <int ndim>
+int SphLeapfrogKDKGpu<ndim>::EndTimestep(int, double t, double timestep) {
+  GDF_TRY(gdf::integ_end_step(ctx, timestep));
+  ctx->tlast = t;
+  return GDF_OK;
+}
+
+template <int ndim>
+int GradhSphSimulationGpu<ndim>::ComputeGlobalTimestep(double* dt) {
+  return gdf::integ_timestep(ctx, dt);
+}
+
+template <int ndim>
+int GradhSphSimulationGpu<ndim>::ComputeForces() {
+  if (ctx->phys.self_gravity) return sphneib.UpdateAllSphForces();
+  if (ctx->phys.hydro_forces) {
+    GDF_TRY(sphneib.UpdateAllSphHydroForces());
+    record(ctx, 3);
+    return GDF_OK;
+  }
+  return gdf::fail(ctx, GDF_ERR_INVALID, "Error: No forces included in simulation");
+}
+
+template <int ndim>
+int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
+  gdf_ctx* c = ctx;
+  // initial smoothing lengths (Sph::InitialSmoothingLengthGuess), then the three h-rho passes
+  // of the reference setup: before periodic ghosts exist, after them, and in the force loop
+  std::vector<double> r(3*(size_t) c->N);
+  GDF_TRY(gdf_get_field(c, "r", r.data()));
+  const double hg = initial_h_guess(r, c->phys.kern.range, c->phys.h_fac);
+  std::vector<double> h((size_t) c->N, hg);
+  GDF_TRY(gdf_set_field(c, "h", h.data()));
+  GDF_TRY(sphneib.BuildTree(true, 0, 1, 1, 0.0));
+  GDF_TRY(sphneib.UpdateAllSphProperties(c->phys.periodic != 0));
+  GDF_TRY(sphneib.BuildTree(true, 0, 1, 1, 0.0));
+  GDF_TRY(sphneib.UpdateAllSphProperties(false));
+  GDF_TRY(sphneib.BuildTree(true, 0, 1, 1, 0.0));
+  GDF_TRY(sphneib.UpdateAllSphProperties(false));
+  GDF_TRY(ComputeForces());
+  c->t = 0.0;
+  c->tlast = 0.0;
+  GDF_TRY(ComputeGlobalTimestep(&c->timestep));
+  // EndTimestep with the particles' initial dt = 0 sets r0, v0, a0, u0, dudt0
+  GDF_TRY(hydroint.EndTimestep(0, c->t, 0.0));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  c->nsteps = 0;
+  c->setup_done = true;
+  return GDF_OK;
+}
+
+template <int ndim>
+int GradhSphSimulationGpu<ndim>::MainLoop() {
+  gdf_ctx* c = ctx;
+  if (!c->setup_done) return gdf::fail(c, GDF_ERR_INVALID, "MainLoop before setup");
+  GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
+  record(c, 0);
+  c->nsteps++;
+  const double dtstep = c->timestep;
+  c->t = c->t + c->timestep;
+  GDF_TRY(hydroint.AdvanceParticles((int) c->nsteps, c->t, c->timestep));
+  GDF_TRY(hydroint.CheckBoundaries());
+  record(c, 1);
+  GDF_TRY(sphneib.BuildTree(true, (int) c->nsteps, 1, 1, c->timestep));
+  record(c, 2);
+  GDF_TRY(sphneib.UpdateAllSphProperties(false));
+  record(c, 5);
+  GDF_TRY(ComputeForces());
+  record(c, 4);
+  double dtnew = 0.0;
+  GDF_TRY(ComputeGlobalTimestep(&dtnew));
+  GDF_TRY(hydroint.EndTimestep((int) c->nsteps, c->t, dtstep));
+  record(c, 6);
+  c->timestep = dtnew;
+  GDF_HIP(c, hipMemcpyAsync(c->h_red + 16000, c->icount, sizeof(long long)*6,
+                            hipMemcpyDeviceToHost, c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  const long long* ic = reinterpret_cast<const long long*>(c->h_red + 16000);
+  for (int k = 0; k < 6; k++) c->icounts_last[k] = ic[k];
+  if (c->timing) {
+    float ms = 0.f;
+    // tree, density, hydro, gravity, integrate
+    hipEventElapsedTime(&ms, c->ev[1], c->ev[2]); c->pass_ms[0] = ms;
+    hipEventElapsedTime(&ms, c->ev[2], c->ev[5]); c->pass_ms[1] = ms;
+    hipEventElapsedTime(&ms, c->ev[5], c->ev[3]); c->pass_ms[2] = ms;
+    hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); c->pass_ms[3] = ms;
+    float a = 0.f, b = 0.f;
+    hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+    hipEventElapsedTime(&b, c->ev[4], c->ev[6]);
+    c->pass_ms[4] = a + b;
+  }
+  return GDF_OK;
+}
+
+template class GradhSphTreeGpu<3>;
+template class SphLeapfrogKDKGpu<3>;
+template class GradhSphSimulationGpu<3>;
+
+}  // namespace gandalf_gpu
+
+extern "C" {
+
+int gdf_setup_simulation(gdf_ctx* c) {
+  if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  gandalf_gpu::GradhSphSimulationGpu<3> sim(c);
+  return sim.PostInitialConditionsSetup();
+}
+
+int gdf_main_loop(gdf_ctx* c, int nsteps) {
+  if (!c || c->N <= 0 || nsteps < 0) return GDF_ERR_INVALID;
+  gandalf_gpu::GradhSphSimulationGpu<3> sim(c);
+  for (int s = 0; s < nsteps; s++) GDF_TRY(sim.MainLoop());
+  return GDF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,263 @@
+"""ctypes binding of libgandalf_gpu (include/gandalf_gpu.h).
+
+The only compute path is the HIP library: if libgandalf_gpu.so is missing or no GPU is
+visible, construction raises.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgandalf_gpu.so")
+
+GDF_KERNEL = {"m4": 0, "quintic": 1}
+GDF_EOS = {"energy_eqn": 0, "isothermal": 1, "barotropic": 2}
+PREC = {"fp64": 0, "mixed": 1}
+FIELDS3 = ("r", "v", "a", "atree", "r0", "v0", "a0")
+
+
+class GdfError(RuntimeError):
+    """Non-zero status from the C ABI (the reference raises ExceptionHandler errors)."""
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int), ("tabulated_kernel", C.c_int),
+        ("h_fac", C.c_double), ("h_converge", C.c_double),
+        ("gas_eos", C.c_int), ("gamma_eos", C.c_double), ("temp0", C.c_double),
+        ("mu_bar", C.c_double), ("rho_bary", C.c_double),
+        ("alpha_visc", C.c_double), ("beta_visc", C.c_double),
+        ("hydro_forces", C.c_int), ("self_gravity", C.c_int),
+        ("thetamaxsqd", C.c_double), ("Nleafmax", C.c_int), ("periodic", C.c_int),
+        ("boxmin", C.c_double * 3), ("boxmax", C.c_double * 3),
+        ("courant_mult", C.c_double), ("accel_mult", C.c_double), ("energy_mult", C.c_double),
+        ("precision", C.c_int), ("group_max", C.c_int),
+    ]
+
+
+_lib = None
+
+# C prototypes of every entry point in include/gandalf_gpu.h
+_vp, _i, _d, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+PROTOTYPES = {
+    "gdf_config_default": (_i, [C.POINTER(Config)]),
+    "gdf_create": (_i, [C.POINTER(Config), C.POINTER(_vp)]),
+    "gdf_destroy": (None, [_vp]),
+    "gdf_last_error": (C.c_char_p, [_vp]),
+    "gdf_set_particles": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "gdf_get_field": (_i, [_vp, C.c_char_p, _vp]),
+    "gdf_set_field": (_i, [_vp, C.c_char_p, _vp]),
+    "gdf_num_particles": (_i, [_vp]),
+    "gdf_build_tree": (_i, [_vp]),
+    "gdf_update_all_sph_properties": (_i, [_vp, _i]),
+    "gdf_update_all_sph_hydro_forces": (_i, [_vp]),
+    "gdf_update_all_sph_forces": (_i, [_vp]),
+    "gdf_maximum_smoothing_length": (_i, [_vp, C.POINTER(_d)]),
+    "gdf_advance_particles": (_i, [_vp, _d]),
+    "gdf_check_boundaries": (_i, [_vp]),
+    "gdf_end_timestep": (_i, [_vp, _d]),
+    "gdf_compute_global_timestep": (_i, [_vp, C.POINTER(_d)]),
+    "gdf_calculate_diagnostics": (_i, [_vp, _vp]),
+    "gdf_setup_simulation": (_i, [_vp]),
+    "gdf_main_loop": (_i, [_vp, _i]),
+    "gdf_time": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)]),
+    "gdf_neighbour_sets": (_i64, [_vp, _i, _vp, _vp, _i64]),
+    "gdf_tree_arrays": (_i, [_vp] + [_vp] * 15 + [C.POINTER(_i)]),
+    "gdf_interaction_counts": (_i, [_vp, _vp]),
+    "gdf_gravity_group_counts": (_i, [_vp, _vp, _vp]),
+    "gdf_pass_times": (_i, [_vp, _vp]),
+    "gdf_set_timing": (_i, [_vp, _i]),
+}
+
+
+def lib():
+    """Load libgandalf_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GdfError(f"{LIB_PATH} is missing: build it with python -m gandalf_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) -> Config:
+    """gdf_config from a reference-style parameter dict (defaults: Parameters.cpp:160-575)."""
+    c = Config()
+    lib().gdf_config_default(C.byref(c))
+    c.kernel = GDF_KERNEL[prm.get("kernel", "m4")]
+    c.tabulated_kernel = int(prm.get("tabulated_kernel", 1))
+    c.h_fac = float(prm.get("h_fac", 1.2))
+    c.h_converge = float(prm.get("h_converge", 0.01))
+    c.gas_eos = GDF_EOS[prm.get("gas_eos", "energy_eqn")]
+    c.gamma_eos = float(prm.get("gamma_eos", 1.66666666666666))
+    c.temp0 = float(prm.get("temp0", 1.0))
+    c.mu_bar = float(prm.get("mu_bar", 1.0))
+    c.rho_bary = float(prm.get("rho_bary", 1.0e-14))
+    c.alpha_visc = float(prm.get("alpha_visc", 1.0))
+    c.beta_visc = float(prm.get("beta_visc", 2.0))
+    c.hydro_forces = int(prm.get("hydro_forces", 1))
+    c.self_gravity = int(prm.get("self_gravity", 0))
+    c.thetamaxsqd = float(prm.get("thetamaxsqd", 0.1))
+    c.Nleafmax = int(prm.get("Nleafmax", 6))
+    c.periodic = 1 if prm.get("boundary_lhs[0]", "open") == "periodic" else 0
+    for k in range(3):
+        c.boxmin[k] = float(prm.get(f"boxmin[{k}]", -9.9e30))
+        c.boxmax[k] = float(prm.get(f"boxmax[{k}]", 9.9e30))
+    c.courant_mult = float(prm.get("courant_mult", 0.15))
+    c.accel_mult = float(prm.get("accel_mult", 0.3))
+    c.energy_mult = float(prm.get("energy_mult", 0.4))
+    c.precision = PREC[precision]
+    c.group_max = int(group_max)
+    return c
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class GpuSim:
+    """One device context (one HIP stream, device-resident particle state)."""
+
+    def __init__(self, params: dict, precision: str = "fp64", group_max: int = 64):
+        self.params = dict(params)
+        self.cfg = config_from_params(params, precision, group_max)
+        h = C.c_void_p()
+        rc = lib().gdf_create(C.byref(self.cfg), C.byref(h))
+        if rc != 0:
+            raise GdfError(f"gdf_create failed ({rc})")
+        self._h = h
+        self.N = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gdf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc < 0:
+            msg = lib().gdf_last_error(self._h).decode()
+            raise GdfError(f"status {rc}: {msg}")
+        return rc
+
+    # ---- particle data ----
+    def set_particles(self, r, v, m, u, h=None):
+        r, v, m, u = _d(r), _d(v), _d(m), _d(u)
+        hh = None if h is None else _d(h)
+        self.N = r.shape[0]
+        self._chk(lib().gdf_set_particles(self._h, self.N, _p(r), _p(v), _p(m), _p(hh), _p(u)))
+
+    def get(self, name):
+        w = 3 if name in FIELDS3 else 1
+        out = np.empty(self.N * w)
+        self._chk(lib().gdf_get_field(self._h, name.encode(), _p(out)))
+        return out.reshape(self.N, 3) if w == 3 else out
+
+    def set(self, name, arr):
+        self._chk(lib().gdf_set_field(self._h, name.encode(), _p(_d(arr))))
+
+    # ---- hot path ----
+    def build_tree(self):
+        self._chk(lib().gdf_build_tree(self._h))
+
+    def density(self, nopbc=False):
+        self._chk(lib().gdf_update_all_sph_properties(self._h, int(nopbc)))
+
+    def hydro_forces(self):
+        self._chk(lib().gdf_update_all_sph_hydro_forces(self._h))
+
+    def all_forces(self):
+        self._chk(lib().gdf_update_all_sph_forces(self._h))
+
+    def advance(self, dt):
+        self._chk(lib().gdf_advance_particles(self._h, dt))
+
+    def end_timestep(self, dt):
+        self._chk(lib().gdf_end_timestep(self._h, dt))
+
+    def global_timestep(self):
+        d = C.c_double()
+        self._chk(lib().gdf_compute_global_timestep(self._h, C.byref(d)))
+        return d.value
+
+    def diagnostics(self):
+        out = np.zeros(11)
+        self._chk(lib().gdf_calculate_diagnostics(self._h, _p(out)))
+        return out
+
+    def setup(self):
+        self._chk(lib().gdf_setup_simulation(self._h))
+
+    def step(self, n=1):
+        self._chk(lib().gdf_main_loop(self._h, int(n)))
+
+    def time(self):
+        t, dt, n = C.c_double(), C.c_double(), C.c_int64()
+        self._chk(lib().gdf_time(self._h, C.byref(t), C.byref(dt), C.byref(n)))
+        return t.value, dt.value, n.value
+
+    def set_timing(self, on=True):
+        self._chk(lib().gdf_set_timing(self._h, int(on)))
+
+    def pass_times(self):
+        out = np.zeros(5)
+        self._chk(lib().gdf_pass_times(self._h, _p(out)))
+        return dict(zip(("tree", "density", "hydro", "gravity", "integrate"), out))
+
+    def interaction_counts(self):
+        out = np.zeros(6, dtype=np.int64)
+        self._chk(lib().gdf_interaction_counts(self._h, _p(out)))
+        return dict(zip(("gather_pairs", "sg_pairs", "grav_pp", "grav_pc", "density_evals",
+                         "nodes"), (int(x) for x in out)))
+
+    # ---- parity taps ----
+    def neighbour_sets(self, mode):
+        off = np.zeros(self.N + 1, dtype=np.int64)
+        tot = self._chk(lib().gdf_neighbour_sets(self._h, mode, _p(off), None, 0))
+        ids = np.zeros(max(tot, 1), dtype=np.int32)
+        got = self._chk(lib().gdf_neighbour_sets(self._h, mode, _p(off), _p(ids), tot))
+        return off, ids[:got]
+
+    def tree_arrays(self):
+        N = self.N
+        nn, ni = 2 * N - 1, max(N - 1, 1)
+        a = dict(perm=np.zeros(N, np.int32), keys=np.zeros(N, np.uint64),
+                 left=np.zeros(ni, np.int32), right=np.zeros(ni, np.int32),
+                 first=np.zeros(ni, np.int32), last=np.zeros(ni, np.int32),
+                 mass=np.zeros(nn), com=np.zeros((nn, 3)), quad=np.zeros((nn, 5)),
+                 bbmin=np.zeros((nn, 3)), bbmax=np.zeros((nn, 3)), hmax=np.zeros(nn),
+                 group_first=np.zeros(N, np.int32), group_count=np.zeros(N, np.int32),
+                 group_node=np.zeros(N, np.int32))
+        ng = C.c_int()
+        order = ("perm", "keys", "left", "right", "first", "last", "mass", "com", "quad",
+                 "bbmin", "bbmax", "hmax", "group_first", "group_count", "group_node")
+        self._chk(lib().gdf_tree_arrays(self._h, *[_p(a[k]) for k in order], C.byref(ng)))
+        g = ng.value
+        for k in ("group_first", "group_count", "group_node"):
+            a[k] = a[k][:g]
+        return a
+
+    def gravity_group_counts(self):
+        a = self.tree_arrays()
+        g = a["group_first"].size
+        nc = np.zeros(g, np.int32)
+        nn = np.zeros(g, np.int32)
+        self._chk(lib().gdf_gravity_group_counts(self._h, _p(nc), _p(nn)))
+        return nc, nn

@@ -1,0 +1,144 @@
+/* gandalf_gpu.h -- C ABI of the MI355X particle hot path (grad-h SPH + tree gravity).
+ *
+ * Plain pointers and sizes only.  Host arrays are in the caller's particle order; positions,
+ * velocities and accelerations are (N,3) row-major doubles.  Every entry point returns
+ * GDF_OK (0) or a negative status; gdf_last_error() gives the message.  A context owns one
+ * HIP stream and its device memory and is not re-entrant; calls are ordered on the stream and
+ * host-visible results are synchronised before return.
+ *
+ * Each entry point names the reference interface it replaces (gandalfcode/gandalf src/).
+ */
+#ifndef GANDALF_GPU_H
+#define GANDALF_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GDF_OK = 0,
+  GDF_ERR_INVALID = -1,      /* invalid argument or call order */
+  GDF_ERR_NOMEM = -2,        /* device allocation failed */
+  GDF_ERR_HIP = -3,          /* HIP runtime error */
+  GDF_ERR_NOCONV = -4,       /* h-rho iteration did not converge (GradhSph.cpp:245-251) */
+  GDF_ERR_CAPACITY = -5,     /* neighbour/walk capacity exhausted after internal retries */
+  GDF_ERR_TIMESTEP = -6      /* invalid timestep (SphIntegration.cpp:122-129) */
+};
+
+enum { GDF_KERNEL_M4 = 0, GDF_KERNEL_QUINTIC = 1 };
+enum { GDF_EOS_ENERGY_EQN = 0, GDF_EOS_ISOTHERMAL = 1, GDF_EOS_BAROTROPIC = 2 };
+enum { GDF_PREC_FP64 = 0, GDF_PREC_MIXED = 1 };
+
+/* Run-time parameters, named after the reference's parameter keys
+ * (src/Common/Parameters.cpp:160-575). */
+typedef struct gdf_config {
+  int kernel;                 /* "kernel": m4 | quintic */
+  int tabulated_kernel;       /* "tabulated_kernel" (1000-entry tables) */
+  double h_fac, h_converge;   /* "h_fac", "h_converge" */
+  int gas_eos;                /* "gas_eos": energy_eqn | isothermal | barotropic */
+  double gamma_eos, temp0, mu_bar, rho_bary;
+  double alpha_visc, beta_visc;   /* "avisc" = mon97 */
+  int hydro_forces, self_gravity;
+  double thetamaxsqd;         /* geometric MAC opening angle squared */
+  int Nleafmax;               /* leaf bucket size of the walks */
+  int periodic;               /* all three axes periodic ("boundary_lhs/rhs" = periodic) */
+  double boxmin[3], boxmax[3];
+  double courant_mult, accel_mult, energy_mult;
+  int precision;              /* GDF_PREC_FP64 or GDF_PREC_MIXED (FP32 pair arithmetic) */
+  int group_max;              /* particles per walk group (<= 64) */
+} gdf_config;
+
+typedef struct gdf_ctx gdf_ctx;
+
+/* reference defaults (Parameters.cpp:160-575) for the fields above */
+int gdf_config_default(gdf_config* cfg);
+
+/* context lifetime; replaces GradhSphSimulation::ProcessSphParameters object creation
+ * (src/GradhSph/GradhSphSimulation.cpp:54-260) for the hot path objects */
+int gdf_create(const gdf_config* cfg, gdf_ctx** out);
+void gdf_destroy(gdf_ctx* ctx);
+const char* gdf_last_error(const gdf_ctx* ctx);
+
+/* particle import; replaces Simulation::ImportArray (src/Common/Simulation.cpp:1442-1620).
+ * h may be NULL (no initial smoothing lengths).  Resets all derived state. */
+int gdf_set_particles(gdf_ctx* ctx, int N, const double* r, const double* v, const double* m,
+                      const double* h, const double* u);
+/* read a per-particle field back in import order; replaces the host particle array reads of
+ * SphSnapshot::CopyDataFromSimulation (src/Common/SphSnapshot.cpp:340-372).  Names:
+ * r v a atree r0 v0 a0 m h rho u u0 dudt dudt0 div_v gpot invomega zeta sound pressure hfactor */
+int gdf_get_field(gdf_ctx* ctx, const char* name, double* out);
+int gdf_set_field(gdf_ctx* ctx, const char* name, const double* in);
+int gdf_num_particles(const gdf_ctx* ctx);
+
+/* ---- hot path: SphNeighbourSearch<3> (src/Headers/SphNeighbourSearch.h:77-94) ---- */
+/* HydroTree::BuildTree (src/Tree/HydroTree.cpp:310-372): Morton sort, LBVH, moments */
+int gdf_build_tree(gdf_ctx* ctx);
+/* GradhSphTree::UpdateAllSphProperties (src/GradhSph/GradhSphTree.cpp:83-271);
+ * nopbc = 1 ignores periodic images (the reference's first setup pass runs before its ghost
+ * particles exist, SphSimulation.cpp:273-281) */
+int gdf_update_all_sph_properties(gdf_ctx* ctx, int nopbc);
+/* GradhSphTree::UpdateAllSphHydroForces (src/GradhSph/GradhSphTree.cpp:280-435) */
+int gdf_update_all_sph_hydro_forces(gdf_ctx* ctx);
+/* GradhSphTree::UpdateAllSphForces (src/GradhSph/GradhSphTree.cpp:444-657) */
+int gdf_update_all_sph_forces(gdf_ctx* ctx);
+/* HydroTree::GetMaximumSmoothingLength (src/Tree/HydroTree.cpp) */
+int gdf_maximum_smoothing_length(gdf_ctx* ctx, double* hmax);
+
+/* ---- TimeIntegration<3> (src/Headers/Integration.h:48-66), SphLeapfrogKDK ---- */
+/* SphLeapfrogKDK::AdvanceParticles (src/Hydrodynamics/SphLeapfrogKDK.cpp:76-127);
+ * dt = t - tlast as the reference computes it */
+int gdf_advance_particles(gdf_ctx* ctx, double dt);
+/* TimeIntegration::CheckBoundaries, periodic wrap (src/Common/Integration.cpp:11-80) */
+int gdf_check_boundaries(gdf_ctx* ctx);
+/* SphLeapfrogKDK::EndTimestep (src/Hydrodynamics/SphLeapfrogKDK.cpp:219-272) */
+int gdf_end_timestep(gdf_ctx* ctx, double dtstep);
+/* SphIntegration::Timestep + Simulation::ComputeGlobalTimestep
+ * (SphIntegration.cpp:81-134, src/Common/Simulation.cpp:1669-1754) */
+int gdf_compute_global_timestep(gdf_ctx* ctx, double* dt_min);
+/* Simulation::CalculateDiagnostics (src/Common/SimAnalysis.hpp:52-185):
+ * out = {Etot, ketot, utot, gpetot, mom[3], angmom[3], mtot} */
+int gdf_calculate_diagnostics(gdf_ctx* ctx, double* out);
+
+/* ---- whole-step drivers: SphSimulation<3> (src/Hydrodynamics/SphSimulation.cpp) ---- */
+/* PostInitialConditionsSetup (:204-565) for initial conditions without smoothing lengths */
+int gdf_setup_simulation(gdf_ctx* ctx);
+/* MainLoop (:574-880) with global timesteps (Nlevels = 1); nsteps steps */
+int gdf_main_loop(gdf_ctx* ctx, int nsteps);
+/* simulation time, current timestep and step count */
+int gdf_time(const gdf_ctx* ctx, double* t, double* timestep, int64_t* nsteps);
+
+/* ---- parity taps (no reference counterpart: the reference's own debug verification,
+ * NeighbourManager::VerifyNeighbourList, NeighbourManager.h:574-684, checks the same sets) */
+/* exact neighbour sets after the last force pass, ascending ids per particle, import order.
+ * mode 0 = gather {j : r2 < k^2 h_i^2}; mode 1 = scatter-gather (NeighbourManager.h:521).
+ * offsets has N+1 entries.  With ids == NULL returns the total entry count; otherwise fills ids
+ * and returns the total, or GDF_ERR_CAPACITY if cap is smaller. */
+int64_t gdf_neighbour_sets(gdf_ctx* ctx, int mode, int64_t* offsets, int32_t* ids,
+                           int64_t cap);
+/* tree arrays in device (sorted) order; any pointer may be NULL.  perm: sorted slot ->
+ * import index [N]; keys [N]; left/right/first/last [N-1] internal nodes (children in the
+ * unified index: internal n, leaf l -> N-1+l); mass [2N-1], com [3(2N-1)], quad [5(2N-1)],
+ * bbmin/bbmax [3(2N-1)], hmax [2N-1]; group first/count/node [ngroups] */
+int gdf_tree_arrays(gdf_ctx* ctx, int32_t* perm, uint64_t* keys, int32_t* left,
+                    int32_t* right, int32_t* first, int32_t* last, double* mass, double* com,
+                    double* quad, double* bbmin, double* bbmax, double* hmax,
+                    int32_t* group_first, int32_t* group_count, int32_t* group_node,
+                    int* ngroups);
+/* interaction counts of the last passes, for the roofline: {gather pairs of the converged
+ * density evaluation, scatter-gather pairs, particle-particle gravity terms,
+ * particle-cell gravity terms, density evaluations incl. iterations, walk nodes visited} */
+int gdf_interaction_counts(gdf_ctx* ctx, int64_t* out6);
+/* per-group gravity interaction counts of the last gravity pass [ngroups] each */
+int gdf_gravity_group_counts(gdf_ctx* ctx, int32_t* ncell, int32_t* nnear);
+
+/* event timing of the named passes of the last step, milliseconds:
+ * {tree, density, hydro, gravity, integrate} */
+int gdf_pass_times(gdf_ctx* ctx, double* out5);
+int gdf_set_timing(gdf_ctx* ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
