@@ -26,10 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}   # MI355X dense vector peaks (FLOP/s)
-# algorithmic work per interaction (frozen accounting): density pair 45, hydro pair 75 (the
-# gravity part of a neighbour pair is charged as a direct term), direct/monopole 22,
-# quadrupole particle-cell 70
-W_DENS, W_HYDRO, W_PP, W_PC = 45, 75, 22, 70
+# algorithmic work per interaction (frozen accounting): density pair 45 (converged evaluation
+# only), hydro pair 75, hydro pair with the kernel-softened gravity of neighbours 135,
+# direct/monopole gravity term 22, quadrupole particle-cell term 70
+W_DENS, W_HYDRO, W_HYDRO_GRAV, W_PP, W_PC = 45, 75, 135, 22, 70
 
 
 def log(msg):
@@ -46,7 +46,8 @@ def run_path(ic, precision, steps, warmup):
     for w in range(warmup):
         s.step(1)
     s.set_timing(True)
-    passes = {k: 0.0 for k in ("tree", "density", "hydro", "gravity", "integrate")}
+    passes = {k: 0.0 for k in ("tree", "density", "hydro", "gravity", "integrate",
+                                "k_gravity")}
     counts = None
     per_step = []
     t_start = time.perf_counter()
@@ -69,19 +70,43 @@ def run_path(ic, precision, steps, warmup):
 
 
 def roofline(res, dtype):
-    c, p = res["counts"], res["passes"]
-    flops = W_PP * c["grav_pp"] + W_PC * c["grav_pc"]
-    ms = p["gravity"]
+    """The gravity walk kernel k_gravity dominates the step: its algorithmic FLOPs over its own
+    event-timed duration on the stream it runs on.  Algorithmic work: 22 FLOP per direct term
+    (walk particle pairs minus the scatter-gather pairs, whose softened gravity the hydro pass
+    sums) and 70 per quadrupole particle-cell term, both counted on the device.  The whole-loop
+    fraction (density 45, hydro with neighbour gravity 135, plus the walk) over the step-loop
+    wall time is reported beside it."""
+    c, p, steps = res["counts"], res["passes"], res["steps"]
+    direct = c["grav_pp"] - c["sg_pairs"]
+    flops = W_PP * direct + W_PC * c["grav_pc"]
+    ms = p["k_gravity"]
     achieved = flops / (ms * 1e-3) if ms > 0 else 0.0
     peak = PEAK[dtype]
+    loop_flops = (W_DENS * c["gather_pairs"] + W_HYDRO_GRAV * c["sg_pairs"] + flops)
+    loop_rate = loop_flops / res["wall"]
     return {"kernel": "k_gravity", "bound": "valu", "achieved": achieved / 1e12,
             "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": None,
-            "flop_per_step": flops / res["steps"],
-            "kernel_ms_per_step": ms / res["steps"]}
+            "flop_per_launch": flops / steps,
+            "launch_ms": ms / steps,
+            "loop": {"achieved": loop_rate / 1e12, "frac": loop_rate / peak,
+                     "flop_per_step": loop_flops / steps}}
 
 
-def cpu_baseline(nsample=16384, steps=1):
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size (torch's copy kernel), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters to bytes: run under rocprofv3 --pmc."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}), flush=True)
+
+
+def cpu_baseline(nsample=131072, steps=3):
     """The CPU oracle (OpenMP restatement of the reference path) on a bounded sample of the
     same workload: the same generator at nsample particles, one step after setup."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -110,7 +135,12 @@ def main():
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU path is a design note only)", file=sys.stderr)

@@ -76,14 +76,14 @@ void free_particles(gdf_ctx* c) {
   for (int f = 0; f < D_NDERIVED; f++) { hipFree(c->df[f]); c->df[f] = nullptr; }
   void* ptrs[] = {c->iorig, c->iorig_alt, c->posm, c->hrecA, c->hrecB, c->grec, c->status,
                   c->keys, c->keys_alt, c->vals, c->vals_alt, c->sort_tmp, c->left, c->right,
-                  c->first, c->last, c->parent, c->visit, c->bbmin, c->bbmax, c->geo_c,
+                  c->first, c->last, c->parent, c->visit, c->front, c->bbmin, c->bbmax, c->geo_c,
                   c->geo_h, c->mp, c->gstart, c->grp_first, c->grp_count, c->grp_node, c->nsel,
                   c->sel_tmp, c->list_g, c->nn_g, c->list_s, c->nn_s, c->stacks, c->grav_nc,
                   c->grav_nn, c->tmpd};
   for (void* p : ptrs) if (p) hipFree(p);
   c->iorig = c->iorig_alt = nullptr; c->posm = c->hrecA = c->hrecB = c->grec = nullptr;
   c->status = nullptr; c->keys = c->keys_alt = nullptr; c->vals = c->vals_alt = nullptr;
-  c->sort_tmp = nullptr; c->left = c->right = c->first = c->last = c->parent = c->visit = nullptr;
+  c->sort_tmp = nullptr; c->left = c->right = c->first = c->last = c->parent = c->visit = c->front = nullptr;
   c->bbmin = c->bbmax = nullptr; c->geo_c = c->geo_h = nullptr; c->mp = nullptr;
   c->gstart = c->grp_first = c->grp_count = c->grp_node = c->nsel = nullptr; c->sel_tmp = nullptr;
   c->list_g = c->nn_g = c->list_s = c->nn_s = c->stacks = nullptr;
@@ -427,9 +427,9 @@ int gdf_gravity_group_counts(gdf_ctx* c, int32_t* ncell, int32_t* nnear) {
   return GDF_OK;
 }
 
-int gdf_pass_times(gdf_ctx* c, double* out5) {
-  if (!c || !out5) return GDF_ERR_INVALID;
-  for (int k = 0; k < 5; k++) out5[k] = c->pass_ms[k];
+int gdf_pass_times(gdf_ctx* c, double* out6) {
+  if (!c || !out6) return GDF_ERR_INVALID;
+  for (int k = 0; k < 6; k++) out6[k] = c->pass_ms[k];
   return GDF_OK;
 }
 

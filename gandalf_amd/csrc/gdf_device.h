@@ -171,13 +171,46 @@ __device__ __forceinline__ void nearest_image(const PhysParams& p, T& dx, T& dy,
   if (dz > T(p.boxhalf[2])) dz -= T(p.boxsize[2]); else if (dz < -T(p.boxhalf[2])) dz += T(p.boxsize[2]);
 }
 
-// squared distance with the reference's operation order (no fused multiply-add), so
-// neighbour-set membership is decided exactly as the oracle decides it
+// squared distance with the reference's operation order and no fused multiply-add, so
+// neighbour-set membership is decided exactly as the oracle decides it.  (HIP's __dadd_rn and
+// __dmul_rn are plain operators that the compiler may still contract; the pragma may not.)
 __device__ __forceinline__ double dot3_rn(double a, double b, double c) {
-  return __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
+#pragma clang fp contract(off)
+  return a*a + b*b + c*c;
 }
 __device__ __forceinline__ double range2_rn(double krsqd, double h) {
-  return __dmul_rn(__dmul_rn(krsqd, h), h);
+#pragma clang fp contract(off)
+  return krsqd*h*h;
+}
+
+// FP32 squared distance of the mixed path with a fixed operation order: the scatter-gather
+// list and the gravity walk both classify a pair with it, so each pair lands in exactly one
+// of the two sums
+__device__ __forceinline__ float dot3_f(float a, float b, float c) {
+#pragma clang fp contract(off)
+  return a*a + b*b + c*c;
+}
+
+// Kernel-range box test of the gravity walk with a relative slack: a node whose box comes within
+// (1 + slack) k max(h) of the group is resolved to particles, so no pair of the scatter-gather
+// set (decided per pair in FP64 or FP32) can end up inside an accepted multipole.
+constexpr double kGravNeibSlack = 1.0 + 1.0e-4;
+
+// ---- fast reciprocal and square-root forms ----
+// FP32: the hardware approximations (about 1 ulp), for the mixed-precision pair sums
+__device__ __forceinline__ float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq_f(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float sqrt_f(float x) { return __builtin_amdgcn_sqrtf(x); }
+// FP64 1/sqrt(x) for x > 0: hardware estimate (about 2^-23 relative) refined by two Newton
+// steps y <- y + y (1 - x y^2)/2, which leaves about an ulp
+__device__ __forceinline__ double rsq_d(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  #pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double e = fma(-(x*y), y, 1.0);
+    y = fma(0.5*y, e, y);
+  }
+  return y;
 }
 
 // ---- wave primitives (wave64) ----

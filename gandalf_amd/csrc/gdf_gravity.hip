@@ -9,16 +9,20 @@
 //
 // One wavefront per walk group.  Acceptance per node, with the group's bounding box G:
 //   * kernel-range test (as the reference's neighbour-cell test, Tree.cpp:662-663): if the
-//     minimum box-box distance is below k*max(hmax_G, hmax_cell) the node is resolved to
-//     particles (bucket of <= Nleafmax particles) or opened;
+//     minimum box-box distance is below k*max(hmax_G, hmax_cell) (with a 1e-4 relative slack
+//     on the squared distance) the node is resolved to particles (bucket of <= Nleafmax
+//     particles) or opened;
 //   * geometric MAC (Tree.h:413-432) evaluated from the nearest point of G: accept the
 //     multipole iff dist(G, centre_cell)^2 >= cdistsqd = max(rmax^2, hmax^2)/theta^2, a
 //     conservative form of the reference's centre-to-centre test;
 //   * otherwise open, or send the bucket's particles to the direct list.
-// Particles always use the unified pair term: kernel-softened with the zeta corrections when
-// inside either smoothing range, Newtonian otherwise (the two agree beyond the kernel range).
-// Accepted cells and direct particles are staged 64 at a time in LDS and every lane (one target
-// particle) sweeps the batch.
+// As in the reference, the kernel-softened gravity of the scatter-gather neighbours is summed by
+// the hydro pass (GradhSph::ComputeSphHydroGravForces); the walk adds the direct 1/r^2 terms of
+// every other particle and the multipole terms, masking the neighbour pairs with the same range
+// test.  Without a hydro pass the walk evaluates the softened pairs itself.  Accepted cells and
+// direct particles are staged 64 at a time in LDS and every lane (one target particle) sweeps
+// the batch; the FP32 path sums each batch in registers and carries Kahan-compensated totals
+// across batches.
 #include <hip/hip_runtime.h>
 
 #include "gdf_internal.h"
@@ -42,7 +46,9 @@ struct GravArgs {
   PhysParams p;
 };
 
-// FP64 pair term, operation order of GradhSph.cpp:498-565 / :671-686
+// Unified FP64 pair term for walks that also own the neighbour pairs (no hydro pass): the
+// kernel-softened form of GradhSph.cpp:555-565 inside either smoothing range, the direct form
+// of GradhSphBase::ComputeDirectGravForces (:657-690) outside
 __device__ __forceinline__ void pair_fp64(const KernelDesc& k, double krsqd, double dx, double dy,
                                           double dz, double mj, double hi, double hj, double zi,
                                           double zj, double hfi, double hfj, double& ax,
@@ -61,46 +67,47 @@ __device__ __forceinline__ void pair_fp64(const KernelDesc& k, double krsqd, dou
     ax += mj*dx*paux; ay += mj*dy*paux; az += mj*dz*paux;
     pot += 0.5*mj*(invh_i*kwpot<double>(k, si) + invh_j*kwpot<double>(k, sj));
   } else {
-    const double dd = d2 + 1.0e-20;
-    const double inv = 1.0/sqrt(dd);
+    const double inv = rsq_d(d2 + 1.0e-20);
     const double inv3 = inv*inv*inv;
     ax += mj*dx*inv3; ay += mj*dy*inv3; az += mj*dz*inv3;
     pot += mj*inv;
   }
 }
 
-__device__ __forceinline__ void pair_fp32(const KernelDesc& k, float krsqd, float dx, float dy,
-                                          float dz, float mj, float hi, float hj, float zi,
-                                          float zj, float hfi, float hfj, KahanF& ax, KahanF& ay,
-                                          KahanF& az, KahanF& pot) {
-  const float d2 = dx*dx + dy*dy + dz*dz;
-  if (d2 < krsqd*hi*hi || d2 < krsqd*hj*hj) {
-    const float invh_i = 1.0f/hi, invh_j = 1.0f/hj;
-    const float drmag = sqrtf(d2 + 1.0e-20f);
-    const float inv = 1.0f/drmag;
+__device__ __forceinline__ void pair_fp32(const KernelDesc& k, float d2, bool soft, float dx,
+                                          float dy, float dz, float mj, float hi, float hj,
+                                          float zi, float zj, float hfi, float hfj, float& ax,
+                                          float& ay, float& az, float& pot) {
+  if (soft) {
+    const float invh_i = rcp_f(hi), invh_j = rcp_f(hj);
+    const float drmag = sqrt_f(d2 + 1.0e-20f);
+    const float inv = rcp_f(drmag);
     dx *= inv; dy *= inv; dz *= inv;
     const float si = drmag*invh_i, sj = drmag*invh_j;
     const float wkerni = hfi*kw1<float>(k, si);
     const float wkernj = hfj*kw1<float>(k, sj);
     const float paux = 0.5f*(invh_i*invh_i*kwgrav<float>(k, si) + zi*wkerni +
                              invh_j*invh_j*kwgrav<float>(k, sj) + zj*wkernj);
-    ax.add(mj*dx*paux); ay.add(mj*dy*paux); az.add(mj*dz*paux);
-    pot.add(0.5f*mj*(invh_i*kwpot<float>(k, si) + invh_j*kwpot<float>(k, sj)));
+    ax += mj*dx*paux; ay += mj*dy*paux; az += mj*dz*paux;
+    pot += 0.5f*mj*(invh_i*kwpot<float>(k, si) + invh_j*kwpot<float>(k, sj));
   } else {
-    const float inv = rsqrtf(d2 + 1.0e-20f);
+    const float inv = rsq_f(d2 + 1.0e-20f);
     const float inv3 = inv*inv*inv;
-    ax.add(mj*dx*inv3); ay.add(mj*dy*inv3); az.add(mj*dz*inv3);
-    pot.add(mj*inv);
+    ax += mj*dx*inv3; ay += mj*dy*inv3; az += mj*dz*inv3;
+    pot += mj*inv;
   }
 }
 
-// quadrupole cell term with dr = rp - com (NeighbourSearch.h:423-449)
+// quadrupole cell term with dr = rp - com (NeighbourSearch.h:423-449); 1/|dr| from the
+// reciprocal square root instead of a division and a square root
 template <typename T>
 __device__ __forceinline__ void quad_term(T dx, T dy, T dz, T mc, const T* q, T& ax, T& ay,
                                           T& az, T& pot) {
   const T drsqd = dx*dx + dy*dy + dz*dz + T(1.0e-20);
-  const T invdrsqd = T(1)/drsqd;
-  const T invdrmag = sqrt(invdrsqd);
+  T invdrmag;
+  if (sizeof(T) == 4) invdrmag = (T) rsq_f((float) drsqd);
+  else invdrmag = (T) rsq_d((double) drsqd);
+  const T invdrsqd = invdrmag*invdrmag;
   const T invdr5 = invdrsqd*invdrsqd*invdrmag;
   const T mono = mc*invdrsqd*invdrmag;
   const T qscalar = q[0]*dx*dx + q[2]*dy*dy - (q[0] + q[2])*dz*dz +
@@ -112,11 +119,16 @@ __device__ __forceinline__ void quad_term(T dx, T dy, T dz, T mc, const T* q, T&
   pot += mc*invdrmag + T(0.5)*qscalar*invdr5;
 }
 
-template <bool MIXED>
+// SOFT: the walk also evaluates the kernel-softened neighbour pairs (no hydro pass ran).
+// Otherwise the hydro pass has already written the softened gravity of every scatter-gather
+// pair into atree/gpot (GradhSph::ComputeSphHydroGravForces), the walk masks exactly those pairs
+// (same squared distance, same range test, so each pair lands in one of the two sums) and adds
+// the direct and multipole terms.
+template <bool MIXED, bool SOFT>
 __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
   __shared__ int pbuf[PCAP];
   __shared__ int cbuf[CCAP];
-  // staged batch: particles (x, y, z, m, h, zeta, hfactor) or cells (com x3, m, q x5)
+  // staged batch: particles (x, y, z, m, r2, h, zeta, hfactor) or cells (com x3, m, q x5)
   __shared__ double stage[9*kWave];
   double (*sd)[kWave] = reinterpret_cast<double (*)[kWave]>(stage);
   float (*sf)[kWave] = reinterpret_cast<float (*)[kWave]>(stage);
@@ -135,6 +147,8 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
     const double hmaxG = gc.w;
     double4 me = make_double4(0, 0, 0, 0), mr = make_double4(1, 0, 0, 0);
     if (has) { me = a.posm[i]; mr = a.grec[i]; }
+    const double r2i = range2_rn(krsqd, mr.x);
+    const float r2if = (float) r2i;
     // accumulators; gpot starts at the self term (m/h) wpot(0) (GradhSphTree.cpp:512)
     double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
     KahanF fx, fy, fz, fp;
@@ -166,7 +180,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
         if (node >= N - 1) { ef = node - (N - 1); ec = 1; }
         else { ef = a.first[node]; ec = a.last[node] - a.first[node] + 1; }
         const bool bucket = ec <= a.nleaf || node >= N - 1;
-        if (dbox2 < hr*hr) {
+        if (dbox2 < hr*hr*kGravNeibSlack) {
           if (bucket) near = true; else open = true;
         } else {
           const double ex = fmax(fabs(dx) - gh.x, 0.0);
@@ -229,13 +243,13 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
         __syncthreads();
         if (has) {
           if (MIXED) {
+            float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
             for (int q = 0; q < nb; q++) {
               const float qq[5] = {sf[4][q], sf[5][q], sf[6][q], sf[7][q], sf[8][q]};
-              float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
               quad_term<float>(rxi - sf[0][q], ryi - sf[1][q], rzi - sf[2][q], sf[3][q], qq, bx,
                                by, bz, bp);
-              fx.add(bx); fy.add(by); fz.add(bz); fp.add(bp);
             }
+            fx.add(bx); fy.add(by); fz.add(bz); fp.add(bp);
           } else {
             for (int q = 0; q < nb; q++) {
               const double qq[5] = {sd[4][q], sd[5][q], sd[6][q], sd[7][q], sd[8][q]};
@@ -253,31 +267,55 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
           const int j = pbuf[(phead + lane) & (PCAP - 1)];
           const double4 pj = a.posm[j];
           const double4 rj = a.grec[j];
-          double dx = pj.x - gc.x, dy = pj.y - gc.y, dz = pj.z - gc.z;
-          nearest_image<double>(a.p, dx, dy, dz);
           if (MIXED) {
+            double dx = pj.x - gc.x, dy = pj.y - gc.y, dz = pj.z - gc.z;
+            nearest_image<double>(a.p, dx, dy, dz);
             sf[0][lane] = (float) dx; sf[1][lane] = (float) dy; sf[2][lane] = (float) dz;
-            sf[3][lane] = (float) pj.w; sf[4][lane] = (float) rj.x; sf[5][lane] = (float) rj.y;
-            sf[6][lane] = (float) rj.z;
+            sf[3][lane] = (float) pj.w; sf[4][lane] = (float) range2_rn(krsqd, rj.x);
+            if (SOFT) {
+              sf[5][lane] = (float) rj.x; sf[6][lane] = (float) rj.y; sf[7][lane] = (float) rj.z;
+            }
           } else {
-            sd[0][lane] = gc.x + dx; sd[1][lane] = gc.y + dy; sd[2][lane] = gc.z + dz;
-            sd[3][lane] = pj.w; sd[4][lane] = rj.x; sd[5][lane] = rj.y; sd[6][lane] = rj.z;
+            sd[0][lane] = pj.x; sd[1][lane] = pj.y; sd[2][lane] = pj.z; sd[3][lane] = pj.w;
+            sd[4][lane] = range2_rn(krsqd, rj.x);
+            if (SOFT) { sd[5][lane] = rj.x; sd[6][lane] = rj.y; sd[7][lane] = rj.z; }
           }
         }
         __syncthreads();
         if (has) {
           if (MIXED) {
             const float hi = (float) mr.x, zi = (float) mr.y, hfi = (float) mr.z;
-            const float krs = (float) krsqd;
-            for (int q = 0; q < nb; q++)
-              pair_fp32(K, krs, sf[0][q] - rxi, sf[1][q] - ryi, sf[2][q] - rzi, sf[3][q], hi,
-                        sf[4][q], zi, sf[5][q], hfi, sf[6][q], fx, fy, fz, fp);
+            float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
+            for (int q = 0; q < nb; q++) {
+              const float dx = sf[0][q] - rxi, dy = sf[1][q] - ryi, dz = sf[2][q] - rzi;
+              const float d2 = dot3_f(dx, dy, dz);
+              const bool soft = d2 < r2if || d2 < sf[4][q];
+              if (SOFT) {
+                pair_fp32(K, d2, soft, dx, dy, dz, sf[3][q], hi, sf[5][q], zi, sf[6][q], hfi,
+                          sf[7][q], bx, by, bz, bp);
+              } else {
+                const float inv = rsq_f(d2 + 1.0e-20f);
+                const float mj = soft ? 0.f : sf[3][q];
+                const float minv3 = mj*inv*inv*inv;
+                bx += minv3*dx; by += minv3*dy; bz += minv3*dz; bp += mj*inv;
+              }
+            }
+            fx.add(bx); fy.add(by); fz.add(bz); fp.add(bp);
           } else {
             for (int q = 0; q < nb; q++) {
               double dx = sd[0][q] - me.x, dy = sd[1][q] - me.y, dz = sd[2][q] - me.z;
-              if (a.p.periodic) nearest_image<double>(a.p, dx, dy, dz);
-              pair_fp64(K, krsqd, dx, dy, dz, sd[3][q], mr.x, sd[4][q], mr.y, sd[5][q], mr.z,
-                        sd[6][q], ax, ay, az, pot);
+              nearest_image<double>(a.p, dx, dy, dz);
+              if (SOFT) {
+                pair_fp64(K, krsqd, dx, dy, dz, sd[3][q], mr.x, sd[5][q], mr.y, sd[6][q], mr.z,
+                          sd[7][q], ax, ay, az, pot);
+              } else {
+                const double d2 = dot3_rn(dx, dy, dz);
+                const bool soft = d2 < r2i || d2 < sd[4][q];
+                const double inv = rsq_d(d2 + 1.0e-20);
+                const double mj = soft ? 0.0 : sd[3][q];
+                const double minv3 = mj*inv*inv*inv;
+                ax += minv3*dx; ay += minv3*dy; az += minv3*dz; pot += mj*inv;
+              }
             }
           }
         }
@@ -291,12 +329,13 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
       continue;
     }
     if (has) {
-      if (MIXED) {
-        a.atx[i] = (double) fx.s; a.aty[i] = (double) fy.s; a.atz[i] = (double) fz.s;
-        a.gpot[i] = pot0 + (double) fp.s;
-      } else {
+      if (MIXED) { ax = (double) fx.s; ay = (double) fy.s; az = (double) fz.s; pot = (double) fp.s; }
+      if (SOFT) {
         a.atx[i] = ax; a.aty[i] = ay; a.atz[i] = az;
         a.gpot[i] = pot0 + pot;
+      } else {
+        a.atx[i] += ax; a.aty[i] += ay; a.atz[i] += az;
+        a.gpot[i] = pot0 + a.gpot[i] + pot;
       }
     }
     if (lane == 0) {
@@ -337,11 +376,24 @@ int gravity_walk(gdf_ctx* c) {
   k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
   k_zero8<<<1, 64, 0, s>>>(c->counters);
   const int grid = std::min(c->ngroups, kWalkSlots);
-  if (c->cfg.precision == GDF_PREC_MIXED) k_gravity<true><<<grid, kWave, 0, s>>>(a);
-  else k_gravity<false><<<grid, kWave, 0, s>>>(a);
+  if (c->timing) hipEventRecord(c->ev[7], s);
+  const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
+  if (c->phys.hydro_forces) {
+    if (mixed) k_gravity<true, false><<<grid, kWave, 0, s>>>(a);
+    else k_gravity<false, false><<<grid, kWave, 0, s>>>(a);
+  } else {
+    if (mixed) k_gravity<true, true><<<grid, kWave, 0, s>>>(a);
+    else k_gravity<false, true><<<grid, kWave, 0, s>>>(a);
+  }
   GDF_HIP(c, hipGetLastError());
+  if (c->timing) hipEventRecord(c->ev[8], s);
   GDF_HIP(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int)*8, hipMemcpyDeviceToHost, s));
   GDF_HIP(c, hipStreamSynchronize(s));
+  if (c->timing) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev[7], c->ev[8]);
+    c->pass_ms[5] = ms;
+  }
   if (c->h_counters[2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in gravity pass");
   return GDF_OK;
 }

@@ -77,8 +77,8 @@ struct gdf_ctx {
   double* df[gdf::D_NDERIVED] = {};      // derived fields
   int* iorig = nullptr; int* iorig_alt = nullptr;
   double4* posm = nullptr;               // packed (x, y, z, m) for neighbour gathers
-  double4* hrecA = nullptr;              // hydro records: (vx, vy, vz, h)
-  double4* hrecB = nullptr;              //                (rho, P*invomega/rho^2, sound, hfactor)
+  double4* hrecA = nullptr;              // hydro records: (vx, vy, vz, 1/h)
+  double4* hrecB = nullptr;              //                (1/rho, P*invomega/rho^2, sound, hfactor)
   double4* grec = nullptr;               // gravity records: (h, zeta, hfactor, 0)
   int* status = nullptr;
 
@@ -88,6 +88,7 @@ struct gdf_ctx {
   void* sort_tmp = nullptr; size_t sort_tmp_bytes = 0;
   int* left = nullptr; int* right = nullptr; int* first = nullptr; int* last = nullptr;
   int* parent = nullptr; int* visit = nullptr;
+  int* front = nullptr;                                // refit frontier [N] + count
   double* bbmin = nullptr; double* bbmax = nullptr;     // 3 per node
   double4* geo_c = nullptr; double4* geo_h = nullptr;
   double* mp = nullptr;                                // 10 per node
@@ -124,7 +125,7 @@ struct gdf_ctx {
   // timing
   bool timing = false;
   hipEvent_t ev[12] = {};
-  double pass_ms[5] = {};
+  double pass_ms[6] = {};
   long long icounts_last[6] = {};
 };
 

@@ -25,6 +25,7 @@ constexpr double NODE_SLACK = 1.0 + 1e-12; // node tests err on the inclusive si
 
 struct ListWalk {
   int N, ngroups, cap, nleaf, mode;         // mode 0 search (k*hs_i, inclusive), 1 scatter-gather
+  int mixed;                                // scatter-gather of the mixed path: FP32 group frame
   const int* grp_node; const int* grp_first; const int* grp_count;
   const int* first; const int* last; const int* left; const int* right;
   const double4* geo_c; const double4* geo_h;
@@ -39,14 +40,22 @@ __device__ __forceinline__ void node_range(int n, int N, const int* first, const
   else { f = first[n]; cnt = last[n] - first[n] + 1; }
 }
 
+// MODE 0: search set |r_ij|^2 <= (k hs_i)^2 (FP64); 1: scatter-gather set
+// |r_ij|^2 < (k h_i)^2 or < (k h_j)^2 (FP64, the oracle's operation order); 2: the same set
+// decided in FP32 in the group-centred frame, exactly as the mixed-precision gravity walk
+// decides it.  PER: periodic nearest images.
+template <int MODE, bool PER>
 __global__ __launch_bounds__(64) void k_build_lists(ListWalk a) {
   __shared__ int cand[CANDCAP];
-  __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave];
+  __shared__ double4 sc[kWave];      // MODE 0/1: (x, y, z, (k h_j)^2)
+  __shared__ float4 fc[kWave];       // MODE 2: group-frame (x, y, z, (k h_j)^2)
   __shared__ int sj[kWave];
   const int lane = threadIdx.x;
   int* stk = a.stacks + (size_t) blockIdx.x*kStackCap;
   const int N = a.N;
   const double krsqd = a.p.kern.rangesqd, kr = a.p.kern.range;
+  PhysParams pp = a.p;
+  pp.periodic = PER ? 1 : 0;
   for (;;) {
     const int g = wave_next_item(a.counter);
     if (g >= a.ngroups) break;
@@ -60,6 +69,8 @@ __global__ __launch_bounds__(64) void k_build_lists(ListWalk a) {
     }
     const double hG = wave_max_d(has ? hi : 0.0);
     const double4 gc = a.geo_c[G], gh = a.geo_h[G];
+    const float rxi = (float) (xi - gc.x), ryi = (float) (yi - gc.y), rzi = (float) (zi - gc.z);
+    const float r2if = (float) r2i;
     int count = 0;
     int sp = 1, head = 0, tail = 0;
     bool overflow = false;
@@ -76,12 +87,12 @@ __global__ __launch_bounds__(64) void k_build_lists(ListWalk a) {
       if (node >= 0) {
         const double4 nc = a.geo_c[node], nh = a.geo_h[node];
         double dx = nc.x - gc.x, dy = nc.y - gc.y, dz = nc.z - gc.z;
-        nearest_image<double>(a.p, dx, dy, dz);
+        nearest_image<double>(pp, dx, dy, dz);
         const double gx = fmax(fabs(dx) - nh.x - gh.x, 0.0);
         const double gy = fmax(fabs(dy) - nh.y - gh.y, 0.0);
         const double gz = fmax(fabs(dz) - nh.z - gh.z, 0.0);
         const double dbox2 = gx*gx + gy*gy + gz*gz;
-        const double rr = a.mode == 0 ? kr*hG : kr*fmax(hG, nc.w);
+        const double rr = MODE == 0 ? kr*hG : kr*fmax(hG, nc.w);
         if (dbox2 <= rr*rr*NODE_SLACK) {
           node_range(node, N, a.first, a.last, ef, ec);
           if (ec <= a.nleaf) emit = true;
@@ -108,18 +119,32 @@ __global__ __launch_bounds__(64) void k_build_lists(ListWalk a) {
         const int nb = min(tail - head, kWave);
         if (lane < nb) {
           const int j = cand[(head + lane) & (CANDCAP - 1)];
-          sj[lane] = j; sx[lane] = a.x[j]; sy[lane] = a.y[j]; sz[lane] = a.z[j];
-          sh[lane] = a.hcand[j];
+          sj[lane] = j;
+          const double r2j = range2_rn(krsqd, a.hcand[j]);
+          if (MODE == 2) {
+            double dx = a.x[j] - gc.x, dy = a.y[j] - gc.y, dz = a.z[j] - gc.z;
+            nearest_image<double>(pp, dx, dy, dz);
+            fc[lane] = make_float4((float) dx, (float) dy, (float) dz, (float) r2j);
+          } else {
+            sc[lane] = make_double4(a.x[j], a.y[j], a.z[j], r2j);
+          }
         }
         __syncthreads();
         if (has) {
           for (int q = 0; q < nb; q++) {
-            double dx = sx[q] - xi, dy = sy[q] - yi, dz = sz[q] - zi;
-            nearest_image<double>(a.p, dx, dy, dz);
-            const double d2 = dot3_rn(dx, dy, dz);
             bool in;
-            if (a.mode == 0) in = d2 <= r2i;
-            else in = (d2 < r2i) || (d2 < range2_rn(krsqd, sh[q]));
+            if (MODE == 2) {
+              const float4 c = fc[q];
+              const float d2 = dot3_f(c.x - rxi, c.y - ryi, c.z - rzi);
+              in = (d2 < r2if) || (d2 < c.w);
+            } else {
+              const double4 c = sc[q];
+              double dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+              nearest_image<double>(pp, dx, dy, dz);
+              const double d2 = dot3_rn(dx, dy, dz);
+              if (MODE == 0) in = d2 <= r2i;
+              else in = (d2 < r2i) || (d2 < c.w);
+            }
             if (in) {
               if (count < a.cap) a.list[(size_t) count*N + i] = sj[q];
               count++;
@@ -284,7 +309,7 @@ __global__ void k_zero_ints(int n, int* __restrict__ p) {
   if (i < n) p[i] = 0;
 }
 
-// hydro neighbour records
+// hydro neighbour records: A = (v, 1/h), B = (1/rho, P invomega/rho^2, sound, hfactor)
 __global__ void k_pack_hydro(int N, const double* __restrict__ vx, const double* __restrict__ vy,
                              const double* __restrict__ vz, const double* __restrict__ h,
                              const double* __restrict__ rho, const double* __restrict__ pres,
@@ -293,23 +318,25 @@ __global__ void k_pack_hydro(int N, const double* __restrict__ vx, const double*
                              double4* __restrict__ B) {
   const int i = blockIdx.x*BS + threadIdx.x;
   if (i >= N) return;
-  A[i] = make_double4(vx[i], vy[i], vz[i], h[i]);
+  A[i] = make_double4(vx[i], vy[i], vz[i], 1/h[i]);
   const double r = rho[i];
-  B[i] = make_double4(r, (pres[i]*iom[i])/(r*r), snd[i], hf[i]);
+  B[i] = make_double4(1/r, (pres[i]*iom[i])/(r*r), snd[i], hf[i]);
 }
 
 struct HydroArgs {
   int N, cap;
   const int* list; const int* nn;
   const double4* posm; const double4* A; const double4* B;
-  const double* invomega; const double* pressure;
+  const double* invomega; const double* pressure; const double* zeta;
   double* ax; double* ay; double* az; double* dudt; double* div_v;
+  double* gx; double* gy; double* gz; double* gpot;    // softened neighbour gravity (VAR 1)
   int* flags; long long* icount;
   PhysParams p;
 };
 
-// VAR 0: ComputeSphHydroForces (GradhSph.cpp:361-460); VAR 1: hydro part of
-// ComputeSphHydroGravForces (:474-585)
+// VAR 0: ComputeSphHydroForces (GradhSph.cpp:361-460); VAR 1: ComputeSphHydroGravForces
+// (:474-585), the hydro terms plus the kernel-softened gravity of the pair (into gx, gy, gz,
+// gpot; the tree walk adds the direct and multipole terms afterwards)
 template <typename T, int VAR>
 __global__ __launch_bounds__(BS) void k_hydro(HydroArgs a) {
   const int i0 = blockIdx.x*BS + threadIdx.x;
@@ -320,11 +347,14 @@ __global__ __launch_bounds__(BS) void k_hydro(HydroArgs a) {
   if (n > a.cap) { atomicMax(&a.flags[1], n); return; }
   const PhysParams& p = a.p;
   const double4 me = a.posm[i], mA = a.A[i], mB = a.B[i];
-  const T invh_i = T(1)/T(mA.w);
-  const T invrho_i = T(1)/T(mB.x);
+  const T invh_i = T(mA.w);
+  const T invrho_i = T(mB.x);
   const T pterm_i = T(mB.y), snd_i = T(mB.z), hf_i = T(mB.w);
   const T alpha = T(p.alpha_visc), beta = T(p.beta_visc);
+  const T invhsqd_i = invh_i*invh_i;
+  const T zeta_i = VAR == 1 ? T(a.zeta[i]) : T(0);
   T ax = 0, ay = 0, az = 0, du = 0, dv = 0;
+  T gx = 0, gy = 0, gz = 0, gp = 0;
   for (int k = 0; k < n; k++) {
     const int j = a.list[(size_t) k*a.N + i];
     const double4 pj = a.posm[j], Aj = a.A[j], Bj = a.B[j];
@@ -332,8 +362,8 @@ __global__ __launch_bounds__(BS) void k_hydro(HydroArgs a) {
     nearest_image<double>(p, ddx, ddy, ddz);
     T dx = T(ddx), dy = T(ddy), dz = T(ddz);
     const T mj = T(pj.w);
-    const T invh_j = T(1)/T(Aj.w);
-    const T invrho_j = T(1)/T(Bj.x);
+    const T invh_j = T(Aj.w);
+    const T invrho_j = T(Bj.x);
     T drmag, dvdr;
     if (VAR == 0) {
       drmag = sqrt(dx*dx + dy*dy + dz*dz);
@@ -343,8 +373,9 @@ __global__ __launch_bounds__(BS) void k_hydro(HydroArgs a) {
       const T inv = T(1)/drmag;
       dx *= inv; dy *= inv; dz *= inv;
     }
-    const T wkerni = hf_i*kw1<T>(p.kern, drmag*invh_i);
-    const T wkernj = T(Bj.w)*kw1<T>(p.kern, drmag*invh_j);
+    const T si = drmag*invh_i, sj = drmag*invh_j;
+    const T wkerni = hf_i*kw1<T>(p.kern, si);
+    const T wkernj = T(Bj.w)*kw1<T>(p.kern, sj);
     if (VAR == 0) {
       dvdr = T(Aj.x)*dx + T(Aj.y)*dy + T(Aj.z)*dz;
       dvdr -= T(mA.x)*dx + T(mA.y)*dy + T(mA.z)*dz;
@@ -361,18 +392,29 @@ __global__ __launch_bounds__(BS) void k_hydro(HydroArgs a) {
       du -= T(0.5)*mj*alpha*vsignal*dvdr*dvdr*winvrho;
     }
     ax += mj*dx*paux; ay += mj*dy*paux; az += mj*dz*paux;
-    if (VAR == 1) dv -= mj*dvdr*wkerni;
+    if (VAR == 1) {
+      const T pg = T(0.5)*(invhsqd_i*kwgrav<T>(p.kern, si) + zeta_i*wkerni +
+                           invh_j*invh_j*kwgrav<T>(p.kern, sj) + T(a.zeta[j])*wkernj);
+      gx += mj*dx*pg; gy += mj*dy*pg; gz += mj*dz*pg;
+      gp += T(0.5)*mj*(invh_i*kwpot<T>(p.kern, si) + invh_j*kwpot<T>(p.kern, sj));
+      dv -= mj*dvdr*wkerni;
+    }
   }
-  double dvd = (double) dv*(1.0/mB.x);
-  const double rinv = 1.0/mB.x;
+  double dvd = (double) dv*mB.x;
+  const double rinv = mB.x;
   double dud = (double) du - a.pressure[i]*dvd*rinv*a.invomega[i];
   a.ax[i] = (double) ax; a.ay[i] = (double) ay; a.az[i] = (double) az;
   a.dudt[i] = dud; a.div_v[i] = dvd;
+  if (VAR == 1) {
+    a.gx[i] = (double) gx; a.gy[i] = (double) gy; a.gz[i] = (double) gz; a.gpot[i] = (double) gp;
+  }
 }
 
-// Kahan-compensated FP32 variant of the hydro sums
+// FP32 variant: pair terms in FP32 (hardware reciprocal and square root), partial sums over
+// blocks of 16 pairs in registers, Kahan-compensated totals across blocks
 template <int VAR>
 __global__ __launch_bounds__(BS) void k_hydro_mixed(HydroArgs a) {
+  constexpr int KB = 16;
   const int i = blockIdx.x*BS + threadIdx.x;
   wave_add_ll(&a.icount[1], i < a.N ? (long long) a.nn[i] : 0LL);
   if (i >= a.N) return;
@@ -380,47 +422,67 @@ __global__ __launch_bounds__(BS) void k_hydro_mixed(HydroArgs a) {
   if (n > a.cap) { atomicMax(&a.flags[1], n); return; }
   const PhysParams& p = a.p;
   const double4 me = a.posm[i], mA = a.A[i], mB = a.B[i];
-  const float invh_i = 1.0f/(float) mA.w;
-  const float invrho_i = 1.0f/(float) mB.x;
+  const float invh_i = (float) mA.w;
+  const float invrho_i = (float) mB.x;
   const float pterm_i = (float) mB.y, snd_i = (float) mB.z, hf_i = (float) mB.w;
   const float alpha = (float) p.alpha_visc, beta = (float) p.beta_visc;
-  KahanF ax, ay, az, du, dv;
-  for (int k = 0; k < n; k++) {
-    const int j = a.list[(size_t) k*a.N + i];
-    const double4 pj = a.posm[j], Aj = a.A[j], Bj = a.B[j];
-    double ddx = pj.x - me.x, ddy = pj.y - me.y, ddz = pj.z - me.z;
-    nearest_image<double>(p, ddx, ddy, ddz);
-    float dx = (float) ddx, dy = (float) ddy, dz = (float) ddz;
-    const float mj = (float) pj.w;
-    const float invh_j = 1.0f/(float) Aj.w;
-    const float invrho_j = 1.0f/(float) Bj.x;
-    float drmag;
-    if (VAR == 0) {
-      drmag = sqrtf(dx*dx + dy*dy + dz*dz);
-      if (drmag > 0.f) { const float inv = 1.0f/drmag; dx *= inv; dy *= inv; dz *= inv; }
-    } else {
-      drmag = sqrtf(dx*dx + dy*dy + dz*dz + 1.0e-20f);
-      const float inv = 1.0f/drmag;
-      dx *= inv; dy *= inv; dz *= inv;
+  const float invhsqd_i = invh_i*invh_i;
+  const float zeta_i = VAR == 1 ? (float) a.zeta[i] : 0.f;
+  KahanF ax, ay, az, du, dv, gx, gy, gz, gp;
+  for (int k0 = 0; k0 < n; k0 += KB) {
+    const int k1 = min(n, k0 + KB);
+    float bax = 0.f, bay = 0.f, baz = 0.f, bdu = 0.f, bdv = 0.f;
+    float bgx = 0.f, bgy = 0.f, bgz = 0.f, bgp = 0.f;
+    for (int k = k0; k < k1; k++) {
+      const int j = a.list[(size_t) k*a.N + i];
+      const double4 pj = a.posm[j], Aj = a.A[j], Bj = a.B[j];
+      double ddx = pj.x - me.x, ddy = pj.y - me.y, ddz = pj.z - me.z;
+      nearest_image<double>(p, ddx, ddy, ddz);
+      float dx = (float) ddx, dy = (float) ddy, dz = (float) ddz;
+      const float mj = (float) pj.w;
+      const float invh_j = (float) Aj.w;
+      const float invrho_j = (float) Bj.x;
+      float drmag;
+      if (VAR == 0) {
+        drmag = sqrt_f(dx*dx + dy*dy + dz*dz);
+        if (drmag > 0.f) { const float inv = rcp_f(drmag); dx *= inv; dy *= inv; dz *= inv; }
+      } else {
+        drmag = sqrt_f(dx*dx + dy*dy + dz*dz + 1.0e-20f);
+        const float inv = rcp_f(drmag);
+        dx *= inv; dy *= inv; dz *= inv;
+      }
+      const float si = drmag*invh_i, sj = drmag*invh_j;
+      const float wkerni = hf_i*kw1<float>(p.kern, si);
+      const float wkernj = (float) Bj.w*kw1<float>(p.kern, sj);
+      const float dvx = (float) (Aj.x - mA.x), dvy = (float) (Aj.y - mA.y), dvz = (float) (Aj.z - mA.z);
+      const float dvdr = dvx*dx + dvy*dy + dvz*dz;
+      bdv -= mj*dvdr*wkerni;
+      float paux = pterm_i*wkerni + (float) Bj.y*wkernj;
+      if (dvdr < 0.f) {
+        const float winvrho = 0.25f*(wkerni + wkernj)*(invrho_i + invrho_j);
+        const float vsignal = snd_i + (float) Bj.z - beta*alpha*dvdr;
+        paux -= alpha*vsignal*dvdr*winvrho;
+        bdu -= 0.5f*mj*alpha*vsignal*dvdr*dvdr*winvrho;
+      }
+      bax += mj*dx*paux; bay += mj*dy*paux; baz += mj*dz*paux;
+      if (VAR == 1) {
+        const float pg = 0.5f*(invhsqd_i*kwgrav<float>(p.kern, si) + zeta_i*wkerni +
+                               invh_j*invh_j*kwgrav<float>(p.kern, sj) + (float) a.zeta[j]*wkernj);
+        bgx += mj*dx*pg; bgy += mj*dy*pg; bgz += mj*dz*pg;
+        bgp += 0.5f*mj*(invh_i*kwpot<float>(p.kern, si) + invh_j*kwpot<float>(p.kern, sj));
+      }
     }
-    const float wkerni = hf_i*kw1<float>(p.kern, drmag*invh_i);
-    const float wkernj = (float) Bj.w*kw1<float>(p.kern, drmag*invh_j);
-    const float dvx = (float) (Aj.x - mA.x), dvy = (float) (Aj.y - mA.y), dvz = (float) (Aj.z - mA.z);
-    const float dvdr = dvx*dx + dvy*dy + dvz*dz;
-    dv.add(-mj*dvdr*wkerni);
-    float paux = pterm_i*wkerni + (float) Bj.y*wkernj;
-    if (dvdr < 0.f) {
-      const float winvrho = 0.25f*(wkerni + wkernj)*(invrho_i + invrho_j);
-      const float vsignal = snd_i + (float) Bj.z - beta*alpha*dvdr;
-      paux -= alpha*vsignal*dvdr*winvrho;
-      du.add(-0.5f*mj*alpha*vsignal*dvdr*dvdr*winvrho);
-    }
-    ax.add(mj*dx*paux); ay.add(mj*dy*paux); az.add(mj*dz*paux);
+    ax.add(bax); ay.add(bay); az.add(baz); du.add(bdu); dv.add(bdv);
+    if (VAR == 1) { gx.add(bgx); gy.add(bgy); gz.add(bgz); gp.add(bgp); }
   }
-  const double dvd = (double) dv.s*(1.0/mB.x);
-  const double dud = (double) du.s - a.pressure[i]*dvd*(1.0/mB.x)*a.invomega[i];
+  const double dvd = (double) dv.s*mB.x;
+  const double dud = (double) du.s - a.pressure[i]*dvd*mB.x*a.invomega[i];
   a.ax[i] = (double) ax.s; a.ay[i] = (double) ay.s; a.az[i] = (double) az.s;
   a.dudt[i] = dud; a.div_v[i] = dvd;
+  if (VAR == 1) {
+    a.gx[i] = (double) gx.s; a.gy[i] = (double) gy.s; a.gz[i] = (double) gz.s;
+    a.gpot[i] = (double) gp.s;
+  }
 }
 
 ListWalk list_walk(gdf_ctx* c, int mode) {
@@ -437,6 +499,7 @@ ListWalk list_walk(gdf_ctx* c, int mode) {
     w.cap = c->cap_s; w.list = c->list_s; w.nn = c->nn_s;
     w.hcand = c->pf[F_H]; w.htarget = c->pf[F_H];
   }
+  w.mixed = c->cfg.precision == GDF_PREC_MIXED;
   w.stacks = c->stacks; w.counter = c->counters + 4; w.flags = c->counters;
   w.p = c->phys;
   return w;
@@ -454,7 +517,17 @@ int launch_list_walk(gdf_ctx* c, int mode, int nopbc) {
   if (nopbc) w.p.periodic = 0;
   k_zero_ints<<<1, BS, 0, c->stream>>>(8, c->counters + 4);
   const int grid = std::min(c->ngroups, kWalkSlots);
-  k_build_lists<<<grid, kWave, 0, c->stream>>>(w);
+  const int m = mode == 0 ? 0 : (w.mixed ? 2 : 1);
+  hipStream_t st = c->stream;
+  if (w.p.periodic) {
+    if (m == 0) k_build_lists<0, true><<<grid, kWave, 0, st>>>(w);
+    else if (m == 1) k_build_lists<1, true><<<grid, kWave, 0, st>>>(w);
+    else k_build_lists<2, true><<<grid, kWave, 0, st>>>(w);
+  } else {
+    if (m == 0) k_build_lists<0, false><<<grid, kWave, 0, st>>>(w);
+    else if (m == 1) k_build_lists<1, false><<<grid, kWave, 0, st>>>(w);
+    else k_build_lists<2, false><<<grid, kWave, 0, st>>>(w);
+  }
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
 }
@@ -555,8 +628,9 @@ int sph_hydro(gdf_ctx* c, int variant) {
   HydroArgs h;
   h.N = N; h.cap = c->cap_s; h.list = c->list_s; h.nn = c->nn_s;
   h.posm = c->posm; h.A = c->hrecA; h.B = c->hrecB;
-  h.invomega = c->df[D_INVOMEGA]; h.pressure = c->df[D_PRESSURE];
+  h.invomega = c->df[D_INVOMEGA]; h.pressure = c->df[D_PRESSURE]; h.zeta = c->df[D_ZETA];
   h.ax = c->df[D_AX]; h.ay = c->df[D_AY]; h.az = c->df[D_AZ];
+  h.gx = c->df[D_ATX]; h.gy = c->df[D_ATY]; h.gz = c->df[D_ATZ]; h.gpot = c->df[D_GPOT];
   h.dudt = c->df[D_DUDT]; h.div_v = c->df[D_DIVV]; h.flags = c->counters; h.p = c->phys;
   h.icount = c->icount;
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;

@@ -43,20 +43,32 @@ __global__ void k_bounds_partial(int N, const double* __restrict__ x, const doub
   if (threadIdx.x < 6) part[blockIdx.x*6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// out: lo[3], scale
-__global__ void k_bounds_final(int nb, const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// out: lo[3], scale; one block of BS threads folds the per-block partials
+__global__ __launch_bounds__(BS) void k_bounds_final(int nb, const double* __restrict__ part,
+                                                     double* __restrict__ out) {
+  __shared__ double s[6][BS];
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int b = 0; b < nb; b++)
+  for (int b = threadIdx.x; b < nb; b += BS)
     for (int k = 0; k < 3; k++) {
       lo[k] = fmin(lo[k], part[b*6 + k]);
       hi[k] = fmax(hi[k], part[b*6 + 3 + k]);
     }
-  double span = hi[0] - lo[0];
-  if (hi[1] - lo[1] > span) span = hi[1] - lo[1];
-  if (hi[2] - lo[2] > span) span = hi[2] - lo[2];
+  for (int k = 0; k < 3; k++) { s[k][threadIdx.x] = lo[k]; s[3+k][threadIdx.x] = hi[k]; }
+  __syncthreads();
+  for (int w = BS/2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+        s[3+k][threadIdx.x] = fmax(s[3+k][threadIdx.x], s[3+k][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double span = s[3][0] - s[0][0];
+  if (s[4][0] - s[1][0] > span) span = s[4][0] - s[1][0];
+  if (s[5][0] - s[2][0] > span) span = s[5][0] - s[2][0];
   if (!(span > 0.0)) span = 1.0;
-  out[0] = lo[0]; out[1] = lo[1]; out[2] = lo[2];
+  out[0] = s[0][0]; out[1] = s[1][0]; out[2] = s[2][0];
   out[3] = 2097152.0/span;
 }
 
@@ -208,27 +220,85 @@ __device__ void merge_h(int n, int L, int R, double theta2, const NodeArgs& a) {
   a.geo_h[n] = gh;
 }
 
+__global__ void k_zero_visit(int n, int* __restrict__ visit) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i < n) visit[i] = 0;
+}
+
+// Bottom-up refit in two phases.  The second child to reach a node merges it (an arrival
+// counter per node decides which), so every node is merged exactly once from finished children
+// in the fixed (left, right) operand order.  Synchronising through device-scope atomics would
+// write back and invalidate the per-XCD L2 on every tree level, so the counters stay within one
+// workgroup: phase 1 gives each workgroup a chunk of RB consecutive leaves and merges the nodes
+// whose particle range lies inside the chunk (an internal node is one end of its own range, so
+// its counter fits the chunk's LDS); the maximal such nodes are appended to a frontier list.
+// Phase 2 is one workgroup that climbs from the frontier through the nodes spanning chunks.
+constexpr int RB = 1024;
+
 template <bool HONLY>
-__global__ void k_refit(int N, const int* __restrict__ parent, const int* __restrict__ left,
-                        const int* __restrict__ right, int* __restrict__ visit, double theta2,
-                        NodeArgs a) {
-  const int l = blockIdx.x*BS + threadIdx.x;
-  if (l >= N || N < 2) return;
-  int p = parent[N - 1 + l];
+__device__ __forceinline__ void merge_any(int p, const int* __restrict__ left,
+                                          const int* __restrict__ right, double theta2,
+                                          const NodeArgs& a) {
+  if (HONLY) merge_h(p, left[p], right[p], theta2, a);
+  else merge_node(p, left[p], right[p], theta2, a);
+}
+
+template <bool HONLY>
+__global__ __launch_bounds__(RB) void k_refit_chunks(
+    int N, const int* __restrict__ parent, const int* __restrict__ left,
+    const int* __restrict__ right, const int* __restrict__ first, const int* __restrict__ last,
+    double theta2, NodeArgs a, int* __restrict__ front) {
+  __shared__ int arrivals[RB];
+  const int c0 = blockIdx.x*RB;
+  const int c1 = min(c0 + RB, N) - 1;
+  arrivals[threadIdx.x] = 0;
+  __syncthreads();
+  const int l = c0 + (int) threadIdx.x;
+  if (l > c1) return;
+  int n = N - 1 + l;
+  int p = parent[n];
   while (p >= 0) {
-    __threadfence();
-    const int old = __hip_atomic_fetch_add(&visit[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;          // first child to arrive; the sibling finishes the parent
-    if (HONLY) merge_h(p, left[p], right[p], theta2, a);
-    else merge_node(p, left[p], right[p], theta2, a);
-    __threadfence();
+    if (first[p] < c0 || last[p] > c1) {
+      front[atomicAdd(&front[N], 1)] = n;
+      return;
+    }
+    const int old = __hip_atomic_fetch_add(&arrivals[p - c0], 1, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0) return;
+    merge_any<HONLY>(p, left, right, theta2, a);
+    n = p;
     p = parent[p];
   }
 }
 
-__global__ void k_zero_visit(int n, int* __restrict__ visit) {
-  const int i = blockIdx.x*BS + threadIdx.x;
-  if (i < n) visit[i] = 0;
+template <bool HONLY>
+__global__ __launch_bounds__(RB) void k_refit_top(
+    int N, const int* __restrict__ parent, const int* __restrict__ left,
+    const int* __restrict__ right, double theta2, NodeArgs a, const int* __restrict__ front,
+    int* __restrict__ visit) {
+  const int nf = front[N];
+  for (int k = threadIdx.x; k < nf; k += RB) {
+    int p = parent[front[k]];
+    while (p >= 0) {
+      const int old = __hip_atomic_fetch_add(&visit[p], 1, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;
+      merge_any<HONLY>(p, left, right, theta2, a);
+      p = parent[p];
+    }
+  }
+}
+
+template <bool HONLY>
+void launch_refit(gdf_ctx* c, const NodeArgs& na, double theta2) {
+  const int N = c->N;
+  hipStream_t s = c->stream;
+  k_zero_visit<<<1, 64, 0, s>>>(1, c->front + N);
+  k_refit_chunks<HONLY><<<(N + RB - 1)/RB, RB, 0, s>>>(N, c->parent, c->left, c->right, c->first,
+                                                       c->last, theta2, na, c->front);
+  if (N > RB)
+    k_refit_top<HONLY><<<1, RB, 0, s>>>(N, c->parent, c->left, c->right, theta2, na, c->front,
+                                        c->visit);
 }
 
 __global__ void k_hleaves(int N, const double* __restrict__ h, double theta2, NodeArgs a,
@@ -298,6 +368,7 @@ int tree_alloc(gdf_ctx* c) {
   GDF_HIP(c, hipMalloc(&c->last, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->parent, sizeof(int)*nn));
   GDF_HIP(c, hipMalloc(&c->visit, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->front, sizeof(int)*(N + 1)));
   GDF_HIP(c, hipMalloc(&c->bbmin, sizeof(double)*3*nn));
   GDF_HIP(c, hipMalloc(&c->bbmax, sizeof(double)*3*nn));
   GDF_HIP(c, hipMalloc(&c->geo_c, sizeof(double4)*nn));
@@ -328,7 +399,7 @@ int tree_build(gdf_ctx* c) {
   const int nbp = std::min(nblocks(N, BS), 1024);
   // 1. bounds and Morton keys
   k_bounds_partial<<<nbp, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->red);
-  k_bounds_final<<<1, 64, 0, s>>>(nbp, c->red, c->red + 6*1024);
+  k_bounds_final<<<1, BS, 0, s>>>(nbp, c->red, c->red + 6*1024);
   k_morton<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->red + 6*1024,
                                          c->keys, c->vals);
   // 2. stable radix sort of (key, index)
@@ -352,9 +423,7 @@ int tree_build(gdf_ctx* c) {
   const double theta2 = c->phys.thetamaxsqd;
   k_leaves<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M],
                                          c->pf[F_H], theta2, na, c->visit, c->parent);
-  if (N > 1)
-    k_refit<false><<<nblocks(N, BS), BS, 0, s>>>(N, c->parent, c->left, c->right, c->visit,
-                                                 theta2, na);
+  if (N > 1) launch_refit<false>(c, na, theta2);
   // 6. walk groups in ascending first-particle order
   GDF_HIP(c, hipMemsetAsync(c->visit, 0, sizeof(int)*((size_t) N), s));
   k_groups_mark<<<nblocks(2*N - 1, BS), BS, 0, s>>>(N, c->cfg.group_max, c->parent, c->first,
@@ -381,9 +450,7 @@ int tree_refit_h(gdf_ctx* c) {
   NodeArgs na = node_args(c);
   const double theta2 = c->phys.thetamaxsqd;
   k_hleaves<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_H], theta2, na, c->visit);
-  if (N > 1)
-    k_refit<true><<<nblocks(N, BS), BS, 0, s>>>(N, c->parent, c->left, c->right, c->visit,
-                                                theta2, na);
+  if (N > 1) launch_refit<true>(c, na, theta2);
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
 }

@@ -217,9 +217,9 @@ class GpuSim:
         self._chk(lib().gdf_set_timing(self._h, int(on)))
 
     def pass_times(self):
-        out = np.zeros(5)
+        out = np.zeros(6)
         self._chk(lib().gdf_pass_times(self._h, _p(out)))
-        return dict(zip(("tree", "density", "hydro", "gravity", "integrate"), out))
+        return dict(zip(("tree", "density", "hydro", "gravity", "integrate", "k_gravity"), out))
 
     def interaction_counts(self):
         out = np.zeros(6, dtype=np.int64)

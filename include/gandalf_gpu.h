@@ -134,8 +134,8 @@ int gdf_interaction_counts(gdf_ctx* ctx, int64_t* out6);
 int gdf_gravity_group_counts(gdf_ctx* ctx, int32_t* ncell, int32_t* nnear);
 
 /* event timing of the named passes of the last step, milliseconds:
- * {tree, density, hydro, gravity, integrate} */
-int gdf_pass_times(gdf_ctx* ctx, double* out5);
+ * {tree, density, hydro, gravity, integrate, gravity walk kernel alone} */
+int gdf_pass_times(gdf_ctx* ctx, double* out6);
 int gdf_set_timing(gdf_ctx* ctx, int enable);
 
 #ifdef __cplusplus

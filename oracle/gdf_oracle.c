@@ -1035,6 +1035,8 @@ static inline void quad_force(const double* com, const double* q, double mc,
   *gpot += mc*invdrmag + 0.5*qscalar*invdr5;
 }
 
+#define GRAV_NEIB_SLACK (1.0 + 1.0e-4)
+
 void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
                       const double* m, const double* h, const double* zeta,
                       const double* hfactor, double* atree, double* gpot,
@@ -1077,7 +1079,9 @@ void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
         }
         const double hr = kr*fmax(hmaxG, t->hmax[c]);
         int open = 0, leafadd = 0;
-        if (dbox2 < hr*hr) {
+        /* relative slack on the kernel-range box test: keeps every scatter-gather pair out of
+           accepted multipoles whatever the rounding of the per-pair test */
+        if (dbox2 < hr*hr*GRAV_NEIB_SLACK) {
           if (cnt <= nleaf || c >= N - 1 || N == 1) leafadd = 1; else open = 1;
         } else {
           double dmac2 = 0.0;

@@ -134,4 +134,10 @@ def test_mixed_precision_accuracy(case):
     a64, a32 = s64.get("a"), s32.get("a")
     e = np.linalg.norm(a32 - a64, axis=1) / np.max(np.linalg.norm(a64, axis=1))
     assert np.max(e) < 1e-4
-    assert relmax(s32.get("h"), s64.get("h")) < 1e-5
+    # h is a fixed point iterated only until successive h differ by less than h_converge
+    # (1e-2): where FP32 rounding moves a particle across that stopping test, the two paths
+    # stop one iteration apart, so single particles may differ by up to about h_converge
+    # times the contraction of the iteration.  Typical particles agree to FP32 rounding.
+    rh = np.abs(s32.get("h") / s64.get("h") - 1.0)
+    assert np.max(rh) < 2e-3
+    assert np.median(rh) < 1e-5
