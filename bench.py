@@ -86,11 +86,22 @@ def roofline(res, dtype):
     loop_rate = loop_flops / res["wall"]
     return {"kernel": "k_gravity", "bound": "valu", "achieved": achieved / 1e12,
             "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": None,
+            "traffic": measured_traffic() if dtype == "fp32" else None,
             "flop_per_launch": flops / steps,
             "launch_ms": ms / steps,
             "loop": {"achieved": loop_rate / 1e12, "frac": loop_rate / peak,
                      "flop_per_step": loop_flops / steps}}
+
+
+def measured_traffic():
+    """HBM bytes per k_gravity launch (mixed path) from the committed rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE runs, scaled by the same counters on a 1 GiB copy; None if the
+    summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_gravity_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return float(json.load(f)["bytes_per_launch"])
 
 
 def copy_calibration(mib):

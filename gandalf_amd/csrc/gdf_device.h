@@ -252,6 +252,33 @@ __device__ __forceinline__ int wave_next_item(int* counter) {
   return __builtin_amdgcn_readfirstlane(old);
 }
 
+// MI355X has 8 XCDs, each with its own L2, and workgroups are dispatched to them round-robin by
+// workgroup id.  Work that neighbours other work in memory (Morton-ordered particles, walk
+// groups) is therefore remapped so each XCD takes one contiguous eighth.
+constexpr int kXcds = 8;
+
+// logical block of physical block b in a grid of nb: XCD x = b % 8 gets the x-th contiguous
+// run of blocks (a bijection for any nb)
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int x = b % kXcds, k = b / kXcds, q = nb / kXcds, r = nb % kXcds;
+  return x*q + min(x, r) + k;
+}
+
+// Next walk group for a persistent wavefront: the groups are split into 8 contiguous ranges with
+// one counter each (counters[0..7]); a wave starts on its own XCD's range and moves to the next
+// range when that is exhausted.  `range` carries the current range between calls; returns
+// ngroups when every range is exhausted.
+__device__ __forceinline__ int wave_next_group(int* counters, int ngroups, int& range) {
+  for (int t = 0; t < kXcds; t++) {
+    const int r = (range + t) % kXcds;
+    const int lo = (int) (((long long) ngroups*r)/kXcds);
+    const int hi = (int) (((long long) ngroups*(r + 1))/kXcds);
+    const int k = wave_next_item(&counters[r]);
+    if (lo + k < hi) { range = r; return lo + k; }
+  }
+  return ngroups;
+}
+
 // Kahan-compensated accumulator for the mixed-precision pair sums
 struct KahanF {
   float s = 0.f, c = 0.f;

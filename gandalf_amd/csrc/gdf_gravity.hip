@@ -137,8 +137,9 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
   const int N = a.N;
   const KernelDesc& K = a.p.kern;
   const double kr = K.range, krsqd = K.rangesqd;
+  int range = blockIdx.x % kXcds;
   for (;;) {
-    const int g = wave_next_item(a.counter);
+    const int g = wave_next_group(a.counter, a.ngroups, range);
     if (g >= a.ngroups) break;
     const int G = a.grp_node[g], gfirst = a.grp_first[g], gcnt = a.grp_count[g];
     const bool has = lane < gcnt;
@@ -158,12 +159,12 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
     int sp = 1, phead = 0, ptail = 0, chead = 0, ctail = 0;
     int nnear = 0, ncell = 0;
     bool overflow = false;
-    if (lane == 0) __hip_atomic_store(&stk[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0) stk[0] = 0;
     __threadfence_block();
     while (sp > 0) {
       const int b = min(sp, kWave);
       int node = -1;
-      if (lane < b) node = __hip_atomic_load(&stk[sp - 1 - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane < b) node = stk[sp - 1 - lane];
       sp -= b;
       bool open = false, near = false, cell = false;
       int ef = 0, ec = 0, cl = 0, cr = 0;
@@ -201,8 +202,8 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
       if (sp + 2*nopen > kStackCap) { overflow = true; break; }
       if (open) {
         const int o = sp + 2*lanes_below(om);
-        __hip_atomic_store(&stk[o], cr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&stk[o + 1], cl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        stk[o] = cr;
+        stk[o + 1] = cl;
       }
       sp += 2*nopen;
       // accepted cells

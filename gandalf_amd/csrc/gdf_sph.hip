@@ -56,8 +56,9 @@ __global__ __launch_bounds__(64) void k_build_lists(ListWalk a) {
   const double krsqd = a.p.kern.rangesqd, kr = a.p.kern.range;
   PhysParams pp = a.p;
   pp.periodic = PER ? 1 : 0;
+  int range = blockIdx.x % kXcds;
   for (;;) {
-    const int g = wave_next_item(a.counter);
+    const int g = wave_next_group(a.counter, a.ngroups, range);
     if (g >= a.ngroups) break;
     const int G = a.grp_node[g], gfirst = a.grp_first[g], gcnt = a.grp_count[g];
     const bool has = lane < gcnt;
@@ -74,13 +75,13 @@ __global__ __launch_bounds__(64) void k_build_lists(ListWalk a) {
     int count = 0;
     int sp = 1, head = 0, tail = 0;
     bool overflow = false;
-    if (lane == 0) __hip_atomic_store(&stk[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0) stk[0] = 0;
     __threadfence_block();
     while (sp > 0) {
       const int b = min(sp, kWave);
       int node = -1;
       if (lane < b)
-        node = __hip_atomic_load(&stk[sp - 1 - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        node = stk[sp - 1 - lane];
       sp -= b;
       bool open = false, emit = false;
       int ef = 0, ec = 0, cl = 0, cr = 0;
@@ -104,8 +105,8 @@ __global__ __launch_bounds__(64) void k_build_lists(ListWalk a) {
       if (sp + 2*nopen > kStackCap) { overflow = true; break; }
       if (open) {
         const int o = sp + 2*lanes_below(om);
-        __hip_atomic_store(&stk[o], cr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&stk[o + 1], cl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        stk[o] = cr;
+        stk[o + 1] = cl;
       }
       sp += 2*nopen;
       const int ecount = emit ? ec : 0;
@@ -276,7 +277,7 @@ __device__ int density_one(const DensArgs& a, const PhysParams& p, int i, int n,
 
 template <typename T>
 __global__ __launch_bounds__(BS) void k_density(DensArgs a) {
-  const int i = blockIdx.x*BS + threadIdx.x;
+  const int i = xcd_block(blockIdx.x, gridDim.x)*BS + threadIdx.x;
   long long ng = 0, ev = 0;
   if (i < a.N && a.status[i] != 0) {
     const int n = a.nn[i];
@@ -339,7 +340,7 @@ struct HydroArgs {
 // gpot; the tree walk adds the direct and multipole terms afterwards)
 template <typename T, int VAR>
 __global__ __launch_bounds__(BS) void k_hydro(HydroArgs a) {
-  const int i0 = blockIdx.x*BS + threadIdx.x;
+  const int i0 = xcd_block(blockIdx.x, gridDim.x)*BS + threadIdx.x;
   wave_add_ll(&a.icount[1], i0 < a.N ? (long long) a.nn[i0] : 0LL);
   const int i = i0;
   if (i >= a.N) return;
@@ -415,7 +416,7 @@ __global__ __launch_bounds__(BS) void k_hydro(HydroArgs a) {
 template <int VAR>
 __global__ __launch_bounds__(BS) void k_hydro_mixed(HydroArgs a) {
   constexpr int KB = 16;
-  const int i = blockIdx.x*BS + threadIdx.x;
+  const int i = xcd_block(blockIdx.x, gridDim.x)*BS + threadIdx.x;
   wave_add_ll(&a.icount[1], i < a.N ? (long long) a.nn[i] : 0LL);
   if (i >= a.N) return;
   const int n = a.nn[i];

@@ -225,15 +225,17 @@ __global__ void k_zero_visit(int n, int* __restrict__ visit) {
   if (i < n) visit[i] = 0;
 }
 
-// Bottom-up refit in two phases.  The second child to reach a node merges it (an arrival
+// Bottom-up refit in three phases.  The second child to reach a node merges it (an arrival
 // counter per node decides which), so every node is merged exactly once from finished children
 // in the fixed (left, right) operand order.  Synchronising through device-scope atomics would
 // write back and invalidate the per-XCD L2 on every tree level, so the counters stay within one
 // workgroup: phase 1 gives each workgroup a chunk of RB consecutive leaves and merges the nodes
 // whose particle range lies inside the chunk (an internal node is one end of its own range, so
-// its counter fits the chunk's LDS); the maximal such nodes are appended to a frontier list.
-// Phase 2 is one workgroup that climbs from the frontier through the nodes spanning chunks.
-constexpr int RB = 1024;
+// its counter fits the chunk's LDS); the maximal such nodes form the chunk's frontier.  Phase 2
+// repeats this over RS chunks per workgroup starting from those frontiers, and phase 3 is one
+// workgroup that climbs the few remaining nodes to the root.
+constexpr int RB = 1024;   // leaves per phase-1 workgroup
+constexpr int RS = 32;     // phase-1 chunks per phase-2 workgroup
 
 template <bool HONLY>
 __device__ __forceinline__ void merge_any(int p, const int* __restrict__ left,
@@ -247,38 +249,83 @@ template <bool HONLY>
 __global__ __launch_bounds__(RB) void k_refit_chunks(
     int N, const int* __restrict__ parent, const int* __restrict__ left,
     const int* __restrict__ right, const int* __restrict__ first, const int* __restrict__ last,
-    double theta2, NodeArgs a, int* __restrict__ front) {
+    double theta2, NodeArgs a, int* __restrict__ front1, int* __restrict__ cnt1) {
   __shared__ int arrivals[RB];
+  __shared__ int nfront;
   const int c0 = blockIdx.x*RB;
   const int c1 = min(c0 + RB, N) - 1;
   arrivals[threadIdx.x] = 0;
+  if (threadIdx.x == 0) nfront = 0;
   __syncthreads();
   const int l = c0 + (int) threadIdx.x;
-  if (l > c1) return;
-  int n = N - 1 + l;
-  int p = parent[n];
-  while (p >= 0) {
-    if (first[p] < c0 || last[p] > c1) {
-      front[atomicAdd(&front[N], 1)] = n;
-      return;
+  if (l <= c1) {
+    int n = N - 1 + l;
+    int p = parent[n];
+    while (p >= 0) {
+      if (first[p] < c0 || last[p] > c1) {
+        // n is a maximal node of this chunk: at most RB of them
+        front1[c0 + atomicAdd(&nfront, 1)] = n;
+        break;
+      }
+      const int old = __hip_atomic_fetch_add(&arrivals[p - c0], 1, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;
+      merge_any<HONLY>(p, left, right, theta2, a);
+      n = p;
+      p = parent[p];
     }
-    const int old = __hip_atomic_fetch_add(&arrivals[p - c0], 1, __ATOMIC_ACQ_REL,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old == 0) return;
-    merge_any<HONLY>(p, left, right, theta2, a);
-    n = p;
-    p = parent[p];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) cnt1[blockIdx.x] = nfront;
+}
+
+// Phase 2: one workgroup per RS consecutive phase-1 chunks climbs from their frontier nodes
+// through the nodes inside its leaf range (every arrival at such a node comes from this
+// workgroup, so workgroup-scope counters suffice); maximal nodes go to the phase-3 frontier.
+template <bool HONLY>
+__global__ __launch_bounds__(RB) void k_refit_mid(
+    int N, int nchunks, const int* __restrict__ parent, const int* __restrict__ left,
+    const int* __restrict__ right, const int* __restrict__ first, const int* __restrict__ last,
+    double theta2, NodeArgs a, const int* __restrict__ front1, const int* __restrict__ cnt1,
+    int* __restrict__ visit, int* __restrict__ front2) {
+  __shared__ int pre[RS + 1];
+  const int k0 = blockIdx.x*RS, k1 = min(k0 + RS, nchunks);
+  const int lo = k0*RB, hi = min(k1*RB, N) - 1;
+  if (threadIdx.x == 0) {
+    pre[0] = 0;
+    for (int c = 0; c < RS; c++) pre[c + 1] = pre[c] + (k0 + c < k1 ? cnt1[k0 + c] : 0);
+  }
+  __syncthreads();
+  const int total = pre[RS];
+  for (int e = threadIdx.x; e < total; e += RB) {
+    int c = 0;
+    while (pre[c + 1] <= e) c++;
+    int n = front1[(k0 + c)*RB + (e - pre[c])];
+    int p = parent[n];
+    while (p >= 0) {
+      if (first[p] < lo || last[p] > hi) {
+        front2[atomicAdd(&front2[N], 1)] = n;
+        break;
+      }
+      const int old = __hip_atomic_fetch_add(&visit[p], 1, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;
+      merge_any<HONLY>(p, left, right, theta2, a);
+      n = p;
+      p = parent[p];
+    }
   }
 }
 
+// Phase 3: one workgroup climbs from the phase-3 frontier to the root.
 template <bool HONLY>
 __global__ __launch_bounds__(RB) void k_refit_top(
     int N, const int* __restrict__ parent, const int* __restrict__ left,
-    const int* __restrict__ right, double theta2, NodeArgs a, const int* __restrict__ front,
+    const int* __restrict__ right, double theta2, NodeArgs a, const int* __restrict__ front2,
     int* __restrict__ visit) {
-  const int nf = front[N];
+  const int nf = front2[N];
   for (int k = threadIdx.x; k < nf; k += RB) {
-    int p = parent[front[k]];
+    int p = parent[front2[k]];
     while (p >= 0) {
       const int old = __hip_atomic_fetch_add(&visit[p], 1, __ATOMIC_ACQ_REL,
                                              __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -293,12 +340,21 @@ template <bool HONLY>
 void launch_refit(gdf_ctx* c, const NodeArgs& na, double theta2) {
   const int N = c->N;
   hipStream_t s = c->stream;
-  k_zero_visit<<<1, 64, 0, s>>>(1, c->front + N);
-  k_refit_chunks<HONLY><<<(N + RB - 1)/RB, RB, 0, s>>>(N, c->parent, c->left, c->right, c->first,
-                                                       c->last, theta2, na, c->front);
-  if (N > RB)
-    k_refit_top<HONLY><<<1, RB, 0, s>>>(N, c->parent, c->left, c->right, theta2, na, c->front,
-                                        c->visit);
+  const int nch = (N + RB - 1)/RB;
+  int* front1 = c->front;                          // nch*RB entries
+  int* cnt1 = front1 + (size_t) nch*RB;            // nch counts
+  int* front2 = cnt1 + nch;                        // N entries + count
+  k_zero_visit<<<1, 64, 0, s>>>(1, front2 + N);
+  k_refit_chunks<HONLY><<<nch, RB, 0, s>>>(N, c->parent, c->left, c->right, c->first, c->last,
+                                           theta2, na, front1, cnt1);
+  if (nch > 1) {
+    const int nmid = (nch + RS - 1)/RS;
+    k_refit_mid<HONLY><<<nmid, RB, 0, s>>>(N, nch, c->parent, c->left, c->right, c->first,
+                                           c->last, theta2, na, front1, cnt1, c->visit, front2);
+    if (nmid > 1)
+      k_refit_top<HONLY><<<1, RB, 0, s>>>(N, c->parent, c->left, c->right, theta2, na, front2,
+                                          c->visit);
+  }
 }
 
 __global__ void k_hleaves(int N, const double* __restrict__ h, double theta2, NodeArgs a,
@@ -368,7 +424,8 @@ int tree_alloc(gdf_ctx* c) {
   GDF_HIP(c, hipMalloc(&c->last, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->parent, sizeof(int)*nn));
   GDF_HIP(c, hipMalloc(&c->visit, sizeof(int)*N));
-  GDF_HIP(c, hipMalloc(&c->front, sizeof(int)*(N + 1)));
+  // refit frontiers: phase 1 (one RB slot per chunk), chunk counts, phase 2 (N + count)
+  GDF_HIP(c, hipMalloc(&c->front, sizeof(int)*(2*N + 1024 + N/1024 + 16)));
   GDF_HIP(c, hipMalloc(&c->bbmin, sizeof(double)*3*nn));
   GDF_HIP(c, hipMalloc(&c->bbmax, sizeof(double)*3*nn));
   GDF_HIP(c, hipMalloc(&c->geo_c, sizeof(double4)*nn));
