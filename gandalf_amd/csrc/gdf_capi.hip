@@ -61,7 +61,7 @@ bool field_ptrs(gdf_ctx* c, const char* name, double** p, int* w) {
     {"a", 1, D_AX, 3}, {"atree", 1, D_ATX, 3}, {"rho", 1, D_RHO, 1}, {"dudt", 1, D_DUDT, 1},
     {"div_v", 1, D_DIVV, 1}, {"gpot", 1, D_GPOT, 1}, {"invomega", 1, D_INVOMEGA, 1},
     {"zeta", 1, D_ZETA, 1}, {"sound", 1, D_SOUND, 1}, {"pressure", 1, D_PRESSURE, 1},
-    {"hfactor", 1, D_HFACTOR, 1}, {"dt", 1, D_DT, 1}};
+    {"hfactor", 1, D_HFACTOR, 1}, {"dt", 1, D_DT, 1}, {"adot", 1, D_ADOTX, 3}};
   for (const E& e : tab) {
     if (strcmp(e.n, name)) continue;
     for (int k = 0; k < e.w; k++) p[k] = e.src == 0 ? c->pf[e.f + k] : c->df[e.f + k];
@@ -78,7 +78,7 @@ void free_particles(gdf_ctx* c) {
                   c->keys, c->keys_alt, c->vals, c->vals_alt, c->sort_tmp, c->left, c->right,
                   c->first, c->last, c->parent, c->visit, c->front, c->bbmin, c->bbmax, c->geo_c,
                   c->geo_h, c->mp, c->gstart, c->grp_first, c->grp_count, c->grp_node, c->nsel,
-                  c->sel_tmp, c->list_g, c->nn_g, c->list_s, c->nn_s, c->stacks, c->grav_nc,
+                  c->sel_tmp, c->stacks, c->grav_nc,
                   c->grav_nn, c->tmpd};
   for (void* p : ptrs) if (p) hipFree(p);
   c->iorig = c->iorig_alt = nullptr; c->posm = c->hrecA = c->hrecB = c->grec = nullptr;
@@ -86,7 +86,8 @@ void free_particles(gdf_ctx* c) {
   c->sort_tmp = nullptr; c->left = c->right = c->first = c->last = c->parent = c->visit = c->front = nullptr;
   c->bbmin = c->bbmax = nullptr; c->geo_c = c->geo_h = nullptr; c->mp = nullptr;
   c->gstart = c->grp_first = c->grp_count = c->grp_node = c->nsel = nullptr; c->sel_tmp = nullptr;
-  c->list_g = c->nn_g = c->list_s = c->nn_s = c->stacks = nullptr;
+  c->stacks = nullptr;
+  sph_free_lists(c);
   c->grav_nc = c->grav_nn = nullptr; c->tmpd = nullptr;
   c->Ncap = 0; c->N = 0;
 }
@@ -199,6 +200,8 @@ int gdf_config_default(gdf_config* cfg) {
   cfg->courant_mult = 0.15; cfg->accel_mult = 0.3; cfg->energy_mult = 0.4;
   cfg->precision = GDF_PREC_FP64;
   cfg->group_max = 64;
+  cfg->nbody = 0; cfg->nbody_direct = 0; cfg->nbody_eps = 0.0;
+  cfg->nbody_mult = 0.1; cfg->fixed_dt = 0.0;
   return GDF_OK;
 }
 
@@ -210,6 +213,9 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
   if (cfg->group_max < 1 || cfg->group_max > kMaxGroup) return GDF_ERR_INVALID;
   if (cfg->Nleafmax < 1 || cfg->Nleafmax > 14) return GDF_ERR_INVALID;
   if (!(cfg->thetamaxsqd > 0.0) || !(cfg->h_fac > 0.0)) return GDF_ERR_INVALID;
+  if (cfg->nbody_direct < 0 || cfg->nbody_direct > 2 || cfg->nbody_eps < 0.0 ||
+      cfg->fixed_dt < 0.0)
+    return GDF_ERR_INVALID;
   if (cfg->periodic)
     for (int k = 0; k < 3; k++)
       if (!(cfg->boxmax[k] > cfg->boxmin[k]) || cfg->boxmax[k] - cfg->boxmin[k] > 1e20)
@@ -242,6 +248,10 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
   p.courant_mult = cfg->courant_mult; p.accel_mult = cfg->accel_mult;
   p.energy_mult = cfg->energy_mult;
   p.hydro_forces = cfg->hydro_forces; p.self_gravity = cfg->self_gravity;
+  p.nbody = cfg->nbody != 0;
+  if (p.nbody) { p.hydro_forces = 0; p.self_gravity = 1; p.energy_integration = 0; }
+  p.eps2 = cfg->nbody_eps*cfg->nbody_eps;
+  p.nbody_mult = cfg->nbody_mult;
   int rc = upload_kernel_tables(c);
   if (rc == GDF_OK) {
     if (hipMalloc(&c->counters, sizeof(int)*16) != hipSuccess ||
@@ -482,43 +492,50 @@ int gdf_tree_arrays(gdf_ctx* c, int32_t* perm, uint64_t* keys, int32_t* left, in
 int64_t gdf_neighbour_sets(gdf_ctx* c, int mode, int64_t* offsets, int32_t* ids, int64_t cap) {
   if (!c || !offsets || c->N <= 0 || (mode != 0 && mode != 1)) return GDF_ERR_INVALID;
   if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "no tree");
-  if (!c->sg_valid) GDF_TRY(sph_build_sg(c));
-  GDF_HIP(c, hipStreamSynchronize(c->stream));
-  const int N = c->N;
-  const int capl = c->cap_s;
-  std::vector<int> nn(N), lst((size_t) capl*N), io(N);
+  std::vector<int> cidx, ncand;
+  std::vector<unsigned long long> cmask;
+  int capb = 0;
+  GDF_TRY(sph_copy_sg(c, cidx, cmask, ncand, capb));
+  const int N = c->N, G = c->ngroups;
+  std::vector<int> io(N), gfirst(G), gcount(G);
   std::vector<double> hh(N), x(N), y(N), z(N);
-  GDF_HIP(c, hipMemcpy(nn.data(), c->nn_s, sizeof(int)*N, hipMemcpyDeviceToHost));
-  GDF_HIP(c, hipMemcpy(lst.data(), c->list_s, sizeof(int)*(size_t) capl*N, hipMemcpyDeviceToHost));
   GDF_HIP(c, hipMemcpy(io.data(), c->iorig, sizeof(int)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(gfirst.data(), c->grp_first, sizeof(int)*G, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(gcount.data(), c->grp_count, sizeof(int)*G, hipMemcpyDeviceToHost));
   GDF_HIP(c, hipMemcpy(hh.data(), c->pf[F_H], sizeof(double)*N, hipMemcpyDeviceToHost));
   GDF_HIP(c, hipMemcpy(x.data(), c->pf[F_X], sizeof(double)*N, hipMemcpyDeviceToHost));
   GDF_HIP(c, hipMemcpy(y.data(), c->pf[F_Y], sizeof(double)*N, hipMemcpyDeviceToHost));
   GDF_HIP(c, hipMemcpy(z.data(), c->pf[F_Z], sizeof(double)*N, hipMemcpyDeviceToHost));
-  // the scatter-gather list holds exactly the mode-1 set; the gather set is its subset with
-  // r2 < k^2 h_i^2, decided with the same arithmetic as the device tests
+  // the scatter-gather candidate masks hold exactly the mode-1 set; the gather set is its
+  // subset with r2 < k^2 h_i^2, decided with the same arithmetic as the device tests
   const double krsqd = c->phys.kern.rangesqd;
   std::vector<std::vector<int>> sets(N);
-  for (int s = 0; s < N; s++) {
-    const int n = std::min(nn[s], capl);
-    const double r2i = (krsqd*hh[s])*hh[s];
-    auto& out = sets[io[s]];
-    for (int k = 0; k < n; k++) {
-      const int j = lst[(size_t) k*N + s];
-      if (mode == 0) {
-        double dx = x[j] - x[s], dy = y[j] - y[s], dz = z[j] - z[s];
-        if (c->phys.periodic) {
-          const double* hb = c->phys.boxhalf; const double* bs = c->phys.boxsize;
-          if (dx > hb[0]) dx -= bs[0]; else if (dx < -hb[0]) dx += bs[0];
-          if (dy > hb[1]) dy -= bs[1]; else if (dy < -hb[1]) dy += bs[1];
-          if (dz > hb[2]) dz -= bs[2]; else if (dz < -hb[2]) dz += bs[2];
+  for (int g = 0; g < G; g++) {
+    const size_t base = (size_t) g*capb*64;
+    const int nc = ncand[g];
+    for (int l = 0; l < gcount[g]; l++) {
+      const int s = gfirst[g] + l;
+      const double r2i = (krsqd*hh[s])*hh[s];
+      auto& out = sets[io[s]];
+      for (int k = 0; k < nc; k++) {
+        const int b = k/64, q = k%64;
+        if (!((cmask[base + (size_t) b*64 + l] >> q) & 1ULL)) continue;
+        const int j = cidx[base + (size_t) b*64 + q];
+        if (mode == 0) {
+          double dx = x[j] - x[s], dy = y[j] - y[s], dz = z[j] - z[s];
+          if (c->phys.periodic) {
+            const double* hb = c->phys.boxhalf; const double* bs = c->phys.boxsize;
+            if (dx > hb[0]) dx -= bs[0]; else if (dx < -hb[0]) dx += bs[0];
+            if (dy > hb[1]) dy -= bs[1]; else if (dy < -hb[1]) dy += bs[1];
+            if (dz > hb[2]) dz -= bs[2]; else if (dz < -hb[2]) dz += bs[2];
+          }
+          const double d2 = (dx*dx + dy*dy) + dz*dz;
+          if (!(d2 < r2i)) continue;
         }
-        const double d2 = (dx*dx + dy*dy) + dz*dz;
-        if (!(d2 < r2i)) continue;
+        out.push_back(io[j]);
       }
-      out.push_back(io[j]);
+      std::sort(out.begin(), out.end());
     }
-    std::sort(out.begin(), out.end());
   }
   int64_t total = 0;
   offsets[0] = 0;

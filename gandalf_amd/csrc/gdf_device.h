@@ -129,6 +129,10 @@ template <typename T> __device__ __forceinline__ T tab_sq(const KernelDesc& k, i
   return ktab<T>(k)[w*k.res + (int) (s2*T(k.resinvsqd))];
 }
 
+template <typename T> __device__ __forceinline__ T kw0(const KernelDesc& k, T s) {
+  if (k.tabulated) return tab_lin<T>(k, KW0, s);
+  return k.type == 0 ? m4_w0<T>(T(k.norm), s) : q_w0<T>(T(k.norm), s);
+}
 template <typename T> __device__ __forceinline__ T kw1(const KernelDesc& k, T s) {
   if (k.tabulated) return tab_lin<T>(k, KW1, s);
   return k.type == 0 ? m4_w1<T>(T(k.norm), s) : q_w1<T>(T(k.norm), s);

@@ -102,8 +102,8 @@ __device__ __forceinline__ void pair_fp32(const KernelDesc& k, float d2, bool so
 // reciprocal square root instead of a division and a square root
 template <typename T>
 __device__ __forceinline__ void quad_term(T dx, T dy, T dz, T mc, const T* q, T& ax, T& ay,
-                                          T& az, T& pot) {
-  const T drsqd = dx*dx + dy*dy + dz*dz + T(1.0e-20);
+                                          T& az, T& pot, T soft2 = T(1.0e-20)) {
+  const T drsqd = dx*dx + dy*dy + dz*dz + soft2;
   T invdrmag;
   if (sizeof(T) == 4) invdrmag = (T) rsq_f((float) drsqd);
   else invdrmag = (T) rsq_d((double) drsqd);
@@ -123,11 +123,14 @@ __device__ __forceinline__ void quad_term(T dx, T dy, T dz, T mc, const T* q, T&
 // Otherwise the hydro pass has already written the softened gravity of every scatter-gather
 // pair into atree/gpot (GradhSph::ComputeSphHydroGravForces), the walk masks exactly those pairs
 // (same squared distance, same range test, so each pair lands in one of the two sums) and adds
-// the direct and multipole terms.
-template <bool MIXED, bool SOFT>
+// the direct and multipole terms.  PLUMMER (pure N-body stars): every particle pair and every
+// multipole term is softened with the Plummer length eps, the star itself is skipped and
+// there is no self potential.
+template <bool MIXED, bool SOFT, bool PLUMMER>
 __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
   __shared__ int pbuf[PCAP];
   __shared__ int cbuf[CCAP];
+  __shared__ int sidx[kWave];
   // staged batch: particles (x, y, z, m, r2, h, zeta, hfactor) or cells (com x3, m, q x5)
   __shared__ double stage[9*kWave];
   double (*sd)[kWave] = reinterpret_cast<double (*)[kWave]>(stage);
@@ -153,7 +156,9 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
     // accumulators; gpot starts at the self term (m/h) wpot(0) (GradhSphTree.cpp:512)
     double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
     KahanF fx, fy, fz, fp;
-    const double pot0 = has ? (me.w/mr.x)*kwpot<double>(K, 0.0) : 0.0;
+    const double pot0 = (has && !PLUMMER) ? (me.w/mr.x)*kwpot<double>(K, 0.0) : 0.0;
+    const double eps2 = a.p.eps2;
+    const float eps2f = (float) eps2;
     // target position relative to the group centre (cell-relative frame of the FP32 path)
     const float rxi = (float) (me.x - gc.x), ryi = (float) (me.y - gc.y), rzi = (float) (me.z - gc.z);
     int sp = 1, phead = 0, ptail = 0, chead = 0, ctail = 0;
@@ -248,14 +253,14 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
             for (int q = 0; q < nb; q++) {
               const float qq[5] = {sf[4][q], sf[5][q], sf[6][q], sf[7][q], sf[8][q]};
               quad_term<float>(rxi - sf[0][q], ryi - sf[1][q], rzi - sf[2][q], sf[3][q], qq, bx,
-                               by, bz, bp);
+                               by, bz, bp, PLUMMER ? eps2f : 1.0e-20f);
             }
             fx.add(bx); fy.add(by); fz.add(bz); fp.add(bp);
           } else {
             for (int q = 0; q < nb; q++) {
               const double qq[5] = {sd[4][q], sd[5][q], sd[6][q], sd[7][q], sd[8][q]};
               quad_term<double>(me.x - sd[0][q], me.y - sd[1][q], me.z - sd[2][q], sd[3][q], qq,
-                                ax, ay, az, pot);
+                                ax, ay, az, pot, PLUMMER ? eps2 : 1.0e-20);
             }
           }
         }
@@ -268,6 +273,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
           const int j = pbuf[(phead + lane) & (PCAP - 1)];
           const double4 pj = a.posm[j];
           const double4 rj = a.grec[j];
+          sidx[lane] = j;
           if (MIXED) {
             double dx = pj.x - gc.x, dy = pj.y - gc.y, dz = pj.z - gc.z;
             nearest_image<double>(a.p, dx, dy, dz);
@@ -291,7 +297,12 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
               const float dx = sf[0][q] - rxi, dy = sf[1][q] - ryi, dz = sf[2][q] - rzi;
               const float d2 = dot3_f(dx, dy, dz);
               const bool soft = d2 < r2if || d2 < sf[4][q];
-              if (SOFT) {
+              if (PLUMMER) {
+                const float inv = rsq_f(d2 + eps2f);
+                const float mj = sidx[q] == i ? 0.f : sf[3][q];
+                const float minv3 = mj*inv*inv*inv;
+                bx += minv3*dx; by += minv3*dy; bz += minv3*dz; bp += mj*inv;
+              } else if (SOFT) {
                 pair_fp32(K, d2, soft, dx, dy, dz, sf[3][q], hi, sf[5][q], zi, sf[6][q], hfi,
                           sf[7][q], bx, by, bz, bp);
               } else {
@@ -306,7 +317,12 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
             for (int q = 0; q < nb; q++) {
               double dx = sd[0][q] - me.x, dy = sd[1][q] - me.y, dz = sd[2][q] - me.z;
               nearest_image<double>(a.p, dx, dy, dz);
-              if (SOFT) {
+              if (PLUMMER) {
+                const double inv = rsq_d(dot3_rn(dx, dy, dz) + eps2);
+                const double mj = sidx[q] == i ? 0.0 : sd[3][q];
+                const double minv3 = mj*inv*inv*inv;
+                ax += minv3*dx; ay += minv3*dy; az += minv3*dz; pot += mj*inv;
+              } else if (SOFT) {
                 pair_fp64(K, krsqd, dx, dy, dz, sd[3][q], mr.x, sd[5][q], mr.y, sd[6][q], mr.z,
                           sd[7][q], ax, ay, az, pot);
               } else {
@@ -348,6 +364,82 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
   }
 }
 
+struct NbDirectArgs {
+  int N;
+  const double* x; const double* y; const double* z;
+  const double* vx; const double* vy; const double* vz;
+  const double* m; const double* h;
+  double* ax; double* ay; double* az; double* adx; double* ady; double* adz; double* gpot;
+  KernelDesc kern;
+};
+
+// Star-star direct sums, one thread per star, source stars staged 256 at a time in LDS.
+// SMOOTH = false: Nbody::CalculateDirectGravForces (src/Nbody/Nbody.cpp:233-293), point
+// masses with jerk; SMOOTH = true: NbodyLeapfrogKDK::CalculateDirectSmoothedGravForces
+// (src/Nbody/NbodyLeapfrogKDK.cpp:78-150), kernel softening over the mean smoothing length,
+// whose (2/(h_i+h_j))^3 factor the reference forms in single precision (powf).
+template <bool SMOOTH>
+__global__ __launch_bounds__(256) void k_nbody_direct(NbDirectArgs a) {
+  constexpr int T = 256;
+  __shared__ double sx[T], sy[T], sz[T], svx[T], svy[T], svz[T], sm[T], sh[T];
+  const int i = blockIdx.x*T + threadIdx.x;
+  const bool has = i < a.N;
+  double xi = 0, yi = 0, zi = 0, vxi = 0, vyi = 0, vzi = 0, hi = 1.0;
+  if (has) {
+    xi = a.x[i]; yi = a.y[i]; zi = a.z[i]; vxi = a.vx[i]; vyi = a.vy[i]; vzi = a.vz[i];
+    if (SMOOTH) hi = a.h[i];
+  }
+  double ax = 0, ay = 0, az = 0, jx = 0, jy = 0, jz = 0, pot = 0;
+  for (int j0 = 0; j0 < a.N; j0 += T) {
+    const int jl = j0 + threadIdx.x;
+    if (jl < a.N) {
+      sx[threadIdx.x] = a.x[jl]; sy[threadIdx.x] = a.y[jl]; sz[threadIdx.x] = a.z[jl];
+      svx[threadIdx.x] = a.vx[jl]; svy[threadIdx.x] = a.vy[jl]; svz[threadIdx.x] = a.vz[jl];
+      sm[threadIdx.x] = a.m[jl]; sh[threadIdx.x] = SMOOTH ? a.h[jl] : 0.0;
+    }
+    __syncthreads();
+    const int nb = min(T, a.N - j0);
+    if (has) {
+      for (int q = 0; q < nb; q++) {
+        if (j0 + q == i) continue;
+        const double dx = sx[q] - xi, dy = sy[q] - yi, dz = sz[q] - zi;
+        const double dvx = svx[q] - vxi, dvy = svy[q] - vyi, dvz = svz[q] - vzi;
+        const double drsqd = dx*dx + dy*dy + dz*dz;
+        const double mj = sm[q];
+        if (!SMOOTH) {
+          const double invdrmag = 1.0/sqrt(drsqd);
+          const double drdt = (dvx*dx + dvy*dy + dvz*dz)*invdrmag;
+          const double inv3 = invdrmag*invdrmag*invdrmag;
+          pot += mj*invdrmag;
+          ax += mj*dx*inv3; ay += mj*dy*inv3; az += mj*dz*inv3;
+          const double c3 = 3.0*drdt*invdrmag;
+          jx += mj*inv3*(dvx - c3*dx); jy += mj*inv3*(dvy - c3*dy); jz += mj*inv3*(dvz - c3*dz);
+        } else {
+          const double drmag = sqrt(drsqd) + 1.0e-20;
+          const double invdrmag = 1.0/drmag;
+          const double invhmean = 2.0/(hi + sh[q]);
+          const double drdt = (dvx*dx + dvy*dy + dvz*dz)*invdrmag;
+          const double s = drmag*invhmean;
+          const double paux = mj*invhmean*invhmean*kwgrav<double>(a.kern, s)*invdrmag;
+          const double wmean = kw0<double>(a.kern, s)*(double) powf((float) invhmean, 3.0f);
+          pot += mj*invhmean*kwpot<double>(a.kern, s);
+          ax += paux*dx; ay += paux*dy; az += paux*dz;
+          const double c3 = 3.0*paux*drdt*invdrmag;
+          const double cw = 2.0*6.28318530717959*mj*drdt*wmean*invdrmag;
+          jx += paux*dvx - c3*dx + cw*dx;
+          jy += paux*dvy - c3*dy + cw*dy;
+          jz += paux*dvz - c3*dz + cw*dz;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!has) return;
+  a.ax[i] = ax; a.ay[i] = ay; a.az[i] = az;
+  a.adx[i] = jx; a.ady[i] = jy; a.adz[i] = jz;
+  a.gpot[i] = pot;
+}
+
 __global__ void k_pack_grav(int N, const double* __restrict__ h, const double* __restrict__ zeta,
                             const double* __restrict__ hf, double4* __restrict__ g) {
   const int i = blockIdx.x*256 + threadIdx.x;
@@ -357,6 +449,25 @@ __global__ void k_pack_grav(int N, const double* __restrict__ h, const double* _
 __global__ void k_zero8(int* p) { if (threadIdx.x < 8) p[threadIdx.x] = 0; }
 
 }  // namespace
+
+int nbody_direct(gdf_ctx* c, int smoothed) {
+  const int N = c->N;
+  if (N <= 0) return fail(c, GDF_ERR_INVALID, "no particles");
+  NbDirectArgs a;
+  a.N = N;
+  a.x = c->pf[F_X]; a.y = c->pf[F_Y]; a.z = c->pf[F_Z];
+  a.vx = c->pf[F_VX]; a.vy = c->pf[F_VY]; a.vz = c->pf[F_VZ];
+  a.m = c->pf[F_M]; a.h = c->pf[F_H];
+  a.ax = c->df[D_AX]; a.ay = c->df[D_AY]; a.az = c->df[D_AZ];
+  a.adx = c->df[D_ADOTX]; a.ady = c->df[D_ADOTY]; a.adz = c->df[D_ADOTZ];
+  a.gpot = c->df[D_GPOT];
+  a.kern = c->phys.kern;
+  const int nb = (N + 255)/256;
+  if (smoothed) k_nbody_direct<true><<<nb, 256, 0, c->stream>>>(a);
+  else k_nbody_direct<false><<<nb, 256, 0, c->stream>>>(a);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
 
 int gravity_walk(gdf_ctx* c) {
   const int N = c->N;
@@ -379,12 +490,15 @@ int gravity_walk(gdf_ctx* c) {
   const int grid = std::min(c->ngroups, kWalkSlots);
   if (c->timing) hipEventRecord(c->ev[7], s);
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
-  if (c->phys.hydro_forces) {
-    if (mixed) k_gravity<true, false><<<grid, kWave, 0, s>>>(a);
-    else k_gravity<false, false><<<grid, kWave, 0, s>>>(a);
+  if (c->phys.nbody) {
+    if (mixed) k_gravity<true, true, true><<<grid, kWave, 0, s>>>(a);
+    else k_gravity<false, true, true><<<grid, kWave, 0, s>>>(a);
+  } else if (c->phys.hydro_forces) {
+    if (mixed) k_gravity<true, false, false><<<grid, kWave, 0, s>>>(a);
+    else k_gravity<false, false, false><<<grid, kWave, 0, s>>>(a);
   } else {
-    if (mixed) k_gravity<true, true><<<grid, kWave, 0, s>>>(a);
-    else k_gravity<false, true><<<grid, kWave, 0, s>>>(a);
+    if (mixed) k_gravity<true, true, false><<<grid, kWave, 0, s>>>(a);
+    else k_gravity<false, true, false><<<grid, kWave, 0, s>>>(a);
   }
   GDF_HIP(c, hipGetLastError());
   if (c->timing) hipEventRecord(c->ev[8], s);

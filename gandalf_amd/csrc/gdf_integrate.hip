@@ -82,7 +82,7 @@ __global__ void k_end_step(EndArgs e) {
 }
 
 struct TsArgs {
-  int N, hydro, energy;
+  int N, hydro, energy, nbody;
   double courant, accel, emult;
   const double* h; const double* sound; const double* div_v; const double* a[3];
   const double* u; const double* dudt;
@@ -96,6 +96,14 @@ __global__ void k_timestep(TsArgs t) {
   for (int i = blockIdx.x*BS + threadIdx.x; i < t.N; i += gridDim.x*BS) {
     const double h = t.h[i];
     double ts;
+    if (t.nbody) {
+      // NbodyLeapfrogKDK::Timestep (src/Nbody/NbodyLeapfrogKDK.cpp:386-401)
+      const double ax = t.a[0][i], ay = t.a[1][i], az = t.a[2][i];
+      ts = t.accel*sqrt(h/(sqrt(ax*ax + ay*ay + az*az) + 1.0e-50));
+      if (t.dt_out) t.dt_out[i] = ts;
+      best = fmin(best, ts);
+      continue;
+    }
     if (t.hydro) ts = t.courant*h/(t.sound[i] + h*fabs(t.div_v[i]) + 1.0e-50);
     else ts = t.courant*h/(h*fabs(t.div_v[i]) + 1.0e-50);
     const double ax = t.a[0][i], ay = t.a[1][i], az = t.a[2][i];
@@ -238,9 +246,12 @@ int integ_end_step(gdf_ctx* c, double dtstep) {
 }
 
 int integ_timestep(gdf_ctx* c, double* dtmin) {
+  if (c->cfg.fixed_dt > 0.0) { *dtmin = c->cfg.fixed_dt; return GDF_OK; }
   TsArgs t;
   t.N = c->N; t.hydro = c->phys.hydro_forces; t.energy = c->phys.energy_integration;
-  t.courant = c->phys.courant_mult; t.accel = c->phys.accel_mult; t.emult = c->phys.energy_mult;
+  t.nbody = c->phys.nbody;
+  t.courant = c->phys.courant_mult; t.emult = c->phys.energy_mult;
+  t.accel = c->phys.nbody ? c->phys.nbody_mult : c->phys.accel_mult;
   t.h = c->pf[F_H]; t.sound = c->df[D_SOUND]; t.div_v = c->df[D_DIVV];
   for (int k = 0; k < 3; k++) t.a[k] = c->df[D_AX + k];
   t.u = c->pf[F_U]; t.dudt = c->df[D_DUDT]; t.dt_out = c->df[D_DT]; t.part = c->red;
@@ -252,7 +263,8 @@ int integ_timestep(gdf_ctx* c, double* dtmin) {
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   *dtmin = c->h_red[0];
   if (!(*dtmin > 0.0) || *dtmin > 1.0e20)
-    return fail(c, GDF_ERR_TIMESTEP, "Error : Hydro timestep too large");
+    return fail(c, GDF_ERR_TIMESTEP, c->phys.nbody ? "Error : N-body timestep invalid"
+                                                   : "Error : Hydro timestep too large");
   return GDF_OK;
 }
 

@@ -27,7 +27,8 @@ enum PField {
 // Derived per-particle fields, recomputed every step after the permutation.
 enum DField {
   D_AX, D_AY, D_AZ, D_ATX, D_ATY, D_ATZ, D_RHO, D_DUDT, D_DIVV, D_GPOT, D_INVOMEGA,
-  D_ZETA, D_SOUND, D_PRESSURE, D_HFACTOR, D_HSEARCH, D_DT, D_NDERIVED
+  D_ZETA, D_SOUND, D_PRESSURE, D_HFACTOR, D_HSEARCH, D_DT, D_ADOTX, D_ADOTY, D_ADOTZ,
+  D_NDERIVED
 };
 
 // Smoothing-kernel description passed by value to device code.
@@ -51,6 +52,9 @@ struct PhysParams {
   int nleaf;
   double courant_mult, accel_mult, energy_mult;
   int hydro_forces, self_gravity;
+  int nbody;                 // pure N-body: stars only, no gas pass
+  double eps2;               // Plummer softening length squared of the N-body tree walk
+  double nbody_mult;
 };
 
 // Tree node arrays in the unified index (internal 0..N-2, leaf l -> N-1+l).
@@ -60,6 +64,17 @@ struct TreeView {
   const double4* geo_c;   // (cx, cy, cz, hmax)
   const double4* geo_h;   // (hx, hy, hz, cdistsqd)
   const double* mp;       // 10 doubles per node: comx comy comz m q0..q4 pad
+};
+
+// Candidate set of the SPH passes, per walk group g: idx[g][b*64 + c] is candidate c of batch b,
+// mask[g][b*64 + l] the 64-bit set of batch-b candidates inside target l's range, ncand[g] the
+// number of candidates.  capb batches are reserved per group.
+struct CandSet {
+  int* idx = nullptr;
+  unsigned long long* mask = nullptr;
+  int* ncand = nullptr;
+  int capb = 0;
+  int gcap = 0;
 };
 
 }  // namespace gdf
@@ -98,10 +113,8 @@ struct gdf_ctx {
   int ngroups = 0;
   bool tree_valid = false;
 
-  // neighbour lists, interleaved: entry k of particle i at k*N + i
-  int cap_g = 0, cap_s = 0;
-  int* list_g = nullptr; int* nn_g = nullptr;
-  int* list_s = nullptr; int* nn_s = nullptr;
+  // per-group candidate sets: gather (h-rho search) and scatter-gather (forces)
+  gdf::CandSet cs_g, cs_s;
   bool sg_valid = false;
 
   // walk scratch
@@ -144,6 +157,9 @@ TreeView tree_view(const gdf_ctx* c);
 
 // SPH passes (gdf_sph.hip)
 int sph_alloc_lists(gdf_ctx* c);
+void sph_free_lists(gdf_ctx* c);
+int sph_copy_sg(gdf_ctx* c, std::vector<int>& idx, std::vector<unsigned long long>& mask,
+                std::vector<int>& ncand, int& capb);
 int sph_density(gdf_ctx* c, int nopbc);
 int sph_build_sg(gdf_ctx* c);
 int sph_hydro(gdf_ctx* c, int variant);
@@ -161,6 +177,7 @@ int integ_max_h(gdf_ctx* c, double* hmax);
 int integ_zero_forces(gdf_ctx* c);
 int integ_add_atree(gdf_ctx* c);
 int integ_pack_posm(gdf_ctx* c);
+int nbody_direct(gdf_ctx* c, int smoothed);
 
 inline int nblocks(long long n, int bs) { return (int) ((n + bs - 1) / bs); }
 

@@ -168,6 +168,79 @@ int GradhSphSimulationGpu<ndim>::MainLoop() {
   return GDF_OK;
 }
 
+template <int ndim>
+int NbodySimulationGpu<ndim>::ComputeForces() {
+  gdf_ctx* c = ctx;
+  GDF_TRY(gdf::integ_zero_forces(c));
+  record(c, 3);
+  if (c->cfg.nbody_direct) {
+    GDF_TRY(gdf::nbody_direct(c, c->cfg.nbody_direct == 2));
+    record(c, 4);
+    return GDF_OK;
+  }
+  GDF_TRY(gdf::tree_build(c));
+  record(c, 5);
+  GDF_TRY(gdf::gravity_walk(c));
+  GDF_TRY(gdf::integ_add_atree(c));
+  record(c, 4);
+  return GDF_OK;
+}
+
+template <int ndim>
+int NbodySimulationGpu<ndim>::PostInitialConditionsSetup() {
+  gdf_ctx* c = ctx;
+  c->t = 0.0;
+  c->tlast = 0.0;
+  GDF_TRY(ComputeForces());
+  GDF_TRY(gdf::integ_timestep(c, &c->timestep));
+  GDF_TRY(nbodyint.EndTimestep(0, c->t, 0.0));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  c->nsteps = 0;
+  c->setup_done = true;
+  return GDF_OK;
+}
+
+template <int ndim>
+int NbodySimulationGpu<ndim>::MainLoop() {
+  gdf_ctx* c = ctx;
+  if (!c->setup_done) return gdf::fail(c, GDF_ERR_INVALID, "MainLoop before setup");
+  GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
+  record(c, 0);
+  c->nsteps++;
+  const double dtstep = c->timestep;
+  c->t = c->t + c->timestep;
+  GDF_TRY(nbodyint.AdvanceParticles((int) c->nsteps, c->t, c->timestep));
+  record(c, 1);
+  GDF_TRY(ComputeForces());
+  double dtnew = 0.0;
+  GDF_TRY(gdf::integ_timestep(c, &dtnew));
+  // NbodyLeapfrogKDK::CorrectionTerms and EndTimestep
+  GDF_TRY(nbodyint.EndTimestep((int) c->nsteps, c->t, dtstep));
+  record(c, 6);
+  c->timestep = dtnew;
+  GDF_HIP(c, hipMemcpyAsync(c->h_red + 16000, c->icount, sizeof(long long)*6,
+                            hipMemcpyDeviceToHost, c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  const long long* ic = reinterpret_cast<const long long*>(c->h_red + 16000);
+  for (int k = 0; k < 6; k++) c->icounts_last[k] = ic[k];
+  if (c->timing) {
+    float a = 0.f, b = 0.f, f = 0.f, g = 0.f;
+    for (int k = 0; k < 6; k++) c->pass_ms[k] = 0.0;
+    hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+    hipEventElapsedTime(&f, c->ev[3], c->ev[4]);
+    hipEventElapsedTime(&b, c->ev[4], c->ev[6]);
+    c->pass_ms[3] = f;
+    c->pass_ms[4] = a + b;
+    if (!c->cfg.nbody_direct) {
+      hipEventElapsedTime(&g, c->ev[3], c->ev[5]);
+      c->pass_ms[0] = g;
+      c->pass_ms[3] = f - g;
+    }
+  }
+  return GDF_OK;
+}
+
+template class NbodySimulationGpu<3>;
 template class GradhSphTreeGpu<3>;
 template class SphLeapfrogKDKGpu<3>;
 template class GradhSphSimulationGpu<3>;
@@ -178,14 +251,30 @@ extern "C" {
 
 int gdf_setup_simulation(gdf_ctx* c) {
   if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  if (c->phys.nbody) {
+    gandalf_gpu::NbodySimulationGpu<3> sim(c);
+    return sim.PostInitialConditionsSetup();
+  }
   gandalf_gpu::GradhSphSimulationGpu<3> sim(c);
   return sim.PostInitialConditionsSetup();
 }
 
 int gdf_main_loop(gdf_ctx* c, int nsteps) {
   if (!c || c->N <= 0 || nsteps < 0) return GDF_ERR_INVALID;
+  if (c->phys.nbody) {
+    gandalf_gpu::NbodySimulationGpu<3> sim(c);
+    for (int s = 0; s < nsteps; s++) GDF_TRY(sim.MainLoop());
+    return GDF_OK;
+  }
   gandalf_gpu::GradhSphSimulationGpu<3> sim(c);
   for (int s = 0; s < nsteps; s++) GDF_TRY(sim.MainLoop());
+  return GDF_OK;
+}
+
+int gdf_nbody_direct_forces(gdf_ctx* c, int smoothed) {
+  if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  GDF_TRY(gdf::nbody_direct(c, smoothed ? 1 : 0));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
   return GDF_OK;
 }
 

@@ -5,6 +5,7 @@
 //   GradhSphTreeGpu      ~ SphNeighbourSearch<ndim> / GradhSphTree (src/Headers/SphNeighbourSearch.h:77-94)
 //   SphLeapfrogKDKGpu    ~ TimeIntegration<ndim> / SphLeapfrogKDK (src/Headers/Integration.h:48-66)
 //   GradhSphSimulationGpu~ SphSimulation<ndim> (src/Hydrodynamics/SphSimulation.cpp:204-880)
+//   NbodySimulationGpu   ~ NbodySimulation<ndim> (src/Nbody/NbodySimulation.cpp:183-375)
 //
 // Each call returns a GDF status code instead of raising; the C ABI in gandalf_gpu.h exposes
 // the simulation driver as gdf_setup_simulation / gdf_main_loop.
@@ -65,6 +66,24 @@ class GradhSphSimulationGpu {
   gdf_ctx* ctx;
   GradhSphTreeGpu<ndim> sphneib;
   SphLeapfrogKDKGpu<ndim> hydroint;
+};
+
+// Pure N-body run: stars only.  Forces by the reference's direct sums (nbody_direct 1/2) or by
+// the tree walk with Plummer softening; NbodyLeapfrogKDK's AdvanceParticles / CorrectionTerms /
+// EndTimestep (src/Nbody/NbodyLeapfrogKDK.cpp:250-370) map onto the same device KDK kernels as
+// the gas (no internal energy).
+template <int ndim>
+class NbodySimulationGpu {
+ public:
+  explicit NbodySimulationGpu(gdf_ctx* c) : ctx(c), nbodyint(c) {}
+  // NbodySimulation::PostInitialConditionsSetup (NbodySimulation.cpp:183-250)
+  int PostInitialConditionsSetup();
+  // NbodySimulation::MainLoop (NbodySimulation.cpp:258-375) with Nlevels = 1
+  int MainLoop();
+ private:
+  int ComputeForces();
+  gdf_ctx* ctx;
+  SphLeapfrogKDKGpu<ndim> nbodyint;
 };
 
 }  // namespace gandalf_gpu

@@ -35,6 +35,8 @@ class Config(C.Structure):
         ("boxmin", C.c_double * 3), ("boxmax", C.c_double * 3),
         ("courant_mult", C.c_double), ("accel_mult", C.c_double), ("energy_mult", C.c_double),
         ("precision", C.c_int), ("group_max", C.c_int),
+        ("nbody", C.c_int), ("nbody_direct", C.c_int), ("nbody_eps", C.c_double),
+        ("nbody_mult", C.c_double), ("fixed_dt", C.c_double),
     ]
 
 
@@ -70,6 +72,7 @@ PROTOTYPES = {
     "gdf_gravity_group_counts": (_i, [_vp, _vp, _vp]),
     "gdf_pass_times": (_i, [_vp, _vp]),
     "gdf_set_timing": (_i, [_vp, _i]),
+    "gdf_nbody_direct_forces": (_i, [_vp, _i]),
 }
 
 
@@ -116,6 +119,14 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
     c.energy_mult = float(prm.get("energy_mult", 0.4))
     c.precision = PREC[precision]
     c.group_max = int(group_max)
+    # pure N-body (sim = nbody): tree walk with Plummer softening nbody_eps (an extension), or
+    # the reference's direct sums selected by nbody_softening
+    c.nbody = 1 if prm.get("sim", "gradhsph") == "nbody" else 0
+    c.nbody_eps = float(prm.get("nbody_eps", 0.0))
+    if c.nbody and c.nbody_eps <= 0.0:
+        c.nbody_direct = 2 if int(prm.get("nbody_softening", 0)) else 1
+    c.nbody_mult = float(prm.get("nbody_mult", 0.1))
+    c.fixed_dt = float(prm.get("fixed_dt", 0.0))
     return c
 
 
@@ -253,6 +264,11 @@ class GpuSim:
         for k in ("group_first", "group_count", "group_node"):
             a[k] = a[k][:g]
         return a
+
+    def nbody_direct(self, smoothed=False):
+        """Star-star direct sums (Nbody::CalculateDirectGravForces, or the kernel-smoothed
+        NbodyLeapfrogKDK variant): fills a, adot and gpot."""
+        self._chk(lib().gdf_nbody_direct_forces(self._h, int(bool(smoothed))))
 
     def gravity_group_counts(self):
         a = self.tree_arrays()

@@ -155,7 +155,9 @@ def plummer_sphere(N=131072, seed=4, a=1.0, mass=1.0, rmax_frac=0.999):
     m = np.full(N, mass / N)
     r -= np.sum(m[:, None] * r, axis=0) / mass
     v -= np.sum(m[:, None] * v, axis=0) / mass
-    return dict(r=r, v=v, m=m, u=np.zeros(N), params=_base_params(sim="nbody"))
+    params = _base_params(sim="nbody", hydro_forces=0, self_gravity=1, thetamaxsqd=0.16,
+                          nbody_eps=0.01, fixed_dt=1.0e-3)
+    return dict(r=r, v=v, m=m, u=np.zeros(N), params=params)
 
 
 def write_column_file(path, ic):

@@ -48,6 +48,14 @@ typedef struct gdf_config {
   double courant_mult, accel_mult, energy_mult;
   int precision;              /* GDF_PREC_FP64 or GDF_PREC_MIXED (FP32 pair arithmetic) */
   int group_max;              /* particles per walk group (<= 64) */
+  /* pure N-body runs (NbodySimulation, src/Nbody/NbodySimulation.cpp): the particles are
+   * stars, there is no gas pass; forces by the tree walk with Plummer softening nbody_eps, or
+   * by the reference's direct sums (nbody_direct 1 = unsoftened, 2 = kernel-smoothed over h) */
+  int nbody;
+  int nbody_direct;
+  double nbody_eps;
+  double nbody_mult;          /* "nbody_mult" timestep multiplier */
+  double fixed_dt;            /* > 0: every step uses this timestep */
 } gdf_config;
 
 typedef struct gdf_ctx gdf_ctx;
@@ -132,6 +140,12 @@ int gdf_tree_arrays(gdf_ctx* ctx, int32_t* perm, uint64_t* keys, int32_t* left,
 int gdf_interaction_counts(gdf_ctx* ctx, int64_t* out6);
 /* per-group gravity interaction counts of the last gravity pass [ngroups] each */
 int gdf_gravity_group_counts(gdf_ctx* ctx, int32_t* ncell, int32_t* nnear);
+
+/* star-star direct sums for every particle of the context, writing a, adot and gpot:
+ * smoothed = 0 replaces Nbody::CalculateDirectGravForces (src/Nbody/Nbody.cpp:233-293),
+ * smoothed = 1 NbodyLeapfrogKDK::CalculateDirectSmoothedGravForces
+ * (src/Nbody/NbodyLeapfrogKDK.cpp:78-150) */
+int gdf_nbody_direct_forces(gdf_ctx* ctx, int smoothed);
 
 /* event timing of the named passes of the last step, milliseconds:
  * {tree, density, hydro, gravity, integrate, gravity walk kernel alone} */

@@ -778,6 +778,62 @@ void orc_direct_plummer(int N, const double* r, const double* m, double eps,
   }
 }
 
+#define ORC_TWOPI 6.28318530717959   /* Constants.h:62 */
+
+/* Nbody::CalculateDirectGravForces (src/Nbody/Nbody.cpp:233-293) */
+void orc_nbody_direct(int N, const double* r, const double* v, const double* m, double* a,
+                      double* adot, double* gpot) {
+  #pragma omp parallel for schedule(dynamic, 16)
+  for (int i=0; i<N; i++) {
+    double ai[3] = {0.0, 0.0, 0.0}, adi[3] = {0.0, 0.0, 0.0}, pot = 0.0;
+    for (int j=0; j<N; j++) {
+      if (i == j) continue;
+      double dr[3], dv[3];
+      for (int k=0; k<3; k++) { dr[k] = r[3*j+k] - r[3*i+k]; dv[k] = v[3*j+k] - v[3*i+k]; }
+      const double drsqd = dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2];
+      const double invdrmag = 1.0/sqrt(drsqd);
+      const double drdt = (dv[0]*dr[0] + dv[1]*dr[1] + dv[2]*dr[2])*invdrmag;
+      const double inv3 = pow(invdrmag, 3);
+      pot += m[j]*invdrmag;
+      for (int k=0; k<3; k++) ai[k] += m[j]*dr[k]*inv3;
+      for (int k=0; k<3; k++) adi[k] += m[j]*inv3*(dv[k] - 3.0*drdt*invdrmag*dr[k]);
+    }
+    for (int k=0; k<3; k++) { a[3*i+k] = ai[k]; adot[3*i+k] = adi[k]; }
+    gpot[i] = pot;
+  }
+}
+
+/* NbodyLeapfrogKDK::CalculateDirectSmoothedGravForces (src/Nbody/NbodyLeapfrogKDK.cpp:78-150) */
+void orc_nbody_direct_smoothed(const orc_params* p, int N, const double* r, const double* v,
+                               const double* m, const double* h, double* a, double* adot,
+                               double* gpot) {
+  okern kern; kern_init(&kern, p);
+  #pragma omp parallel for schedule(dynamic, 16)
+  for (int i=0; i<N; i++) {
+    double ai[3] = {0.0, 0.0, 0.0}, adi[3] = {0.0, 0.0, 0.0}, pot = 0.0;
+    for (int j=0; j<N; j++) {
+      if (i == j) continue;
+      double dr[3], dv[3];
+      for (int k=0; k<3; k++) { dr[k] = r[3*j+k] - r[3*i+k]; dv[k] = v[3*j+k] - v[3*i+k]; }
+      const double drsqd = dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2];
+      const double drmag = sqrt(drsqd) + SMALL_NUMBER;
+      const double invdrmag = 1.0/drmag;
+      const double invhmean = 2.0/(h[i] + h[j]);
+      const double drdt = (dv[0]*dr[0] + dv[1]*dr[1] + dv[2]*dr[2])*invdrmag;
+      const double paux = m[j]*invhmean*invhmean*K_wgrav(&kern, drmag*invhmean)*invdrmag;
+      const double wmean = K_w0(&kern, drmag*invhmean)*powf(invhmean, 3);
+      pot += m[j]*invhmean*K_wpot(&kern, drmag*invhmean);
+      for (int k=0; k<3; k++) ai[k] += paux*dr[k];
+      for (int k=0; k<3; k++)
+        adi[k] += paux*dv[k] - 3.0*paux*drdt*invdrmag*dr[k] +
+                  2.0*ORC_TWOPI*m[j]*drdt*wmean*invdrmag*dr[k];
+    }
+    for (int k=0; k<3; k++) { a[3*i+k] = ai[k]; adot[3*i+k] = adi[k]; }
+    gpot[i] = pot;
+  }
+  kern_free(&kern);
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Device-tree twin: Morton keys, LBVH (Karras 2012), bottom-up moments, walk groups           */
 /* ------------------------------------------------------------------------------------------ */

@@ -77,6 +77,16 @@ void orc_direct_gravity(const orc_params* p, int N, const double* r, const doubl
 void orc_direct_plummer(int N, const double* r, const double* m, double eps,
                         const int32_t* targets, int ntargets, double* a, double* pot);
 
+/* Star-star direct sums of the N-body integrator, every star active, outputs overwritten:
+ * Nbody::CalculateDirectGravForces (src/Nbody/Nbody.cpp:233-293), unsoftened with jerk, and
+ * NbodyLeapfrogKDK::CalculateDirectSmoothedGravForces (src/Nbody/NbodyLeapfrogKDK.cpp:78-150),
+ * kernel-softened over the mean smoothing length 2/(h_i + h_j). */
+void orc_nbody_direct(int N, const double* r, const double* v, const double* m, double* a,
+                      double* adot, double* gpot);
+void orc_nbody_direct_smoothed(const orc_params* p, int N, const double* r, const double* v,
+                               const double* m, const double* h, double* a, double* adot,
+                               double* gpot);
+
 /* ---- CPU twin of the device tree (Morton keys, LBVH, bottom-up moments, groups) ----
  * Arrays are sized: perm[N], parent[2N-1], left/right/first/last[N-1] (internal nodes),
  * nodes' moments [2N-1] in the unified index (internal 0..N-2, leaf l -> N-1+l).

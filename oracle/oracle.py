@@ -74,6 +74,8 @@ def lib():
         L.orc_hydro_forces.argtypes = [pp, i, i] + [vp] * 12
         L.orc_direct_gravity.argtypes = [pp, i] + [vp] * 7
         L.orc_direct_plummer.argtypes = [i, vp, vp, d, vp, i, vp, vp]
+        L.orc_nbody_direct.argtypes = [i] + [vp] * 6
+        L.orc_nbody_direct_smoothed.argtypes = [pp, i] + [vp] * 7
         L.orc_tree_alloc.argtypes = [C.POINTER(Tree), i]
         L.orc_tree_free.argtypes = [C.POINTER(Tree)]
         L.orc_tree_build.argtypes = [pp, C.POINTER(Tree), vp, vp, vp, i]
@@ -187,6 +189,25 @@ def direct_plummer(r, m, eps, targets):
     a = np.zeros((t.size, 3)); pot = np.zeros(t.size)
     lib().orc_direct_plummer(r.shape[0], _p(r), _p(m), eps, _p(t), t.size, _p(a), _p(pot))
     return a, pot
+
+
+def nbody_direct(r, v, m):
+    """Nbody::CalculateDirectGravForces: unsoftened star-star sums (a, adot, gpot)."""
+    r, v, m = _d(r), _d(v), _d(m)
+    N = r.shape[0]
+    a = np.zeros((N, 3)); ad = np.zeros((N, 3)); gp = np.zeros(N)
+    lib().orc_nbody_direct(N, _p(r), _p(v), _p(m), _p(a), _p(ad), _p(gp))
+    return a, ad, gp
+
+
+def nbody_direct_smoothed(p: Params, r, v, m, h):
+    """NbodyLeapfrogKDK::CalculateDirectSmoothedGravForces (a, adot, gpot)."""
+    r, v, m, h = _d(r), _d(v), _d(m), _d(h)
+    N = r.shape[0]
+    a = np.zeros((N, 3)); ad = np.zeros((N, 3)); gp = np.zeros(N)
+    lib().orc_nbody_direct_smoothed(C.byref(p), N, _p(r), _p(v), _p(m), _p(h), _p(a), _p(ad),
+                                    _p(gp))
+    return a, ad, gp
 
 
 class TreeTwin:
