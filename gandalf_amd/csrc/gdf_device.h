@@ -10,6 +10,23 @@
 
 namespace gdf {
 
+// ---- fast reciprocal and square-root forms ----
+// FP32: the hardware approximations (about 1 ulp), for the mixed-precision pair sums
+__device__ __forceinline__ float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq_f(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float sqrt_f(float x) { return __builtin_amdgcn_sqrtf(x); }
+// FP64 1/sqrt(x) for x > 0: hardware estimate (about 2^-23 relative) refined by two Newton
+// steps y <- y + y (1 - x y^2)/2, which leaves about an ulp
+__device__ __forceinline__ double rsq_d(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  #pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double e = fma(-(x*y), y, 1.0);
+    y = fma(0.5*y, e, y);
+  }
+  return y;
+}
+
 enum { KW0 = 0, KW1, KWOMEGA, KWZETA, KWGRAV, KWPOT, KW0S2, KWOMEGAS2, KWZETAS2 };
 
 template <typename T> __device__ __forceinline__ const T* ktab(const KernelDesc& k);
@@ -133,10 +150,40 @@ template <typename T> __device__ __forceinline__ T kw0(const KernelDesc& k, T s)
   if (k.tabulated) return tab_lin<T>(k, KW0, s);
   return k.type == 0 ? m4_w0<T>(T(k.norm), s) : q_w0<T>(T(k.norm), s);
 }
+// ---- branch-free analytic forms for the pair loops ----
+// W and dW/ds as truncated powers, i.e. the reference's piecewise polynomials summed without the
+// cancellation of their expanded form: M4 W = (norm/4)[(2-s)^3_+ - 4(1-s)^3_+], quintic
+// W = norm[(3-s)^5_+ - 6(2-s)^5_+ + 15(1-s)^5_+]; womega = -3 w0 - s w1 exactly for both.
+template <typename T> __device__ __forceinline__ T tpos(T x) { return x > T(0) ? x : T(0); }
+
+template <typename T>
+__device__ __forceinline__ void kw01_an(int type, T norm, T s, T& w0, T& w1) {
+  if (type == 0) {
+    const T A = tpos(T(2) - s), B = tpos(T(1) - s);
+    const T A2 = A*A, B2 = B*B;
+    w0 = T(0.25)*norm*(A2*A - T(4)*(B2*B));
+    w1 = T(-0.75)*norm*(A2 - T(4)*B2);
+  } else {
+    const T A = tpos(T(3) - s), B = tpos(T(2) - s), C = tpos(T(1) - s);
+    const T A2 = A*A, B2 = B*B, C2 = C*C;
+    const T A4 = A2*A2, B4 = B2*B2, C4 = C2*C2;
+    w0 = norm*(A4*A - T(6)*(B4*B) + T(15)*(C4*C));
+    w1 = T(-5)*norm*(A4 - T(6)*B4 + T(15)*C4);
+  }
+}
+
 template <typename T> __device__ __forceinline__ T kw1(const KernelDesc& k, T s) {
   if (k.tabulated) return tab_lin<T>(k, KW1, s);
-  return k.type == 0 ? m4_w1<T>(T(k.norm), s) : q_w1<T>(T(k.norm), s);
+  const T norm = T(k.norm);
+  if (k.type == 0) {
+    const T A = tpos(T(2) - s), B = tpos(T(1) - s);
+    return T(-0.75)*norm*(A*A - T(4)*(B*B));
+  }
+  const T A = tpos(T(3) - s), B = tpos(T(2) - s), C = tpos(T(1) - s);
+  const T A2 = A*A, B2 = B*B, C2 = C*C;
+  return T(-5)*norm*(A2*A2 - T(6)*(B2*B2) + T(15)*(C2*C2));
 }
+
 template <typename T> __device__ __forceinline__ T kwgrav(const KernelDesc& k, T s) {
   if (k.tabulated) {
     if (s >= T(k.range)) return T(1)/(s*s);
@@ -161,9 +208,11 @@ __device__ __forceinline__ void kdens(const KernelDesc& k, T s2, T& w0, T& wom, 
     w0 = t[KW0S2*k.res + idx]; wom = t[KWOMEGAS2*k.res + idx]; wz = t[KWZETAS2*k.res + idx];
     return;
   }
-  const T s = sqrt(s2);
-  if (k.type == 0) { w0 = m4_w0<T>(T(k.norm), s); wom = m4_womega<T>(T(k.norm), s); wz = m4_wzeta<T>(s); }
-  else { w0 = q_w0<T>(T(k.norm), s); wom = q_womega<T>(T(k.norm), s); wz = q_wzeta<T>(s); }
+  const T s = sizeof(T) == 4 ? (T) sqrt_f((float) s2) : (T) sqrt((double) s2);
+  T w1;
+  kw01_an<T>(k.type, T(k.norm), s, w0, w1);
+  wom = T(-3)*w0 - s*w1;
+  wz = k.type == 0 ? m4_wzeta<T>(s) : q_wzeta<T>(s);
 }
 
 // ---- periodic nearest image (GhostNeighbours.hpp:119-138) ----
@@ -187,35 +236,17 @@ __device__ __forceinline__ double range2_rn(double krsqd, double h) {
   return krsqd*h*h;
 }
 
-// FP32 squared distance of the mixed path with a fixed operation order: the scatter-gather
-// list and the gravity walk both classify a pair with it, so each pair lands in exactly one
-// of the two sums
+// FP32 squared distance of the mixed path with a fixed operation order (explicit fused
+// multiply-adds): the scatter-gather walk and the gravity walk both classify a pair with it,
+// so each pair lands in exactly one of the two sums
 __device__ __forceinline__ float dot3_f(float a, float b, float c) {
-#pragma clang fp contract(off)
-  return a*a + b*b + c*c;
+  return fmaf(c, c, fmaf(b, b, a*a));
 }
 
 // Kernel-range box test of the gravity walk with a relative slack: a node whose box comes within
 // (1 + slack) k max(h) of the group is resolved to particles, so no pair of the scatter-gather
 // set (decided per pair in FP64 or FP32) can end up inside an accepted multipole.
 constexpr double kGravNeibSlack = 1.0 + 1.0e-4;
-
-// ---- fast reciprocal and square-root forms ----
-// FP32: the hardware approximations (about 1 ulp), for the mixed-precision pair sums
-__device__ __forceinline__ float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float rsq_f(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ float sqrt_f(float x) { return __builtin_amdgcn_sqrtf(x); }
-// FP64 1/sqrt(x) for x > 0: hardware estimate (about 2^-23 relative) refined by two Newton
-// steps y <- y + y (1 - x y^2)/2, which leaves about an ulp
-__device__ __forceinline__ double rsq_d(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  #pragma unroll
-  for (int it = 0; it < 2; it++) {
-    const double e = fma(-(x*y), y, 1.0);
-    y = fma(0.5*y, e, y);
-  }
-  return y;
-}
 
 // ---- wave primitives (wave64) ----
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

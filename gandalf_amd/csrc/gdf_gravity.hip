@@ -296,7 +296,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
             for (int q = 0; q < nb; q++) {
               const float dx = sf[0][q] - rxi, dy = sf[1][q] - ryi, dz = sf[2][q] - rzi;
               const float d2 = dot3_f(dx, dy, dz);
-              const bool soft = d2 < r2if || d2 < sf[4][q];
+              const bool soft = (d2 < r2if) | (d2 < sf[4][q]);
               if (PLUMMER) {
                 const float inv = rsq_f(d2 + eps2f);
                 const float mj = sidx[q] == i ? 0.f : sf[3][q];
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
                           sd[7][q], ax, ay, az, pot);
               } else {
                 const double d2 = dot3_rn(dx, dy, dz);
-                const bool soft = d2 < r2i || d2 < sd[4][q];
+                const bool soft = (d2 < r2i) | (d2 < sd[4][q]);
                 const double inv = rsq_d(d2 + 1.0e-20);
                 const double mj = soft ? 0.0 : sd[3][q];
                 const double minv3 = mj*inv*inv*inv;
@@ -421,7 +421,12 @@ __global__ __launch_bounds__(256) void k_nbody_direct(NbDirectArgs a) {
           const double drdt = (dvx*dx + dvy*dy + dvz*dz)*invdrmag;
           const double s = drmag*invhmean;
           const double paux = mj*invhmean*invhmean*kwgrav<double>(a.kern, s)*invdrmag;
-          const double wmean = kw0<double>(a.kern, s)*(double) powf((float) invhmean, 3.0f);
+          // the reference's single-precision powf(invhmean, 3): the correctly rounded cube
+          // (exact in double before the one rounding to float), which is what glibc's powf
+          // returns for all but ~0.07% of arguments (those differ by one float ulp)
+          const float hmf = (float) invhmean;
+          const double wmean = kw0<double>(a.kern, s)*
+                               (double) (float) ((double) hmf*(double) hmf*(double) hmf);
           pot += mj*invhmean*kwpot<double>(a.kern, s);
           ax += paux*dx; ay += paux*dy; az += paux*dz;
           const double c3 = 3.0*paux*drdt*invdrmag;

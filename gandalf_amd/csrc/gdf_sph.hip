@@ -155,14 +155,14 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
             if (MODE == 2) {
               const float4 c = fc[q];
               const float d2 = dot3_f(c.x - rxi, c.y - ryi, c.z - rzi);
-              in = (d2 < r2if) || (d2 < c.w);
+              in = (d2 < r2if) | (d2 < c.w);
             } else {
               const double4 c = sc[q];
               double dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
               nearest_image<double>(pp, dx, dy, dz);
               const double d2 = dot3_rn(dx, dy, dz);
               if (MODE == 0) in = d2 <= r2i;
-              else in = (d2 < r2i) || (d2 < c.w);
+              else in = (d2 < r2i) | (d2 < c.w);
             }
             m |= (unsigned long long) in << q;
           }

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libgandalf_gpu.so")
 GDF_KERNEL = {"m4": 0, "quintic": 1}
 GDF_EOS = {"energy_eqn": 0, "isothermal": 1, "barotropic": 2}
 PREC = {"fp64": 0, "mixed": 1}
-FIELDS3 = ("r", "v", "a", "atree", "r0", "v0", "a0")
+FIELDS3 = ("r", "v", "a", "atree", "r0", "v0", "a0", "adot")
 
 
 class GdfError(RuntimeError):

@@ -34,3 +34,15 @@ def test_library_exports_every_declared_symbol():
 def test_python_binding_covers_header():
     from gandalf_amd import gpu
     assert set(declared_symbols()) == set(gpu.PROTOTYPES)
+
+
+def test_python_field_widths_match_library_table():
+    """gdf_get_field writes w*N doubles for a field of width w: the binding must size its
+    buffer from the same widths the library's field table (gdf_capi.hip) gives."""
+    from gandalf_amd import gpu
+    src = open(os.path.join(ROOT, "gandalf_amd", "csrc", "gdf_capi.hip")).read()
+    tab = re.findall(r'\{"([a-z0-9_]+)",\s*[01],\s*[A-Z0-9_]+,\s*([13])\}', src)
+    assert len(tab) >= 20
+    for name, w in tab:
+        assert (name in gpu.FIELDS3) == (w == "3"), name
+    assert set(gpu.FIELDS3) <= {n for n, _ in tab}

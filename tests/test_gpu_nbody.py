@@ -31,8 +31,8 @@ def gpu_sim(g, precision="fp64", h=None):
 @pytest.mark.parametrize("smoothed", [0, 1])
 def test_nbody_direct_matches_oracle(smoothed):
     """Nbody::CalculateDirectGravForces / the kernel-smoothed variant at 4096 stars: a and gpot
-    to 1e-12 of their maxima; the jerk to 1e-12 (unsoftened) or 1e-9 (smoothed, whose
-    (2/(h_i+h_j))^3 factor the reference rounds to single precision with powf)."""
+    to 1e-12 of their maxima; the jerk to 1e-12 (unsoftened) or within one float ulp of each
+    pair's single-precision (2/(h_i+h_j))^3 factor (smoothed)."""
     g = plummer(4096, 21, nbody_eps=0.0, nbody_softening=smoothed, kernel="m4")
     rng = np.random.default_rng(3)
     h = 0.02 + 0.05 * rng.random(4096)
@@ -45,7 +45,29 @@ def test_nbody_direct_matches_oracle(smoothed):
         a, ad, gp = O.nbody_direct(g["r"], g["v"], g["m"])
     assert relmax(s.get("a"), a) < 1e-12
     assert relmax(s.get("gpot"), gp) < 1e-12
-    assert relmax(s.get("adot"), ad) < (1e-9 if smoothed else 1e-12)
+    adot = s.get("adot")
+    if not smoothed:
+        assert relmax(adot, ad) < 1e-12
+        return
+    # The smoothed jerk's 4 pi m_j (dr/dt) wmean term carries the reference's single-precision
+    # powf(2/(h_i+h_j), 3); glibc's powf is not correctly rounded for ~0.07% of arguments and
+    # the device takes the correctly rounded cube, so per pair the term may differ by one
+    # float ulp (2^-23 relative).  Bound per star, with the M4 w0 <= w0(0) = 1/pi:
+    #   |d adot_i| <= 2^-23 * sum_{j: s_ij < 2} 4 pi m_j |dr/dt| (1/pi) (2/(h_i+h_j))^3
+    r, v, m = g["r"], g["v"], g["m"]
+    bound = np.zeros(len(m))
+    for i0 in range(0, len(m), 512):
+        dr = r[None, :, :] - r[i0:i0 + 512, None, :]
+        d = np.sqrt((dr ** 2).sum(-1)) + 1e-300
+        drdt = np.abs(((v[None, :, :] - v[i0:i0 + 512, None, :]) * dr).sum(-1)) / d
+        ihm = 2.0 / (h[i0:i0 + 512, None] + h[None, :])
+        near = d * ihm < 2.0
+        bound[i0:i0 + 512] = (near * 4.0 * m[None, :] * drdt * ihm ** 3).sum(1) * 2.0 ** -23
+    err = np.abs(adot - ad).max(1)
+    scale = np.abs(ad).max()
+    print(f"smoothed jerk: max rel {relmax(adot, ad):.3e}; "
+          f"worst err/bound {np.max(err / (bound + 1e-12 * scale)):.3f}")
+    assert np.all(err <= bound + 1e-12 * scale)
 
 
 def test_plummer_tree_against_direct_sum():
