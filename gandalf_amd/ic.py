@@ -73,6 +73,19 @@ def jittered_lattice_box(n_side=32, seed=1, jitter=0.1, rho=1.0, u=1.0, vamp=0.0
     return dict(r=r, v=v, m=m, u=uu, params=params)
 
 
+def hot_blob_box(n_side=16, seed=1, blob_u=40.0, sigma=0.1, nlevels=4, level_diff_max=1,
+                 jitter=0.1):
+    """Periodic jittered-lattice box at rest (rho = 1, u = 1) with a hot Gaussian blob at the
+    centre, u = 1 + blob_u exp(-|r - c|^2 / (2 sigma^2)): the sound speed, and with it the
+    Courant timestep, varies by ~sqrt(1 + blob_u) across the box, so block timesteps spread
+    the particles over several levels and the expanding blob moves them between levels."""
+    g = jittered_lattice_box(n_side=n_side, seed=seed, jitter=jitter, vamp=0.0)
+    d2 = ((g["r"] - 0.5) ** 2).sum(1)
+    g["u"] = 1.0 + blob_u * np.exp(-d2 / (2.0 * sigma * sigma))
+    g["params"].update(Nlevels=nlevels, level_diff_max=level_diff_max)
+    return g
+
+
 def uniform_sphere(N, seed, radius=1.0):
     """N points uniform in a sphere by seeded rejection sampling from the enclosing cube."""
     rng = np.random.default_rng(seed)

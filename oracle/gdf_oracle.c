@@ -544,9 +544,12 @@ static int compute_h_one(const okern* kern, const orc_params* p, const double* x
   return h <= hmax ? 1 : -1;
 }
 
-int orc_compute_h(const orc_params* p, int N, const double* r, const double* m, int nopbc,
-                  double* h, double* rho, double* invomega, double* zeta, double* hfactor,
-                  double* u, double* sound, double* pressure) {
+/* act (optional): only particles with act[i] != 0 are iterated, as UpdateAllSphProperties does
+ * for the active particles of a block step (GradhSphTree.cpp:83-271) */
+static int compute_h_masked(const orc_params* p, int N, const double* r, const double* m,
+                            int nopbc, double* h, double* rho, double* invomega, double* zeta,
+                            double* hfactor, double* u, double* sound, double* pressure,
+                            const unsigned char* act) {
   okern kern; kern_init(&kern, p);
   obox b; box_init(&b, p, nopbc);
   const double kr = kern.kernrange;
@@ -561,6 +564,7 @@ int orc_compute_h(const orc_params* p, int N, const double* r, const double* m, 
     double* nm = (double*) malloc(sizeof(double)*cap);
     #pragma omp for schedule(dynamic, 32)
     for (int i=0; i<N; i++) {
+      if (act && !act[i]) continue;
       const double* xi = &r[3*i];
       double hmax = h[i];
       int ok;
@@ -617,6 +621,13 @@ int orc_compute_h(const orc_params* p, int N, const double* r, const double* m, 
   return fail ? -1 : 0;
 }
 
+int orc_compute_h(const orc_params* p, int N, const double* r, const double* m, int nopbc,
+                  double* h, double* rho, double* invomega, double* zeta, double* hfactor,
+                  double* u, double* sound, double* pressure) {
+  return compute_h_masked(p, N, r, m, nopbc, h, rho, invomega, zeta, hfactor, u, sound,
+                          pressure, NULL);
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Hydro forces                                                                                */
 /* ------------------------------------------------------------------------------------------ */
@@ -638,10 +649,16 @@ static void scatter_gather_lists(const orc_params* p, const obox* b, int N, cons
   *out = l;
 }
 
-void orc_hydro_forces(const orc_params* p, int variant, int N, const double* r,
-                      const double* v, const double* m, const double* h, const double* rho,
-                      const double* invomega, const double* pressure, const double* sound,
-                      const double* hfactor, double* a, double* dudt, double* div_v) {
+/* act (optional): forces of the active particles only; level/levelneib (optional): the
+ * neighbour-level bookkeeping of GradhSph.cpp:445-446 merged as GradhSphTree.cpp:375-417:
+ * for an active i and every j of its scatter-gather set (i itself included),
+ * levelneib_i = max(levelneib_i, level_j) and levelneib_j = max(levelneib_j, level_i) */
+static void hydro_forces_masked(const orc_params* p, int variant, int N, const double* r,
+                                const double* v, const double* m, const double* h,
+                                const double* rho, const double* invomega,
+                                const double* pressure, const double* sound,
+                                const double* hfactor, double* a, double* dudt, double* div_v,
+                                const unsigned char* act, const int* level, int* levelneib) {
   okern kern; kern_init(&kern, p);
   obox b; box_init(&b, p, 0);
   ivec* lists;
@@ -650,6 +667,7 @@ void orc_hydro_forces(const orc_params* p, int variant, int N, const double* r,
   const double gammam1 = p->gamma - 1.0;
   #pragma omp parallel for schedule(dynamic, 32)
   for (int i=0; i<N; i++) {
+    if (act && !act[i]) continue;
     double ai[3] = {0.0, 0.0, 0.0}, du = 0.0, dv = 0.0;
     const double invh_i = 1/h[i];
     const double invrho_i = 1/rho[i];
@@ -699,9 +717,27 @@ void orc_hydro_forces(const orc_params* p, int variant, int N, const double* r,
     dudt[i] = du;
     div_v[i] = dv;
   }
+  if (level) {
+    for (int i=0; i<N; i++) {
+      if (act && !act[i]) continue;
+      for (int t=0; t<lists[i].n; t++) {
+        const int j = lists[i].data[t];
+        if (level[j] > levelneib[i]) levelneib[i] = level[j];
+        if (level[i] > levelneib[j]) levelneib[j] = level[i];
+      }
+    }
+  }
   for (int i=0; i<N; i++) free(lists[i].data);
   free(lists);
   kern_free(&kern);
+}
+
+void orc_hydro_forces(const orc_params* p, int variant, int N, const double* r,
+                      const double* v, const double* m, const double* h, const double* rho,
+                      const double* invomega, const double* pressure, const double* sound,
+                      const double* hfactor, double* a, double* dudt, double* div_v) {
+  hydro_forces_masked(p, variant, N, r, v, m, h, rho, invomega, pressure, sound, hfactor, a,
+                      dudt, div_v, NULL, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -734,15 +770,17 @@ static inline void grav_pair(const okern* kern, double krsqd, const double* dr_i
   }
 }
 
-void orc_direct_gravity(const orc_params* p, int N, const double* r, const double* m,
-                        const double* h, const double* zeta, const double* hfactor,
-                        double* atree, double* gpot) {
+static void direct_gravity_masked(const orc_params* p, int N, const double* r,
+                                  const double* m, const double* h, const double* zeta,
+                                  const double* hfactor, double* atree, double* gpot,
+                                  const unsigned char* act) {
   okern kern; kern_init(&kern, p);
   obox b; box_init(&b, p, 0);
   const double krsqd = kern.kernrangesqd;
   const double wpot0 = K_wpot(&kern, 0.0);
   #pragma omp parallel for schedule(dynamic, 16)
   for (int i=0; i<N; i++) {
+    if (act && !act[i]) continue;
     double acc[3] = {0.0, 0.0, 0.0};
     double pot = (m[i]/h[i])*wpot0;
     for (int j=0; j<N; j++) {
@@ -755,6 +793,12 @@ void orc_direct_gravity(const orc_params* p, int N, const double* r, const doubl
     gpot[i] = pot;
   }
   kern_free(&kern);
+}
+
+void orc_direct_gravity(const orc_params* p, int N, const double* r, const double* m,
+                        const double* h, const double* zeta, const double* hfactor,
+                        double* atree, double* gpot) {
+  direct_gravity_masked(p, N, r, m, h, zeta, hfactor, atree, gpot, NULL);
 }
 
 void orc_direct_plummer(int N, const double* r, const double* m, double eps,
@@ -1093,10 +1137,12 @@ static inline void quad_force(const double* com, const double* q, double mc,
 
 #define GRAV_NEIB_SLACK (1.0 + 1.0e-4)
 
-void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
-                      const double* m, const double* h, const double* zeta,
-                      const double* hfactor, double* atree, double* gpot,
-                      int32_t* ncell, int32_t* nnear) {
+/* act (optional, input order): groups without an active particle are skipped (counts 0) and
+ * only active targets are written */
+static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const double* r,
+                                const double* m, const double* h, const double* zeta,
+                                const double* hfactor, double* atree, double* gpot,
+                                int32_t* ncell, int32_t* nnear, const unsigned char* act) {
   okern kern; kern_init(&kern, p);
   obox b; box_init(&b, p, 0);
   const int N = t->N;
@@ -1110,6 +1156,15 @@ void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
     int* near = malloc(sizeof(int)*(N + 2));
     #pragma omp for schedule(dynamic, 4)
     for (int g=0; g<t->ngroups; g++) {
+      if (act) {
+        int any = 0;
+        for (int tt=0; tt<t->group_count[g]; tt++) any |= act[t->perm[t->group_first[g] + tt]];
+        if (!any) {
+          if (ncell) ncell[g] = 0;
+          if (nnear) nnear[g] = 0;
+          continue;
+        }
+      }
       const int G = t->group_node[g];
       double cG[3], halfG[3];
       for (int k=0; k<3; k++) {
@@ -1163,6 +1218,7 @@ void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
       if (nnear) nnear[g] = np;
       for (int tt=0; tt<t->group_count[g]; tt++) {
         const int i = t->perm[t->group_first[g] + tt];
+        if (act && !act[i]) continue;
         double acc[3] = {0.0, 0.0, 0.0};
         double pot = (m[i]/h[i])*wpot0;
         for (int q=0; q<np; q++) {
@@ -1190,6 +1246,13 @@ void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
     free(stack); free(cells); free(near);
   }
   kern_free(&kern);
+}
+
+void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
+                      const double* m, const double* h, const double* zeta,
+                      const double* hfactor, double* atree, double* gpot,
+                      int32_t* ncell, int32_t* nnear) {
+  tree_gravity_masked(p, t, r, m, h, zeta, hfactor, atree, gpot, ncell, nnear, NULL);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -1285,23 +1348,35 @@ void orc_diagnostics(const orc_params* p, int N, const double* r, const double* 
 /* ------------------------------------------------------------------------------------------ */
 /* Full-step driver                                                                            */
 /* ------------------------------------------------------------------------------------------ */
+#define ORC_INVLOGETWO 1.44269504088896   /* Constants.h:81 */
+
 struct orc_sim {
   orc_params p;
   int N;
   double t, timestep, tlast;
   double *r, *v, *a, *atree, *m, *h, *rho, *u, *dudt, *div_v, *gpot, *invomega, *zeta;
   double *sound, *pressure, *hfactor, *r0, *v0, *a0, *u0, *dudt0;
+  /* block timesteps (Particle.h:140-167, Simulation.h:187-215) */
+  int n, nresync, level_max, level_step;
+  double dt_max;
+  int *level, *levelneib, *nstep, *nlast;
+  double *ptlast, *pdt, *dt_next, *dtw;
+  unsigned char *active, *endstep;
+  long long nfinished;   /* particles that ended a step in the last MainLoop */
 };
 
 orc_sim* orc_sim_create(const orc_params* p, int N, const double* r, const double* v,
                         const double* m, const double* u) {
   orc_sim* s = calloc(1, sizeof(orc_sim));
   s->p = *p; s->N = N;
-#define ALLOC(f, n) s->f = calloc((size_t) (n), sizeof(double))
+#define ALLOC(f, n) s->f = calloc((size_t) (n), sizeof(*s->f))
   ALLOC(r, 3*N); ALLOC(v, 3*N); ALLOC(a, 3*N); ALLOC(atree, 3*N); ALLOC(m, N); ALLOC(h, N);
   ALLOC(rho, N); ALLOC(u, N); ALLOC(dudt, N); ALLOC(div_v, N); ALLOC(gpot, N);
   ALLOC(invomega, N); ALLOC(zeta, N); ALLOC(sound, N); ALLOC(pressure, N); ALLOC(hfactor, N);
   ALLOC(r0, 3*N); ALLOC(v0, 3*N); ALLOC(a0, 3*N); ALLOC(u0, N); ALLOC(dudt0, N);
+  ALLOC(level, N); ALLOC(levelneib, N); ALLOC(nstep, N); ALLOC(nlast, N);
+  ALLOC(ptlast, N); ALLOC(pdt, N); ALLOC(dt_next, N); ALLOC(dtw, N);
+  ALLOC(active, N); ALLOC(endstep, N);
 #undef ALLOC
   memcpy(s->r, r, sizeof(double)*3*N);
   memcpy(s->v, v, sizeof(double)*3*N);
@@ -1316,29 +1391,187 @@ void orc_sim_destroy(orc_sim* s) {
   free(s->u); free(s->dudt); free(s->div_v); free(s->gpot); free(s->invomega); free(s->zeta);
   free(s->sound); free(s->pressure); free(s->hfactor); free(s->r0); free(s->v0); free(s->a0);
   free(s->u0); free(s->dudt0);
+  free(s->level); free(s->levelneib); free(s->nstep); free(s->nlast); free(s->ptlast);
+  free(s->pdt); free(s->dt_next); free(s->dtw); free(s->active); free(s->endstep);
   free(s);
 }
 
-static int sim_forces(orc_sim* s) {
+/* Sph::ZeroAccelerations (Sph.cpp:126-140), active particles */
+static void zero_active(orc_sim* s, const unsigned char* act) {
+  for (int i=0; i<s->N; i++) {
+    if (act && !act[i]) continue;
+    s->levelneib[i] = 0;
+    s->div_v[i] = 0.0; s->dudt[i] = 0.0; s->gpot[i] = 0.0;
+    for (int k=0; k<3; k++) { s->a[3*i+k] = 0.0; s->atree[3*i+k] = 0.0; }
+  }
+}
+
+/* density, thermal state and forces of the active particles (all when act is NULL):
+ * the body of SphSimulation::MainLoop's force loop (SphSimulation.cpp:653-720) */
+static int sim_forces(orc_sim* s, const unsigned char* act) {
   const int N = s->N;
   const orc_params* p = &s->p;
-  if (orc_compute_h(p, N, s->r, s->m, 0, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
-                    s->u, s->sound, s->pressure)) return -1;
-  orc_hydro_forces(p, p->self_gravity ? 1 : 0, N, s->r, s->v, s->m, s->h, s->rho, s->invomega,
-                   s->pressure, s->sound, s->hfactor, s->a, s->dudt, s->div_v);
-  if (p->self_gravity) {
+  if (compute_h_masked(p, N, s->r, s->m, 0, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
+                       s->u, s->sound, s->pressure, act)) return -1;
+  zero_active(s, act);
+  /* ComputeThermalProperties for every particle (nradstep = 1) */
+  if (act) for (int i=0; i<N; i++) eos_thermal(p, s->rho[i], &s->u[i], &s->sound[i],
+                                               &s->pressure[i]);
+  hydro_forces_masked(p, p->self_gravity ? 1 : 0, N, s->r, s->v, s->m, s->h, s->rho,
+                      s->invomega, s->pressure, s->sound, s->hfactor, s->a, s->dudt, s->div_v,
+                      act, s->level, s->levelneib);
+  if (p->self_gravity && p->grav_direct) {
+    direct_gravity_masked(p, N, s->r, s->m, s->h, s->zeta, s->hfactor, s->atree, s->gpot, act);
+    for (int i=0; i<N; i++) {
+      if (act && !act[i]) continue;
+      for (int k=0; k<3; k++) s->a[3*i+k] += s->atree[3*i+k];
+    }
+  } else if (p->self_gravity) {
     orc_tree tr;
     orc_tree_alloc(&tr, N);
     orc_tree_build(p, &tr, s->r, s->m, s->h, 64);
-    orc_tree_gravity(p, &tr, s->r, s->m, s->h, s->zeta, s->hfactor, s->atree, s->gpot,
-                     NULL, NULL);
+    tree_gravity_masked(p, &tr, s->r, s->m, s->h, s->zeta, s->hfactor, s->atree, s->gpot,
+                        NULL, NULL, act);
     orc_tree_free(&tr);
-    for (int i=0; i<3*N; i++) s->a[i] += s->atree[i];
-  } else {
-    memset(s->atree, 0, sizeof(double)*3*N);
-    memset(s->gpot, 0, sizeof(double)*N);
+    for (int i=0; i<N; i++) {
+      if (act && !act[i]) continue;
+      for (int k=0; k<3; k++) s->a[3*i+k] += s->atree[3*i+k];
+    }
   }
   return 0;
+}
+
+static int ipow2(int e) { return (int) pow(2.0, (double) e); }
+
+/* InlineFuncs.h:550-557 */
+static int timestep_level(double dt, double dt_max) {
+  const int level = (int) (ORC_INVLOGETWO*log(dt_max/dt)) + 1;
+  return level > 0 ? level : 0;
+}
+
+/* Simulation::ComputeBlockTimesteps (Simulation.cpp:1764-2197), hydro particles only, KDK
+ * (integration_step = 1), no sinks, sph_single_timestep = 0 */
+static void block_timesteps(orc_sim* s) {
+  const int N = s->N;
+  const orc_params* p = &s->p;
+  const int ldm = p->level_diff_max;
+  double dmin = orc_timesteps(p, N, s->h, s->sound, s->div_v, s->a, s->u, s->dudt, s->dtw);
+  if (s->n == s->nresync) {
+    s->n = 0;
+    s->timestep = dmin;
+    s->level_max = p->nlevels - 1;
+    s->level_step = s->level_max;
+    s->dt_max = s->timestep*pow(2.0, s->level_max);
+    for (int i=0; i<N; i++) {
+      int level = timestep_level(s->dtw[i], s->dt_max);
+      if (level > s->level_max) level = s->level_max;
+      s->level[i] = level;
+      s->levelneib[i] = level;
+      s->nstep[i] = ipow2(s->level_step - level);
+      s->nlast[i] = s->n;
+      s->dt_next[i] = s->nstep[i]*s->timestep;
+      s->endstep[i] = 1;
+    }
+    s->nresync = ipow2(s->level_step);
+    s->timestep = s->dt_max/(double) s->nresync;
+    return;
+  }
+  const int level_max_old = s->level_max;
+  int level_max = 0;
+  for (int i=0; i<N; i++) {
+    const int n = s->n;
+    if (n - s->nlast[i] == s->nstep[i] && s->nstep[i] != ipow2(s->level_step - s->level[i])) {
+      /* step ended early by CheckTimesteps (Saitoh & Makino limiter) */
+      int level = timestep_level(s->dtw[i], s->dt_max);
+      if (s->levelneib[i] - ldm > level) level = s->levelneib[i] - ldm;
+      if (level > s->level[i]) s->level[i] = level;
+      s->levelneib[i] = s->level[i];
+      s->nlast[i] = n;
+      s->nstep[i] = ipow2(s->level_step - s->level[i]);
+      s->dt_next[i] = s->nstep[i]*s->timestep;
+      s->endstep[i] = 1;
+    } else if (n - s->nlast[i] == s->nstep[i]) {
+      const int nstep = s->nstep[i], last_level = s->level[i];
+      int level = timestep_level(s->dtw[i], s->dt_max);
+      if (s->levelneib[i] - ldm > level) level = s->levelneib[i] - ldm;
+      if (level < last_level && last_level > 1 && n%(2*nstep) == 0) s->level[i] = last_level - 1;
+      else if (level > last_level) s->level[i] = level;
+      else s->level[i] = last_level;
+      s->levelneib[i] = level;
+      s->nlast[i] = n;
+      s->nstep[i] = ipow2(s->level_step - s->level[i]);
+      s->dt_next[i] = s->nstep[i]*s->timestep;
+      s->endstep[i] = 1;
+    }
+    if (s->level[i] > level_max) level_max = s->level[i];
+  }
+  const int istep = ipow2(s->level_step - level_max_old + 1);
+  if (level_max > level_max_old) {
+    const int nfactor = ipow2(level_max - level_max_old);
+    s->n *= nfactor;
+    for (int i=0; i<N; i++) { s->nstep[i] *= nfactor; s->nlast[i] *= nfactor; }
+  } else if (level_max <= level_max_old - 1 && level_max_old > 1 && s->n%istep == 0) {
+    level_max = level_max_old - 1;
+    const int nfactor = ipow2(level_max_old - level_max);
+    s->n /= nfactor;
+    for (int i=0; i<N; i++) { s->nlast[i] /= nfactor; s->nstep[i] /= nfactor; }
+  } else {
+    level_max = level_max_old;
+  }
+  s->level_max = level_max;
+  s->level_step = level_max;
+  s->nresync = ipow2(s->level_step);
+  s->timestep = s->dt_max/(double) s->nresync;
+  for (int i=0; i<N; i++)
+    if (s->nlast[i] == s->n) s->nstep[i] = ipow2(s->level_step - s->level[i]);
+}
+
+/* SphLeapfrogKDK::CheckTimesteps (SphLeapfrogKDK.cpp:284-329) */
+static int check_timesteps(orc_sim* s) {
+  int activecount = 0;
+  for (int i=0; i<s->N; i++) {
+    const int dn = s->n - s->nlast[i];
+    if (dn == s->nstep[i]) continue;
+    if (s->levelneib[i] - s->level[i] > s->p.level_diff_max) {
+      const int level_new = s->levelneib[i] - s->p.level_diff_max;
+      const int nnewstep = ipow2(s->level_step - level_new);
+      if (nnewstep > 0 && dn%nnewstep == 0) {
+        if (dn > 0) s->nstep[i] = dn;
+        s->level[i] = level_new;
+        s->active[i] = 1;
+        activecount++;
+      }
+    }
+  }
+  return activecount;
+}
+
+/* SphLeapfrogKDK::EndTimestep (SphLeapfrogKDK.cpp:219-272), particles flagged end_timestep */
+static void end_timestep(orc_sim* s) {
+  const int ei = s->p.energy_integration;
+  long long nf = 0;
+  for (int i=0; i<s->N; i++) {
+    if (!s->endstep[i]) continue;
+    nf++;
+    const double dt = s->pdt[i];
+    for (int k=0; k<3; k++) s->v[3*i+k] += 0.5*dt*(s->a[3*i+k] - s->a0[3*i+k]);
+    for (int k=0; k<3; k++) s->r0[3*i+k] = s->r[3*i+k];
+    for (int k=0; k<3; k++) s->v0[3*i+k] = s->v[3*i+k];
+    for (int k=0; k<3; k++) s->a0[3*i+k] = s->a[3*i+k];
+    if (ei) {
+      s->u[i] += 0.5*(s->dudt[i] - s->dudt0[i])*dt;
+      if (s->u[i] <= 0.0) s->u[i] = s->u0[i] + s->dudt0[i]*dt;
+      s->u0[i] = s->u[i];
+      s->dudt0[i] = s->dudt[i];
+    }
+    s->nlast[i] = s->n;
+    s->ptlast[i] = s->t;
+    s->pdt[i] = s->dt_next[i];
+    s->dt_next[i] = 0.0;
+    s->active[i] = 0;
+    s->endstep[i] = 0;
+  }
+  s->nfinished = nf;
 }
 
 /* SphSimulation.cpp:204-565 with file initial conditions (no h provided) */
@@ -1351,42 +1584,83 @@ int orc_sim_setup(orc_sim* s) {
                     s->u, s->sound, s->pressure)) return -1;
   if (orc_compute_h(p, N, s->r, s->m, 0, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
                     s->u, s->sound, s->pressure)) return -1;
-  if (sim_forces(s)) return -1;
   s->t = 0.0;
-  s->timestep = orc_timesteps(p, N, s->h, s->sound, s->div_v, s->a, s->u, s->dudt, NULL);
-  memcpy(s->r0, s->r, sizeof(double)*3*N);
-  memcpy(s->v0, s->v, sizeof(double)*3*N);
-  memcpy(s->a0, s->a, sizeof(double)*3*N);
-  memcpy(s->u0, s->u, sizeof(double)*N);
-  memcpy(s->dudt0, s->dudt, sizeof(double)*N);
+  s->n = 0; s->nresync = 0;
+  for (int i=0; i<N; i++) { s->level[i] = 0; s->levelneib[i] = 0; s->ptlast[i] = s->t; }
+  if (sim_forces(s, NULL)) return -1;
+  if (p->nlevels > 1) {
+    block_timesteps(s);
+    end_timestep(s);          /* pdt = 0: sets r0, v0, a0, u0, dudt0 and the first dt */
+  } else {
+    s->timestep = orc_timesteps(p, N, s->h, s->sound, s->div_v, s->a, s->u, s->dudt, NULL);
+    memcpy(s->r0, s->r, sizeof(double)*3*N);
+    memcpy(s->v0, s->v, sizeof(double)*3*N);
+    memcpy(s->a0, s->a, sizeof(double)*3*N);
+    memcpy(s->u0, s->u, sizeof(double)*N);
+    memcpy(s->dudt0, s->dudt, sizeof(double)*N);
+  }
   s->tlast = s->t;
   return 0;
 }
 
-/* SphSimulation.cpp:574-880 with Nlevels = 1 */
+static void wrap_with_r0(orc_sim* s) {
+  const orc_params* p = &s->p;
+  if (!p->periodic) return;
+  for (int i=0; i<s->N; i++)
+    for (int k=0; k<3; k++) {
+      double size = p->boxmax[k] - p->boxmin[k];
+      if (s->r[3*i+k] < p->boxmin[k]) { s->r[3*i+k] += size; s->r0[3*i+k] += size; }
+      if (s->r[3*i+k] > p->boxmax[k]) { s->r[3*i+k] -= size; s->r0[3*i+k] -= size; }
+    }
+}
+
+/* SphSimulation::MainLoop (SphSimulation.cpp:574-880) with block timesteps */
+static int sim_step_block(orc_sim* s) {
+  const int N = s->N;
+  const orc_params* p = &s->p;
+  s->n = s->n + 1;
+  s->t = s->t + s->timestep;
+  /* SphLeapfrogKDK::AdvanceParticles (SphLeapfrogKDK.cpp:76-127) */
+  for (int i=0; i<N; i++) {
+    const double dt = s->t - s->ptlast[i];
+    for (int k=0; k<3; k++)
+      s->r[3*i+k] = s->r0[3*i+k] + s->v0[3*i+k]*dt + 0.5*s->a0[3*i+k]*dt*dt;
+    for (int k=0; k<3; k++) s->v[3*i+k] = s->v0[3*i+k] + s->a0[3*i+k]*dt;
+    if (p->energy_integration) s->u[i] = s->u0[i] + s->dudt0[i]*dt;
+    s->active[i] = (s->n - s->nlast[i] == s->nstep[i]);
+  }
+  wrap_with_r0(s);
+  int activecount;
+  do {
+    if (sim_forces(s, s->active)) return -1;
+    memset(s->active, 0, (size_t) N);
+    activecount = check_timesteps(s);
+  } while (activecount > 0);
+  block_timesteps(s);
+  end_timestep(s);
+  s->tlast = s->t;
+  return 0;
+}
+
+/* SphSimulation.cpp:574-880 */
 int orc_sim_step(orc_sim* s) {
   const int N = s->N;
   const orc_params* p = &s->p;
+  if (p->nlevels > 1) return sim_step_block(s);
   const double dtstep = s->timestep;
   s->t = s->t + s->timestep;
   const double dt = s->t - s->tlast;
   orc_advance(N, dt, p->energy_integration, s->r0, s->v0, s->a0, s->u0, s->dudt0, s->r, s->v,
               s->u);
-  if (p->periodic) {
-    for (int i=0; i<N; i++)
-      for (int k=0; k<3; k++) {
-        double size = p->boxmax[k] - p->boxmin[k];
-        if (s->r[3*i+k] < p->boxmin[k]) { s->r[3*i+k] += size; s->r0[3*i+k] += size; }
-        if (s->r[3*i+k] > p->boxmax[k]) { s->r[3*i+k] -= size; s->r0[3*i+k] -= size; }
-      }
-  }
-  if (sim_forces(s)) return -1;
+  wrap_with_r0(s);
+  if (sim_forces(s, NULL)) return -1;
   const double newstep = orc_timesteps(p, N, s->h, s->sound, s->div_v, s->a, s->u, s->dudt,
                                        NULL);
   orc_end_step(N, dtstep, p->energy_integration, s->a, s->dudt, s->v, s->u, s->r0, s->v0,
                s->a0, s->u0, s->dudt0, s->r);
   s->tlast = s->t;
   s->timestep = newstep;
+  s->nfinished = N;
   return 0;
 }
 
@@ -1397,13 +1671,26 @@ int orc_sim_get(orc_sim* s, const char* f, double* out) {
     {"h", s->h, 1}, {"rho", s->rho, 1}, {"u", s->u, 1}, {"dudt", s->dudt, 1},
     {"div_v", s->div_v, 1}, {"gpot", s->gpot, 1}, {"invomega", s->invomega, 1},
     {"zeta", s->zeta, 1}, {"sound", s->sound, 1}, {"pressure", s->pressure, 1},
-    {"hfactor", s->hfactor, 1}};
+    {"hfactor", s->hfactor, 1}, {"tlast", s->ptlast, 1}, {"dt", s->pdt, 1}};
   for (unsigned k=0; k<sizeof(tab)/sizeof(tab[0]); k++)
     if (!strcmp(tab[k].name, f)) {
       memcpy(out, tab[k].p, sizeof(double)*N*tab[k].w);
+      return 0;
+    }
+  struct { const char* name; int* p; } itab[] = {
+    {"level", s->level}, {"levelneib", s->levelneib}, {"nstep", s->nstep}, {"nlast", s->nlast}};
+  for (unsigned k=0; k<sizeof(itab)/sizeof(itab[0]); k++)
+    if (!strcmp(itab[k].name, f)) {
+      for (int i=0; i<N; i++) out[i] = (double) itab[k].p[i];
       return 0;
     }
   return -1;
 }
 double orc_sim_time(orc_sim* s) { return s->t; }
 double orc_sim_timestep(orc_sim* s) { return s->timestep; }
+
+/* n, level_max, level_step, nresync, dt_max, particles that ended a step in the last step */
+void orc_sim_block_state(orc_sim* s, int* iout, double* dout) {
+  iout[0] = s->n; iout[1] = s->level_max; iout[2] = s->level_step; iout[3] = s->nresync;
+  dout[0] = s->dt_max; dout[1] = (double) s->nfinished;
+}

@@ -31,6 +31,10 @@ typedef struct {
   int periodic;            /* 1 = all three axes periodic */
   double boxmin[3], boxmax[3];
   double courant_mult, accel_mult, energy_mult;
+  int nlevels;             /* block timestep levels (1 = global timestep) */
+  int level_diff_max;      /* max. level difference to neighbours (Simulation.cpp:1209) */
+  int grav_direct;         /* 1: the sim driver's gravity is the exact direct sum, as the
+                              reference's neib_search = bruteforce (BruteForceTree.cpp:214) */
 } orc_params;
 
 enum { ORC_W0 = 0, ORC_W1, ORC_WOMEGA, ORC_WZETA, ORC_WGRAV, ORC_WPOT,
@@ -145,6 +149,9 @@ int orc_sim_step(orc_sim* s);
 int orc_sim_get(orc_sim* s, const char* field, double* out);
 double orc_sim_time(orc_sim* s);
 double orc_sim_timestep(orc_sim* s);
+/* block-timestep state: iout = n, level_max, level_step, nresync; dout = dt_max, number of
+ * particles that ended a step in the last MainLoop */
+void orc_sim_block_state(orc_sim* s, int* iout, double* dout);
 
 #ifdef __cplusplus
 }

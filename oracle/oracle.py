@@ -30,6 +30,7 @@ class Params(C.Structure):
         ("thetamaxsqd", C.c_double), ("nleafmax", C.c_int), ("periodic", C.c_int),
         ("boxmin", C.c_double * 3), ("boxmax", C.c_double * 3),
         ("courant_mult", C.c_double), ("accel_mult", C.c_double), ("energy_mult", C.c_double),
+        ("nlevels", C.c_int), ("level_diff_max", C.c_int), ("grav_direct", C.c_int),
     ]
 
 
@@ -93,6 +94,7 @@ def lib():
         L.orc_sim_get.argtypes = [vp, C.c_char_p, vp]; L.orc_sim_get.restype = i
         L.orc_sim_time.argtypes = [vp]; L.orc_sim_time.restype = d
         L.orc_sim_timestep.argtypes = [vp]; L.orc_sim_timestep.restype = d
+        L.orc_sim_block_state.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -124,6 +126,9 @@ def params_from_dict(prm: dict) -> Params:
     p.courant_mult = float(prm.get("courant_mult", 0.15))
     p.accel_mult = float(prm.get("accel_mult", 0.3))
     p.energy_mult = float(prm.get("energy_mult", 0.4))
+    p.nlevels = int(prm.get("Nlevels", 1))
+    p.level_diff_max = int(prm.get("level_diff_max", 1))
+    p.grav_direct = 1 if prm.get("neib_search", "kdtree") == "bruteforce" else 0
     return p
 
 
@@ -282,7 +287,7 @@ def diagnostics(p: Params, r, v, m, u, gpot):
 
 
 class Sim:
-    """Full-step oracle driver (reference MainLoop with global timesteps)."""
+    """Full-step oracle driver (reference MainLoop with global or block timesteps)."""
 
     FIELDS3 = ("r", "v", "a", "atree")
 
@@ -320,3 +325,10 @@ class Sim:
     @property
     def timestep(self):
         return lib().orc_sim_timestep(self._s)
+
+    def block_state(self):
+        """(n, level_max, level_step, nresync, dt_max, particles that ended a step last)"""
+        iv = np.zeros(4, dtype=np.int32)
+        dv = np.zeros(2)
+        lib().orc_sim_block_state(self._s, _p(iv), _p(dv))
+        return int(iv[0]), int(iv[1]), int(iv[2]), int(iv[3]), float(dv[0]), int(dv[1])

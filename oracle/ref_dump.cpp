@@ -22,7 +22,7 @@
 static const char* FIELD_NAMES[] = {
   "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "atx", "aty", "atz",
   "m", "h", "rho", "u", "dudt", "div_v", "gpot", "invomega", "zeta", "sound",
-  "pressure", "hfactor", "iorig", "levelneib", "dt"
+  "pressure", "hfactor", "iorig", "levelneib", "dt", "level", "nstep", "nlast", "tlast"
 };
 static const int NFIELDS = sizeof(FIELD_NAMES)/sizeof(FIELD_NAMES[0]);
 
@@ -54,6 +54,10 @@ static void dump(Simulation<3>* sim, const std::string& fname)
     f[(size_t)(k++)*N + i] = p.iorig;
     f[(size_t)(k++)*N + i] = p.levelneib;
     f[(size_t)(k++)*N + i] = p.dt;
+    f[(size_t)(k++)*N + i] = p.level;
+    f[(size_t)(k++)*N + i] = p.nstep;
+    f[(size_t)(k++)*N + i] = p.nlast;
+    f[(size_t)(k++)*N + i] = p.tlast;
   }
   FILE* fp = fopen(fname.c_str(), "wb");
   if (!fp) { perror(fname.c_str()); exit(2); }
@@ -67,9 +71,16 @@ static void diag_line(Simulation<3>* sim, FILE* fp)
 {
   sim->CalculateDiagnostics();
   const Diagnostics<3>& d = sim->diag;
-  fprintf(fp, "%d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+  // block-timestep structure: integer time, levels, and the particles that began a new step
+  // at this integer time (nlast == n)
+  Hydrodynamics<3>* hydro = sim->hydro;
+  int nnew = 0;
+  for (int i=0; i<hydro->Nhydro; i++) if (hydro->GetParticlePointer(i).nlast == sim->n) nnew++;
+  fprintf(fp, "%d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g"
+          " %d %d %d %d %.17g %d\n",
           sim->Nsteps, (double) sim->t, (double) sim->timestep, d.Etot, d.ketot, d.utot, d.gpetot,
-          d.mom[0], d.mom[1], d.mom[2], d.angmom[0], d.angmom[1], d.angmom[2], d.mtot);
+          d.mom[0], d.mom[1], d.mom[2], d.angmom[0], d.angmom[1], d.angmom[2], d.mtot,
+          sim->n, sim->level_max, sim->level_step, sim->nresync, (double) sim->dt_max, nnew);
   fflush(fp);
 }
 
@@ -93,7 +104,8 @@ int main(int argc, char** argv)
   Simulation<3>* sim = static_cast<Simulation<3>*>(base);
 
   FILE* dfp = fopen((prefix + "_diag.txt").c_str(), "w");
-  fprintf(dfp, "# Nsteps t timestep Etot ketot utot gpetot momx momy momz angx angy angz mtot\n");
+  fprintf(dfp, "# Nsteps t timestep Etot ketot utot gpetot momx momy momz angx angy angz mtot"
+          " n level_max level_step nresync dt_max nnew\n");
   diag_line(sim, dfp);
   dump(sim, prefix + "_s0.bin");
   for (int s=1; s<=nsteps; s++) {

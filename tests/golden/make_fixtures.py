@@ -27,7 +27,7 @@ from gandalf_amd import ic  # noqa: E402
 REF = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
 FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "atx", "aty", "atz", "m", "h",
           "rho", "u", "dudt", "div_v", "gpot", "invomega", "zeta", "sound", "pressure",
-          "hfactor", "iorig", "levelneib", "dt"]
+          "hfactor", "iorig", "levelneib", "dt", "level", "nstep", "nlast", "tlast"]
 KEEP = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "atx", "aty", "atz", "h", "rho",
         "u", "dudt", "div_v", "gpot", "invomega", "zeta", "sound", "pressure", "hfactor"]
 
@@ -53,7 +53,14 @@ CASES = {
     # rotating m=2 cloud, quintic kernel, barotropic EOS, brute-force gravity
     "cloud4k_quintic_bf": (lambda: ic.rotating_m2_cloud(N=4096, seed=14, rho_bary=1.0),
                            {"neib_search": "bruteforce"}, 3),
+    # block timesteps: hot blob in a periodic box over 4 levels, three block cycles
+    "blob16_m4_blk": (lambda: ic.hot_blob_box(n_side=16, seed=15), {}, 24),
+    # block timesteps with self-gravity: the cloud with 8 levels, brute-force gravity
+    "cloud4k_quintic_blk": (lambda: ic.rotating_m2_cloud(N=4096, seed=16, rho_bary=1.0,
+                                                         nlevels=8),
+                            {"neib_search": "bruteforce"}, 12),
 }
+BLOCK_KEEP = ["level", "nstep", "nlast", "tlast", "levelneib", "dt"]
 
 
 def run_case(name, gen, over, nsteps, outdir):
@@ -70,18 +77,19 @@ def run_case(name, gen, over, nsteps, outdir):
                        cwd=td, check=True, stdout=subprocess.DEVNULL)
         s0 = _load(os.path.join(td, "out_s0.bin"))
         arrs = {"in_r": g["r"], "in_v": g["v"], "in_m": g["m"], "in_u": g["u"]}
-        for k in KEEP:
+        keep = KEEP + (BLOCK_KEEP if prm.get("Nlevels", 1) > 1 else [])
+        for k in keep:
             arrs[f"s0_{k}"] = s0[k]
         if nsteps > 0:
             sN = _load(os.path.join(td, f"out_s{nsteps}.bin"))
-            for k in KEEP:
+            for k in keep:
                 arrs[f"s{nsteps}_{k}"] = sN[k]
         diag = np.loadtxt(os.path.join(td, "out_diag.txt"))
         arrs["diag"] = np.atleast_2d(diag)
     meta = {"params": prm, "nsteps": nsteps, "build": "serial, COMPILER_MODE=STANDARD, "
             "PRECISION=DOUBLE, DEBUG_LEVEL=1 (oracle/Makefile.ref)",
             "diag_columns": "Nsteps t timestep Etot ketot utot gpetot momx momy momz angx "
-                            "angy angz mtot"}
+                            "angy angz mtot n level_max level_step nresync dt_max nnew"}
     arrs["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez_compressed(os.path.join(outdir, f"{name}.npz"), **arrs)
     print(name, "ok", g["r"].shape[0], "particles")

@@ -89,3 +89,38 @@ def test_reference_energy_bookkeeping():
                         d["s0_u"], d["s0_gpot"])
     assert out[0] == pytest.approx(d["diag"][0, 3], rel=1e-12)
     assert out[3] == pytest.approx(d["diag"][0, 6], rel=1e-12)
+
+
+@pytest.mark.parametrize("case", ["blob16_m4_blk", "cloud4k_quintic_blk"])
+def test_block_timesteps_match_reference(case):
+    """Block timesteps (Simulation.cpp:1764-2197, SphLeapfrogKDK.cpp:76-329) against the
+    reference run step by step: the integer time, level_max, level_step, nresync, dt_max and
+    the number of particles ending a step agree exactly at every step (including the steps
+    where the reference adds or removes a level and rescales the integer time), and after the
+    last step every particle's level, nstep, nlast and levelneib agree exactly and r, v, u,
+    h to rounding."""
+    d = load_golden(case)
+    p, s = _setup_and_run(d)
+    diag = d["diag"]
+    n0 = s.block_state()
+    assert n0[:4] == tuple(int(x) for x in diag[0, 14:18])
+    assert n0[4] == pytest.approx(diag[0, 18], rel=1e-14)
+    assert s.timestep == pytest.approx(diag[0, 2], rel=1e-14)
+    for f in ("level", "levelneib", "nstep", "nlast"):
+        assert np.array_equal(s.get(f), d[f"s0_{f}"]), f
+    nsteps = d["meta"]["nsteps"]
+    for k in range(1, nsteps + 1):
+        s.step()
+        n, lmax, lstep, nres, dtmax, nfin = s.block_state()
+        assert (n, lmax, lstep, nres) == tuple(int(x) for x in diag[k, 14:18]), k
+        assert dtmax == pytest.approx(diag[k, 18], rel=1e-13), k
+        assert nfin == int(diag[k, 19]), k
+        assert s.t == pytest.approx(diag[k, 1], rel=1e-13), k
+        assert s.timestep == pytest.approx(diag[k, 2], rel=1e-13), k
+    for f in ("level", "levelneib", "nstep", "nlast"):
+        assert np.array_equal(s.get(f), d[f"s{nsteps}_{f}"]), f
+    assert np.allclose(s.get("tlast"), d[f"s{nsteps}_tlast"], rtol=1e-13, atol=0)
+    assert np.max(np.abs(s.get("r") - vec(d, f"s{nsteps}"))) < 1e-11
+    assert relmax(s.get("v"), vec(d, f"s{nsteps}", ("vx", "vy", "vz"))) < 1e-9
+    assert relmax(s.get("u"), d[f"s{nsteps}_u"]) < 1e-11
+    assert relmax(s.get("h"), d[f"s{nsteps}_h"]) < 1e-11
