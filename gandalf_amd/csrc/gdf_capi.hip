@@ -51,6 +51,20 @@ __global__ void k_iota(int N, int* p) {
   if (i < N) p[i] = i;
 }
 
+// integer per-particle state exported as doubles
+int int_field(const char* name) {
+  static const char* names[] = {"level", "levelneib", "nstep", "nlast"};
+  static const int ids[] = {I_LEVEL, I_LEVELNEIB, I_NSTEP, I_NLAST};
+  for (int k = 0; k < 4; k++) if (!strcmp(name, names[k])) return ids[k];
+  return -1;
+}
+
+__global__ void k_scatter_out_int(int N, const int* __restrict__ src, const int* __restrict__ io,
+                                  double* __restrict__ out) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i < N) out[io[i]] = (double) src[i];
+}
+
 // resolve a field name to up to three device arrays
 bool field_ptrs(gdf_ctx* c, const char* name, double** p, int* w) {
   struct E { const char* n; int src; int f; int w; };
@@ -61,7 +75,8 @@ bool field_ptrs(gdf_ctx* c, const char* name, double** p, int* w) {
     {"a", 1, D_AX, 3}, {"atree", 1, D_ATX, 3}, {"rho", 1, D_RHO, 1}, {"dudt", 1, D_DUDT, 1},
     {"div_v", 1, D_DIVV, 1}, {"gpot", 1, D_GPOT, 1}, {"invomega", 1, D_INVOMEGA, 1},
     {"zeta", 1, D_ZETA, 1}, {"sound", 1, D_SOUND, 1}, {"pressure", 1, D_PRESSURE, 1},
-    {"hfactor", 1, D_HFACTOR, 1}, {"dt", 1, D_DT, 1}, {"adot", 1, D_ADOTX, 3}};
+    {"hfactor", 1, D_HFACTOR, 1}, {"dt", 1, D_DT, 1}, {"adot", 1, D_ADOTX, 3},
+    {"tlast", 0, F_TLAST, 1}, {"dtstep", 0, F_PDT, 1}};
   for (const E& e : tab) {
     if (strcmp(e.n, name)) continue;
     for (int k = 0; k < e.w; k++) p[k] = e.src == 0 ? c->pf[e.f + k] : c->df[e.f + k];
@@ -73,7 +88,10 @@ bool field_ptrs(gdf_ctx* c, const char* name, double** p, int* w) {
 
 void free_particles(gdf_ctx* c) {
   for (int f = 0; f < F_NPERSIST; f++) { hipFree(c->pf[f]); hipFree(c->pf_alt[f]); c->pf[f] = c->pf_alt[f] = nullptr; }
-  for (int f = 0; f < D_NDERIVED; f++) { hipFree(c->df[f]); c->df[f] = nullptr; }
+  for (int f = 0; f < D_NDERIVED; f++) {
+    hipFree(c->df[f]); hipFree(c->df_alt[f]); c->df[f] = c->df_alt[f] = nullptr;
+  }
+  for (int f = 0; f < I_NINT; f++) { hipFree(c->pi[f]); hipFree(c->pi_alt[f]); c->pi[f] = c->pi_alt[f] = nullptr; }
   void* ptrs[] = {c->iorig, c->iorig_alt, c->posm, c->hrecA, c->hrecB, c->grec, c->status,
                   c->keys, c->keys_alt, c->vals, c->vals_alt, c->sort_tmp, c->left, c->right,
                   c->first, c->last, c->parent, c->visit, c->front, c->bbmin, c->bbmax, c->geo_c,
@@ -100,7 +118,14 @@ int alloc_particles(gdf_ctx* c, int N) {
     GDF_HIP(c, hipMalloc(&c->pf[f], sizeof(double)*n));
     GDF_HIP(c, hipMalloc(&c->pf_alt[f], sizeof(double)*n));
   }
-  for (int f = 0; f < D_NDERIVED; f++) GDF_HIP(c, hipMalloc(&c->df[f], sizeof(double)*n));
+  for (int f = 0; f < D_NDERIVED; f++) {
+    GDF_HIP(c, hipMalloc(&c->df[f], sizeof(double)*n));
+    if (c->nlevels > 1) GDF_HIP(c, hipMalloc(&c->df_alt[f], sizeof(double)*n));
+  }
+  for (int f = 0; f < I_NINT; f++) {
+    GDF_HIP(c, hipMalloc(&c->pi[f], sizeof(int)*n));
+    GDF_HIP(c, hipMalloc(&c->pi_alt[f], sizeof(int)*n));
+  }
   GDF_HIP(c, hipMalloc(&c->iorig, sizeof(int)*n));
   GDF_HIP(c, hipMalloc(&c->iorig_alt, sizeof(int)*n));
   GDF_HIP(c, hipMalloc(&c->posm, sizeof(double4)*n));
@@ -202,6 +227,7 @@ int gdf_config_default(gdf_config* cfg) {
   cfg->group_max = 64;
   cfg->nbody = 0; cfg->nbody_direct = 0; cfg->nbody_eps = 0.0;
   cfg->nbody_mult = 0.1; cfg->fixed_dt = 0.0;
+  cfg->Nlevels = 1; cfg->level_diff_max = 1;
   return GDF_OK;
 }
 
@@ -216,6 +242,8 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
   if (cfg->nbody_direct < 0 || cfg->nbody_direct > 2 || cfg->nbody_eps < 0.0 ||
       cfg->fixed_dt < 0.0)
     return GDF_ERR_INVALID;
+  if (cfg->Nlevels < 1 || cfg->Nlevels > 20 || cfg->level_diff_max < 1) return GDF_ERR_INVALID;
+  if (cfg->Nlevels > 1 && (cfg->nbody || cfg->fixed_dt > 0.0)) return GDF_ERR_INVALID;
   if (cfg->periodic)
     for (int k = 0; k < 3; k++)
       if (!(cfg->boxmax[k] > cfg->boxmin[k]) || cfg->boxmax[k] - cfg->boxmin[k] > 1e20)
@@ -252,6 +280,7 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
   if (p.nbody) { p.hydro_forces = 0; p.self_gravity = 1; p.energy_integration = 0; }
   p.eps2 = cfg->nbody_eps*cfg->nbody_eps;
   p.nbody_mult = cfg->nbody_mult;
+  c->nlevels = cfg->Nlevels; c->level_diff_max = cfg->level_diff_max;
   int rc = upload_kernel_tables(c);
   if (rc == GDF_OK) {
     if (hipMalloc(&c->counters, sizeof(int)*16) != hipSuccess ||
@@ -317,7 +346,9 @@ int gdf_set_particles(gdf_ctx* c, int N, const double* r, const double* v, const
   GDF_TRY(up(c->pf[F_U], u, 1, 0));
   GDF_TRY(up(c->pf[F_U0], u, 1, 0));
   GDF_TRY(up(c->pf[F_DUDT0], nullptr, 1, 0));
+  for (int f : {F_TLAST, F_PDT, F_DTNEXT}) GDF_TRY(up(c->pf[f], nullptr, 1, 0));
   for (int f = 0; f < D_NDERIVED; f++) GDF_HIP(c, hipMemset(c->df[f], 0, sizeof(double)*n));
+  for (int f = 0; f < I_NINT; f++) GDF_HIP(c, hipMemset(c->pi[f], 0, sizeof(int)*n));
   k_iota<<<nblocks(N, BS), BS, 0, c->stream>>>(N, c->iorig);
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   c->tree_valid = false; c->sg_valid = false; c->setup_done = false;
@@ -330,6 +361,14 @@ int gdf_get_field(gdf_ctx* c, const char* name, double* out) {
   if (c->N <= 0) return fail(c, GDF_ERR_INVALID, "no particles");
   double* p[3] = {nullptr, nullptr, nullptr};
   int w = 0;
+  const int fi = int_field(name);
+  if (fi >= 0) {
+    k_scatter_out_int<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, c->pi[fi], c->iorig, c->tmpd);
+    GDF_HIP(c, hipGetLastError());
+    GDF_HIP(c, hipMemcpyAsync(out, c->tmpd, sizeof(double)*(size_t) c->N, hipMemcpyDeviceToHost, c->stream));
+    GDF_HIP(c, hipStreamSynchronize(c->stream));
+    return GDF_OK;
+  }
   if (!field_ptrs(c, name, p, &w)) return fail(c, GDF_ERR_INVALID, std::string("unknown field ") + name);
   k_scatter_out<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, w, p[0], p[1], p[2], c->iorig, c->tmpd);
   GDF_HIP(c, hipGetLastError());
@@ -421,6 +460,16 @@ int gdf_time(const gdf_ctx* c, double* t, double* timestep, int64_t* nsteps) {
   if (t) *t = c->t;
   if (timestep) *timestep = c->timestep;
   if (nsteps) *nsteps = c->nsteps;
+  return GDF_OK;
+}
+
+int gdf_block_state(const gdf_ctx* c, int64_t* iout5, double* dout1) {
+  if (!c) return GDF_ERR_INVALID;
+  if (iout5) {
+    iout5[0] = c->n_int; iout5[1] = c->level_max; iout5[2] = c->level_step;
+    iout5[3] = c->nresync; iout5[4] = c->finished_last;
+  }
+  if (dout1) dout1[0] = c->dt_max;
   return GDF_OK;
 }
 

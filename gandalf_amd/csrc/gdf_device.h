@@ -274,6 +274,25 @@ __device__ __forceinline__ double wave_max_d(double v) {
   return v;
 }
 __device__ __forceinline__ int wave_bcast(int v, int src) { return __shfl(v, src, 64); }
+// sum over the wavefront, one atomic add by lane 0; every lane of the wave must take part
+__device__ __forceinline__ void wave_add_ll(long long* dst, long long v) {
+  #pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if (__lane_id() == 0 && v != 0) atomicAdd((unsigned long long*) dst, (unsigned long long) v);
+}
+
+// GradhSph::ComputeThermalProperties (GradhSph.cpp:335-347) with the adiabatic
+// (AdiabaticEOS.cpp), isothermal (IsothermalEOS.cpp) and barotropic (BarotropicEOS.cpp) closures:
+// u (replaced for the last two), sound speed and pressure from rho and u
+__device__ __forceinline__ void eos_state(const PhysParams& p, double rho, double& u,
+                                          double& snd, double& press) {
+  if (p.eos == 1) { u = p.temp0/p.gammam1/p.mu_bar; snd = sqrt(p.gammam1*u); }
+  else if (p.eos == 2) {
+    u = p.temp0*(1.0 + pow(rho*p.invrho_bary, p.gammam1))/p.gammam1/p.mu_bar;
+    snd = sqrt(p.gammam1*u);
+  } else snd = sqrt(p.gamma*p.gammam1*u);
+  press = p.gammam1*rho*u;
+}
 
 // Next work item for a persistent wavefront.  Every active lane takes part in the atomic (the
 // first active lane adds one, the others zero) and the result is broadcast with readfirstlane,

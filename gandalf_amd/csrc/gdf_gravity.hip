@@ -43,6 +43,7 @@ struct GravArgs {
   double* atx; double* aty; double* atz; double* gpot;
   int* stacks; int* counter; int* flags; long long* icount;
   int32_t* gnc; int32_t* gnn;
+  const int* act;         // block timesteps: only active targets (NULL = all)
   PhysParams p;
 };
 
@@ -145,8 +146,13 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
     const int g = wave_next_group(a.counter, a.ngroups, range);
     if (g >= a.ngroups) break;
     const int G = a.grp_node[g], gfirst = a.grp_first[g], gcnt = a.grp_count[g];
-    const bool has = lane < gcnt;
+    const bool has = lane < gcnt && (!a.act || a.act[gfirst + lane]);
     const int i = gfirst + lane;
+    const int nact = __popcll(__ballot(has));
+    if (nact == 0) {                 // no active target in this group
+      if (lane == 0) { a.gnc[g] = 0; a.gnn[g] = 0; }
+      continue;
+    }
     const double4 gc = a.geo_c[G], gh = a.geo_h[G];
     const double hmaxG = gc.w;
     double4 me = make_double4(0, 0, 0, 0), mr = make_double4(1, 0, 0, 0);
@@ -358,8 +364,8 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
     if (lane == 0) {
       a.gnc[g] = ncell;
       a.gnn[g] = nnear;
-      atomicAdd((unsigned long long*) &a.icount[2], (unsigned long long) nnear*gcnt);
-      atomicAdd((unsigned long long*) &a.icount[3], (unsigned long long) ncell*gcnt);
+      atomicAdd((unsigned long long*) &a.icount[2], (unsigned long long) nnear*nact);
+      atomicAdd((unsigned long long*) &a.icount[3], (unsigned long long) ncell*nact);
     }
   }
 }
@@ -489,6 +495,7 @@ int gravity_walk(gdf_ctx* c) {
   a.atx = c->df[D_ATX]; a.aty = c->df[D_ATY]; a.atz = c->df[D_ATZ]; a.gpot = c->df[D_GPOT];
   a.stacks = c->stacks; a.counter = c->counters + 4; a.flags = c->counters;
   a.icount = c->icount; a.gnc = c->grav_nc; a.gnn = c->grav_nn;
+  a.act = c->nlevels > 1 ? c->pi[I_ACT] : nullptr;
   a.p = c->phys;
   k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
   k_zero8<<<1, 64, 0, s>>>(c->counters);

@@ -190,9 +190,10 @@ __global__ void k_zero_d(int n, double* __restrict__ p) {
 
 __global__ void k_add_atree(int N, double* __restrict__ ax, double* __restrict__ ay,
                             double* __restrict__ az, const double* __restrict__ tx,
-                            const double* __restrict__ ty, const double* __restrict__ tz) {
+                            const double* __restrict__ ty, const double* __restrict__ tz,
+                            const int* __restrict__ act) {
   const int i = blockIdx.x*BS + threadIdx.x;
-  if (i >= N) return;
+  if (i >= N || (act && !act[i])) return;
   ax[i] += tx[i]; ay[i] += ty[i]; az[i] += tz[i];
 }
 
@@ -313,7 +314,8 @@ int integ_zero_forces(gdf_ctx* c) {
 int integ_add_atree(gdf_ctx* c) {
   k_add_atree<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, c->df[D_AX], c->df[D_AY],
                                                         c->df[D_AZ], c->df[D_ATX], c->df[D_ATY],
-                                                        c->df[D_ATZ]);
+                                                        c->df[D_ATZ],
+                                                        c->nlevels > 1 ? c->pi[I_ACT] : nullptr);
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
 }
@@ -321,6 +323,347 @@ int integ_add_atree(gdf_ctx* c) {
 int integ_pack_posm(gdf_ctx* c) {
   k_pack_posm<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, c->pf[F_X], c->pf[F_Y],
                                                         c->pf[F_Z], c->pf[F_M], c->posm);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Block timesteps: SphLeapfrogKDK::AdvanceParticles / CheckTimesteps / EndTimestep with per-
+// particle steps (SphLeapfrogKDK.cpp:76-329) and Simulation::ComputeBlockTimesteps
+// (Simulation.cpp:1764-2197) for hydro particles, KDK (integration_step = 1), no sinks,
+// sph_single_timestep = 0.  Integer state per particle: level, levelneib, nstep, nlast, and the
+// active / end_timestep flags; the integer time n, level_max, level_step, nresync and dt_max
+// are host scalars, updated from two small reductions per step.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+// (int) pow(2, e) as the reference computes step lengths: 0 for negative e
+__device__ __host__ __forceinline__ int ipow2(int e) { return e >= 0 ? (1 << e) : 0; }
+
+// ComputeTimestepLevel (InlineFuncs.h:550-557)
+__device__ __forceinline__ int timestep_level(double dt, double dt_max) {
+  const int level = (int) (1.44269504088896*log(dt_max/dt)) + 1;
+  return level > 0 ? level : 0;
+}
+
+struct BlkArgs {
+  int N, n, level_step, level_max, ldm, energy;
+  double t, timestep, dt_max;
+  double* r[3]; double* v[3]; const double* r0[3]; const double* v0[3]; const double* a0[3];
+  double* u; const double* u0; const double* dudt0;
+  double* tlast; double* pdt; double* dtnext;
+  int* level; int* levelneib; int* nstep; int* nlast; int* act; int* end;
+  const double* dtw;
+  int* counter; long long* finished;
+};
+
+__global__ void k_block_init(BlkArgs b) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= b.N) return;
+  b.level[i] = 0; b.levelneib[i] = 0; b.nstep[i] = 0; b.nlast[i] = 0;
+  b.act[i] = 1; b.end[i] = 0;
+  b.tlast[i] = b.t; b.pdt[i] = 0.0; b.dtnext[i] = 0.0;
+}
+
+// drift every particle from the start of its own step; active = at the end of its step
+__global__ void k_block_advance(BlkArgs b) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= b.N) return;
+  const double dt = b.t - b.tlast[i];
+  for (int k = 0; k < 3; k++) b.r[k][i] = b.r0[k][i] + b.v0[k][i]*dt + 0.5*b.a0[k][i]*dt*dt;
+  for (int k = 0; k < 3; k++) b.v[k][i] = b.v0[k][i] + b.a0[k][i]*dt;
+  if (b.energy) b.u[i] = b.u0[i] + b.dudt0[i]*dt;
+  b.act[i] = (b.n - b.nlast[i] == b.nstep[i]) ? 1 : 0;
+}
+
+// Sph::ZeroAccelerations (Sph.cpp:126-140) for the active particles
+struct ZeroArgs { int N; const int* act; int* levelneib; double* f[9]; };
+__global__ void k_zero_active(ZeroArgs z) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= z.N || !z.act[i]) return;
+  z.levelneib[i] = 0;
+  #pragma unroll
+  for (int k = 0; k < 9; k++) z.f[k][i] = 0.0;
+}
+
+// ComputeThermalProperties for every particle (SphSimulation.cpp:667-676 with nradstep = 1)
+__global__ void k_eos_all(int N, PhysParams p, const double* __restrict__ rho,
+                          double* __restrict__ u, double* __restrict__ sound,
+                          double* __restrict__ pressure) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= N) return;
+  double uu = u[i], snd, press;
+  eos_state(p, rho[i], uu, snd, press);
+  u[i] = uu; sound[i] = snd; pressure[i] = press;
+}
+
+// SphLeapfrogKDK::CheckTimesteps: end a step early where a neighbour's level is too far above;
+// every particle's active flag is rewritten (set only for these), counter += their number
+__global__ void k_check_timesteps(BlkArgs b) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  int newly = 0;
+  if (i < b.N) {
+    int a = 0;
+    const int dn = b.n - b.nlast[i];
+    const int level = b.level[i], ln = b.levelneib[i];
+    if (dn != b.nstep[i] && ln - level > b.ldm) {
+      const int level_new = ln - b.ldm;
+      const int nnewstep = ipow2(b.level_step - level_new);
+      if (nnewstep > 0 && dn%nnewstep == 0) {
+        if (dn > 0) b.nstep[i] = dn;
+        b.level[i] = level_new;
+        a = 1;
+      }
+    }
+    b.act[i] = a;
+    newly = a;
+  }
+  const long long tot = newly;
+  wave_add_ll(reinterpret_cast<long long*>(b.counter), tot);
+}
+
+// resynchronisation branch: every particle gets a fresh level from its timestep
+__global__ void k_block_resync(BlkArgs b) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= b.N) return;
+  const int level = min(timestep_level(b.dtw[i], b.dt_max), b.level_max);
+  b.level[i] = level;
+  b.levelneib[i] = level;
+  const int nstep = ipow2(b.level_step - level);
+  b.nstep[i] = nstep;
+  b.nlast[i] = b.n;
+  b.dtnext[i] = nstep*b.timestep;
+  b.end[i] = 1;
+}
+
+// particles at the end of their step choose their next level; counter = max level of all
+__global__ void k_block_update(BlkArgs b) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  int lv = 0;
+  if (i < b.N) {
+    const int n = b.n;
+    int level = b.level[i];
+    const int nstep = b.nstep[i], nlast = b.nlast[i];
+    if (n - nlast == nstep) {
+      int lnew = timestep_level(b.dtw[i], b.dt_max);
+      lnew = max(lnew, b.levelneib[i] - b.ldm);
+      if (nstep != ipow2(b.level_step - level)) {
+        // step ended early by CheckTimesteps
+        level = max(level, lnew);
+        b.levelneib[i] = level;
+      } else {
+        if (lnew < level && level > 1 && n%(2*nstep) == 0) level = level - 1;
+        else if (lnew > level) level = lnew;
+        b.levelneib[i] = lnew;
+      }
+      b.level[i] = level;
+      b.nlast[i] = n;
+      const int ns = ipow2(b.level_step - level);
+      b.nstep[i] = ns;
+      b.dtnext[i] = ns*b.timestep;
+      b.end[i] = 1;
+    }
+    lv = level;
+  }
+  lv = wave_max_int(lv);
+  if (__lane_id() == 0) atomicMax(b.counter, lv);
+}
+
+// integer time rescaled when level_max changes: mul = 1 multiplies nstep and nlast by f, else
+// divides them
+__global__ void k_block_rescale(BlkArgs b, int f, int mul) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= b.N) return;
+  if (mul) { b.nstep[i] *= f; b.nlast[i] *= f; }
+  else { b.nlast[i] /= f; b.nstep[i] /= f; }
+}
+
+__global__ void k_block_fix_nstep(BlkArgs b) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= b.N) return;
+  if (b.nlast[i] == b.n) b.nstep[i] = ipow2(b.level_step - b.level[i]);
+}
+
+// SphLeapfrogKDK::EndTimestep for the particles flagged end_timestep
+struct BlkEndArgs {
+  int N, n, energy;
+  double t;
+  const double* a[3]; double* v[3]; double* r0[3]; double* v0[3]; double* a0[3];
+  const double* r[3];
+  const double* dudt; double* u; double* u0; double* dudt0;
+  double* tlast; double* pdt; double* dtnext;
+  int* nlast; int* act; int* end;
+  long long* finished;
+};
+
+__global__ void k_block_end_step(BlkEndArgs e) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x*BS + threadIdx.x;
+  long long fin = 0;
+  if (i < e.N && e.end[i]) {
+    fin = 1;
+    const double dt = e.pdt[i];
+    for (int k = 0; k < 3; k++) {
+      const double a = e.a[k][i], a0 = e.a0[k][i];
+      const double v = e.v[k][i] + 0.5*dt*(a - a0);
+      e.v[k][i] = v;
+      e.r0[k][i] = e.r[k][i];
+      e.v0[k][i] = v;
+      e.a0[k][i] = a;
+    }
+    if (e.energy) {
+      const double dudt = e.dudt[i], dudt0 = e.dudt0[i];
+      double u = e.u[i] + 0.5*(dudt - dudt0)*dt;
+      if (u <= 0.0) u = e.u0[i] + dudt0*dt;
+      e.u[i] = u;
+      e.u0[i] = u;
+      e.dudt0[i] = dudt;
+    }
+    e.nlast[i] = e.n;
+    e.tlast[i] = e.t;
+    e.pdt[i] = e.dtnext[i];
+    e.dtnext[i] = 0.0;
+    e.act[i] = 0;
+    e.end[i] = 0;
+  }
+  wave_add_ll(e.finished, fin);
+}
+
+BlkArgs blk_args(gdf_ctx* c) {
+  BlkArgs b;
+  b.N = c->N; b.n = c->n_int; b.level_step = c->level_step; b.level_max = c->level_max;
+  b.ldm = c->level_diff_max; b.energy = c->phys.energy_integration;
+  b.t = c->t; b.timestep = c->timestep; b.dt_max = c->dt_max;
+  for (int k = 0; k < 3; k++) {
+    b.r[k] = c->pf[F_X + k]; b.v[k] = c->pf[F_VX + k];
+    b.r0[k] = c->pf[F_X0 + k]; b.v0[k] = c->pf[F_VX0 + k]; b.a0[k] = c->pf[F_AX0 + k];
+  }
+  b.u = c->pf[F_U]; b.u0 = c->pf[F_U0]; b.dudt0 = c->pf[F_DUDT0];
+  b.tlast = c->pf[F_TLAST]; b.pdt = c->pf[F_PDT]; b.dtnext = c->pf[F_DTNEXT];
+  b.level = c->pi[I_LEVEL]; b.levelneib = c->pi[I_LEVELNEIB]; b.nstep = c->pi[I_NSTEP];
+  b.nlast = c->pi[I_NLAST]; b.act = c->pi[I_ACT]; b.end = c->pi[I_END];
+  b.dtw = c->df[D_DT];
+  b.counter = c->counters + 12;
+  b.finished = c->icount + 6;
+  return b;
+}
+
+int read_counter(gdf_ctx* c, int* out) {
+  GDF_HIP(c, hipMemcpyAsync(c->h_counters + 12, c->counters + 12, sizeof(int)*2,
+                            hipMemcpyDeviceToHost, c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  *out = c->h_counters[12];
+  return GDF_OK;
+}
+
+int zero_counter(gdf_ctx* c) {
+  GDF_HIP(c, hipMemsetAsync(c->counters + 12, 0, sizeof(int)*2, c->stream));
+  return GDF_OK;
+}
+
+}  // namespace
+
+int block_init(gdf_ctx* c) {
+  c->n_int = 0; c->nresync = 0; c->level_max = 0; c->level_step = 0; c->dt_max = 0.0;
+  BlkArgs b = blk_args(c);
+  k_block_init<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int block_advance(gdf_ctx* c) {
+  BlkArgs b = blk_args(c);
+  k_block_advance<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
+  GDF_HIP(c, hipGetLastError());
+  c->tree_valid = false;
+  return GDF_OK;
+}
+
+int block_zero_active(gdf_ctx* c) {
+  ZeroArgs z;
+  z.N = c->N; z.act = c->pi[I_ACT]; z.levelneib = c->pi[I_LEVELNEIB];
+  const int fields[9] = {D_AX, D_AY, D_AZ, D_ATX, D_ATY, D_ATZ, D_DUDT, D_DIVV, D_GPOT};
+  for (int k = 0; k < 9; k++) z.f[k] = c->df[fields[k]];
+  k_zero_active<<<nblocks(c->N, BS), BS, 0, c->stream>>>(z);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int block_eos_all(gdf_ctx* c) {
+  k_eos_all<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, c->phys, c->df[D_RHO], c->pf[F_U],
+                                                      c->df[D_SOUND], c->df[D_PRESSURE]);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int block_check_timesteps(gdf_ctx* c, int* activecount) {
+  GDF_TRY(zero_counter(c));
+  BlkArgs b = blk_args(c);
+  k_check_timesteps<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
+  GDF_HIP(c, hipGetLastError());
+  return read_counter(c, activecount);
+}
+
+int block_timesteps(gdf_ctx* c) {
+  double dmin = 0.0;
+  GDF_TRY(integ_timestep(c, &dmin));      // per-particle criteria into D_DT, and their minimum
+  if (c->n_int == c->nresync) {
+    c->n_int = 0;
+    c->timestep = dmin;
+    c->level_max = c->nlevels - 1;
+    c->level_step = c->level_max;
+    c->dt_max = c->timestep*std::pow(2.0, c->level_max);
+    BlkArgs b = blk_args(c);
+    k_block_resync<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
+    GDF_HIP(c, hipGetLastError());
+    c->nresync = ipow2(c->level_step);
+    c->timestep = c->dt_max/(double) c->nresync;
+    return GDF_OK;
+  }
+  const int level_max_old = c->level_max;
+  GDF_TRY(zero_counter(c));
+  BlkArgs b = blk_args(c);
+  k_block_update<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
+  GDF_HIP(c, hipGetLastError());
+  int level_max = 0;
+  GDF_TRY(read_counter(c, &level_max));
+  const int istep = ipow2(c->level_step - level_max_old + 1);
+  if (level_max > level_max_old) {
+    const int nfactor = ipow2(level_max - level_max_old);
+    c->n_int *= nfactor;
+    k_block_rescale<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b, nfactor, 1);
+  } else if (level_max <= level_max_old - 1 && level_max_old > 1 && c->n_int%istep == 0) {
+    level_max = level_max_old - 1;
+    const int nfactor = ipow2(level_max_old - level_max);
+    c->n_int /= nfactor;
+    k_block_rescale<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b, nfactor, 0);
+  } else {
+    level_max = level_max_old;
+  }
+  GDF_HIP(c, hipGetLastError());
+  c->level_max = level_max;
+  c->level_step = level_max;
+  c->nresync = ipow2(c->level_step);
+  c->timestep = c->dt_max/(double) c->nresync;
+  b = blk_args(c);
+  k_block_fix_nstep<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int block_end_step(gdf_ctx* c) {
+  BlkEndArgs e;
+  e.N = c->N; e.n = c->n_int; e.energy = c->phys.energy_integration; e.t = c->t;
+  for (int k = 0; k < 3; k++) {
+    e.a[k] = c->df[D_AX + k]; e.v[k] = c->pf[F_VX + k]; e.r0[k] = c->pf[F_X0 + k];
+    e.v0[k] = c->pf[F_VX0 + k]; e.a0[k] = c->pf[F_AX0 + k]; e.r[k] = c->pf[F_X + k];
+  }
+  e.dudt = c->df[D_DUDT]; e.u = c->pf[F_U]; e.u0 = c->pf[F_U0]; e.dudt0 = c->pf[F_DUDT0];
+  e.tlast = c->pf[F_TLAST]; e.pdt = c->pf[F_PDT]; e.dtnext = c->pf[F_DTNEXT];
+  e.nlast = c->pi[I_NLAST]; e.act = c->pi[I_ACT]; e.end = c->pi[I_END];
+  e.finished = c->icount + 6;
+  k_block_end_step<<<nblocks(c->N, BS), BS, 0, c->stream>>>(e);
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
 }

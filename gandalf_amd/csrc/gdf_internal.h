@@ -20,10 +20,16 @@ constexpr int kWalkSlots = 4096;
 constexpr int kStackCap = 8192;
 
 // Per-particle persistent fields, permuted with the particles at every tree build.
+// F_TLAST, F_PDT, F_DTNEXT: start time, length and next length of the particle's block step
+// (Particle.h:166-167, dt).
 enum PField {
   F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_X0, F_Y0, F_Z0, F_VX0, F_VY0, F_VZ0,
-  F_AX0, F_AY0, F_AZ0, F_M, F_H, F_U, F_U0, F_DUDT0, F_NPERSIST
+  F_AX0, F_AY0, F_AZ0, F_M, F_H, F_U, F_U0, F_DUDT0, F_TLAST, F_PDT, F_DTNEXT, F_NPERSIST
 };
+// Per-particle integer state, permuted with the particles: block-timestep level, neighbour
+// level, integer step length and start (Particle.h:140-143), and the active / end-of-step
+// flags (Flags.h active, end_timestep) as 0/1.
+enum IField { I_LEVEL, I_LEVELNEIB, I_NSTEP, I_NLAST, I_ACT, I_END, I_NINT };
 // Derived per-particle fields, recomputed every step after the permutation.
 enum DField {
   D_AX, D_AY, D_AZ, D_ATX, D_ATY, D_ATZ, D_RHO, D_DUDT, D_DIVV, D_GPOT, D_INVOMEGA,
@@ -90,6 +96,10 @@ struct gdf_ctx {
   double* pf[gdf::F_NPERSIST] = {};      // persistent fields (sorted order)
   double* pf_alt[gdf::F_NPERSIST] = {};  // permutation targets
   double* df[gdf::D_NDERIVED] = {};      // derived fields
+  double* df_alt[gdf::D_NDERIVED] = {};  // permutation targets (block timesteps only: inactive
+                                         // particles keep their derived state across steps)
+  int* pi[gdf::I_NINT] = {};             // integer state (sorted order)
+  int* pi_alt[gdf::I_NINT] = {};
   int* iorig = nullptr; int* iorig_alt = nullptr;
   double4* posm = nullptr;               // packed (x, y, z, m) for neighbour gathers
   double4* hrecA = nullptr;              // hydro records: (vx, vy, vz, 1/h)
@@ -134,6 +144,11 @@ struct gdf_ctx {
   double t = 0.0, timestep = 0.0, tlast = 0.0;
   int64_t nsteps = 0;
   bool setup_done = false;
+  // block timesteps (Simulation.h:187-215): integer time n, nresync, level_max, level_step
+  int nlevels = 1, level_diff_max = 1;
+  int n_int = 0, nresync = 0, level_max = 0, level_step = 0;
+  double dt_max = 0.0;
+  long long finished_last = 0;   // particles that ended a step in the last MainLoop
 
   // timing
   bool timing = false;
@@ -169,6 +184,14 @@ int gravity_walk(gdf_ctx* c);
 
 // integration (gdf_integrate.hip)
 int integ_advance(gdf_ctx* c, double dt);
+// block timesteps (gdf_integrate.hip)
+int block_init(gdf_ctx* c);
+int block_advance(gdf_ctx* c);
+int block_zero_active(gdf_ctx* c);
+int block_eos_all(gdf_ctx* c);
+int block_check_timesteps(gdf_ctx* c, int* activecount);
+int block_timesteps(gdf_ctx* c);
+int block_end_step(gdf_ctx* c);
 int integ_wrap(gdf_ctx* c);
 int integ_end_step(gdf_ctx* c, double dtstep);
 int integ_timestep(gdf_ctx* c, double* dtmin);

@@ -28,12 +28,6 @@ constexpr int CANDCAP = 1024;              // ring buffer of candidate slots (po
 constexpr double NODE_SLACK = 1.0 + 1e-12; // node tests err on the inclusive side
 constexpr int kInitBatches = 16;           // initial candidate capacity per group, in batches
 
-__device__ __forceinline__ void wave_add_ll(long long* dst, long long v) {
-  #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  if (__lane_id() == 0 && v != 0) atomicAdd((unsigned long long*) dst, (unsigned long long) v);
-}
-
 __device__ __forceinline__ void node_range(int n, int N, const int* first, const int* last,
                                            int& f, int& cnt) {
   if (n >= N - 1) { f = n - (N - 1); cnt = 1; }
@@ -231,13 +225,9 @@ __device__ void hrho_finish(const DensArgs& a, const PhysParams& p, int i, doubl
   iom = 1.0/iom;
   const double zt = deriv*zeta*iom;
   a.h[i] = h; a.rho[i] = rho; a.invomega[i] = iom; a.zeta[i] = zt; a.hfactor[i] = hfac4;
-  double u = a.u[i], snd;
-  if (p.eos == 1) { u = p.temp0/p.gammam1/p.mu_bar; snd = sqrt(p.gammam1*u); }
-  else if (p.eos == 2) {
-    u = p.temp0*(1.0 + pow(rho*p.invrho_bary, p.gammam1))/p.gammam1/p.mu_bar;
-    snd = sqrt(p.gammam1*u);
-  } else snd = sqrt(p.gamma*p.gammam1*u);
-  a.u[i] = u; a.sound[i] = snd; a.pressure[i] = p.gammam1*rho*u;
+  double u = a.u[i], snd, press;
+  eos_state(p, rho, u, snd, press);
+  a.u[i] = u; a.sound[i] = snd; a.pressure[i] = press;
 }
 
 // h-rho pass over one walk group per workgroup.  T = double: FP64 sums in candidate order;
@@ -359,12 +349,13 @@ __global__ __launch_bounds__(64) void k_density(DensArgs a) {
   wave_add_ll(&a.icount[4], evals);
 }
 
+// act (block timesteps): only the active particles iterate; NULL = all
 __global__ void k_init_density(int N, const double* __restrict__ h, double* __restrict__ hs,
-                               int* __restrict__ status) {
+                               int* __restrict__ status, const int* __restrict__ act) {
   const int i = blockIdx.x*BS + threadIdx.x;
   if (i >= N) return;
   hs[i] = 1.1*h[i];
-  status[i] = 1;
+  status[i] = act ? act[i] : 1;
 }
 
 __global__ void k_zero_ints(int n, int* __restrict__ p) {
@@ -395,14 +386,25 @@ struct HydroArgs {
   double* ax; double* ay; double* az; double* dudt; double* div_v;
   double* gx; double* gy; double* gz; double* gpot;    // softened neighbour gravity (VAR 1)
   long long* icount;
+  // block timesteps (BLK): active flags, levels and neighbour levels
+  const int* act; const int* level; int* levelneib;
   PhysParams p;
 };
+
+// Neighbour levels of a block step (GradhSph.cpp:445-446, merged as GradhSphTree.cpp:375-417):
+// an active i takes the largest level of its scatter-gather set (itself included), and every
+// inactive j of that set takes max(levelneib_j, level_i).  The set relation is symmetric, so an
+// active j gets level_i from its own pass; records carry level_j and the bound below which an
+// update of j is needed (255 for an active j: never).
+__device__ __forceinline__ int blk_record(const HydroArgs& a, int j) {
+  return a.level[j] | ((a.act[j] ? 255 : a.levelneib[j]) << 8);
+}
 
 // VAR 0: ComputeSphHydroForces (GradhSph.cpp:361-460); VAR 1: ComputeSphHydroGravForces
 // (:474-585), the hydro terms plus the kernel-softened gravity of the pair (into gx, gy, gz,
 // gpot; the tree walk adds the direct and multipole terms afterwards).  FP64, one walk group
 // per workgroup, neighbour records staged in LDS per batch of 64 candidates.
-template <int VAR>
+template <int VAR, bool BLK>
 __global__ __launch_bounds__(64) void k_hydro(HydroArgs a) {
   __shared__ double4 sP[kWave], sA[kWave], sB[kWave];
   __shared__ double sZ[kWave];
@@ -411,7 +413,10 @@ __global__ __launch_bounds__(64) void k_hydro(HydroArgs a) {
   const int nc = a.ncand[g];
   const int gfirst = a.grp_first[g], gcnt = a.grp_count[g];
   const int i = gfirst + lane;
-  const bool has = lane < gcnt;
+  const bool has = lane < gcnt && (!BLK || a.act[gfirst + lane]);
+  __shared__ int sJ[kWave], sL[kWave];
+  int lmax = 0;
+  const int level_i = (BLK && has) ? a.level[i] : 0;
   const PhysParams& p = a.p;
   double4 me = make_double4(0.0, 0.0, 0.0, 1.0), mA = make_double4(0.0, 0.0, 0.0, 1.0);
   double4 mB = make_double4(1.0, 0.0, 0.0, 0.0);
@@ -436,6 +441,7 @@ __global__ __launch_bounds__(64) void k_hydro(HydroArgs a) {
       const int j = idx[b*kWave + lane];
       sP[lane] = a.posm[j]; sA[lane] = a.A[j]; sB[lane] = a.B[j];
       if (VAR == 1) sZ[lane] = a.zeta[j];
+      if (BLK) { sJ[lane] = j; sL[lane] = blk_record(a, j); }
     }
     __syncthreads();
     if (has) {
@@ -444,6 +450,11 @@ __global__ __launch_bounds__(64) void k_hydro(HydroArgs a) {
       while (mk) {
         const int q = __builtin_ctzll(mk);
         mk &= mk - 1;
+        if (BLK) {
+          const int lj = sL[q];
+          lmax = max(lmax, lj & 255);
+          if (level_i > (lj >> 8)) atomicMax(&a.levelneib[sJ[q]], level_i);
+        }
         const double4 pj = sP[q], Aj = sA[q], Bj = sB[q];
         double dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
         nearest_image<double>(p, dx, dy, dz);
@@ -490,6 +501,7 @@ __global__ __launch_bounds__(64) void k_hydro(HydroArgs a) {
   }
   wave_add_ll(&a.icount[1], npairs);
   if (!has) return;
+  if (BLK) a.levelneib[i] = lmax;
   const double dvd = dv*mB.x;
   const double dud = du - a.pressure[i]*dvd*mB.x*a.invomega[i];
   a.ax[i] = ax; a.ay[i] = ay; a.az[i] = az;
@@ -500,7 +512,7 @@ __global__ __launch_bounds__(64) void k_hydro(HydroArgs a) {
 // FP32 variant: records staged in LDS in the group frame (positions relative to the group
 // centre, velocities relative to the group's first particle), pair terms in FP32 with the
 // hardware reciprocal and square root, per-batch register sums, Kahan-compensated totals.
-template <int VAR>
+template <int VAR, bool BLK>
 __global__ __launch_bounds__(64) void k_hydro_mixed(HydroArgs a) {
   __shared__ float4 fP[kWave], fV[kWave], fB[kWave];
   __shared__ float fZ[kWave];
@@ -509,7 +521,10 @@ __global__ __launch_bounds__(64) void k_hydro_mixed(HydroArgs a) {
   const int nc = a.ncand[g];
   const int gfirst = a.grp_first[g], gcnt = a.grp_count[g];
   const int i = gfirst + lane;
-  const bool has = lane < gcnt;
+  const bool has = lane < gcnt && (!BLK || a.act[gfirst + lane]);
+  __shared__ int sJ[kWave], sL[kWave];
+  int lmax = 0;
+  const int level_i = (BLK && has) ? a.level[i] : 0;
   const PhysParams& p = a.p;
   const double4 gc = a.geo_c[a.grp_node[g]];
   const double4 vref = a.A[gfirst];
@@ -544,6 +559,7 @@ __global__ __launch_bounds__(64) void k_hydro_mixed(HydroArgs a) {
                              (float) (Aj.z - vref.z), (float) Aj.w);
       fB[lane] = make_float4((float) Bj.x, (float) Bj.y, (float) Bj.z, (float) Bj.w);
       if (VAR == 1) fZ[lane] = (float) a.zeta[j];
+      if (BLK) { sJ[lane] = j; sL[lane] = blk_record(a, j); }
     }
     __syncthreads();
     if (has) {
@@ -554,6 +570,11 @@ __global__ __launch_bounds__(64) void k_hydro_mixed(HydroArgs a) {
       while (mk) {
         const int q = __builtin_ctzll(mk);
         mk &= mk - 1;
+        if (BLK) {
+          const int lj = sL[q];
+          lmax = max(lmax, lj & 255);
+          if (level_i > (lj >> 8)) atomicMax(&a.levelneib[sJ[q]], level_i);
+        }
         const float4 P = fP[q], V = fV[q], B = fB[q];
         float dx = P.x - rxi, dy = P.y - ryi, dz = P.z - rzi;
         const float mj = P.w;
@@ -596,6 +617,7 @@ __global__ __launch_bounds__(64) void k_hydro_mixed(HydroArgs a) {
   }
   wave_add_ll(&a.icount[1], npairs);
   if (!has) return;
+  if (BLK) a.levelneib[i] = lmax;
   const double dvd = (double) dv.s*mB.x;
   const double dud = (double) du.s - a.pressure[i]*dvd*mB.x*a.invomega[i];
   a.ax[i] = (double) ax.s; a.ay[i] = (double) ay.s; a.az[i] = (double) az.s;
@@ -699,7 +721,8 @@ int sph_density(gdf_ctx* c, int nopbc) {
   hipStream_t s = c->stream;
   if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "density pass before tree build");
   GDF_TRY(ensure_cand(c, c->cs_g));
-  k_init_density<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_H], c->df[D_HSEARCH], c->status);
+  k_init_density<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_H], c->df[D_HSEARCH], c->status,
+                                               c->nlevels > 1 ? c->pi[I_ACT] : nullptr);
   DensArgs d;
   d.N = N; d.ngroups = c->ngroups; d.nopbc = nopbc;
   d.grp_node = c->grp_node; d.grp_first = c->grp_first; d.grp_count = c->grp_count;
@@ -735,7 +758,7 @@ int sph_build_sg(gdf_ctx* c) {
   GDF_TRY(ensure_cand(c, c->cs_s));
   for (int round = 0; round < 8; round++) {
     k_zero_ints<<<1, BS, 0, c->stream>>>(4, c->counters);
-    GDF_TRY(launch_walk(c, 1, 0, c->cs_s, nullptr));
+    GDF_TRY(launch_walk(c, 1, 0, c->cs_s, c->nlevels > 1 ? c->pi[I_ACT] : nullptr));
     GDF_TRY(read_flags(c));
     if (c->h_counters[2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in force pass");
     if (c->h_counters[1] > 0) { GDF_TRY(grow_cand(c, c->cs_s, c->h_counters[1])); continue; }
@@ -764,14 +787,26 @@ int sph_hydro(gdf_ctx* c, int variant) {
   h.gx = c->df[D_ATX]; h.gy = c->df[D_ATY]; h.gz = c->df[D_ATZ]; h.gpot = c->df[D_GPOT];
   h.dudt = c->df[D_DUDT]; h.div_v = c->df[D_DIVV]; h.p = c->phys;
   h.icount = c->icount;
+  h.act = c->pi[I_ACT]; h.level = c->pi[I_LEVEL]; h.levelneib = c->pi[I_LEVELNEIB];
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
+  const bool blk = c->nlevels > 1;
   const int G = c->ngroups;
   if (variant == 0) {
-    if (mixed) k_hydro_mixed<0><<<G, kWave, 0, s>>>(h);
-    else k_hydro<0><<<G, kWave, 0, s>>>(h);
+    if (blk) {
+      if (mixed) k_hydro_mixed<0, true><<<G, kWave, 0, s>>>(h);
+      else k_hydro<0, true><<<G, kWave, 0, s>>>(h);
+    } else {
+      if (mixed) k_hydro_mixed<0, false><<<G, kWave, 0, s>>>(h);
+      else k_hydro<0, false><<<G, kWave, 0, s>>>(h);
+    }
   } else {
-    if (mixed) k_hydro_mixed<1><<<G, kWave, 0, s>>>(h);
-    else k_hydro<1><<<G, kWave, 0, s>>>(h);
+    if (blk) {
+      if (mixed) k_hydro_mixed<1, true><<<G, kWave, 0, s>>>(h);
+      else k_hydro<1, true><<<G, kWave, 0, s>>>(h);
+    } else {
+      if (mixed) k_hydro_mixed<1, false><<<G, kWave, 0, s>>>(h);
+      else k_hydro<1, false><<<G, kWave, 0, s>>>(h);
+    }
   }
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;

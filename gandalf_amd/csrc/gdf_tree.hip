@@ -98,7 +98,10 @@ __global__ void k_morton(int N, const double* __restrict__ x, const double* __re
   vals[i] = i;
 }
 
-struct PermArgs { const double* src[F_NPERSIST]; double* dst[F_NPERSIST]; };
+struct PermArgs {
+  const double* src[F_NPERSIST]; double* dst[F_NPERSIST];
+  const int* isrc[I_NINT]; int* idst[I_NINT];
+};
 
 __global__ void k_permute(int N, const int* __restrict__ vals, PermArgs a,
                           const int* __restrict__ io, int* __restrict__ io_alt) {
@@ -107,7 +110,20 @@ __global__ void k_permute(int N, const int* __restrict__ vals, PermArgs a,
   const int o = vals[s];
   #pragma unroll
   for (int f = 0; f < F_NPERSIST; f++) a.dst[f][s] = a.src[f][o];
+  #pragma unroll
+  for (int f = 0; f < I_NINT; f++) a.idst[f][s] = a.isrc[f][o];
   io_alt[s] = io[o];
+}
+
+// block timesteps: the derived fields persist for the particles that are not recomputed
+struct PermDArgs { const double* src[D_NDERIVED]; double* dst[D_NDERIVED]; };
+
+__global__ void k_permute_d(int N, const int* __restrict__ vals, PermDArgs a) {
+  const int s = blockIdx.x*BS + threadIdx.x;
+  if (s >= N) return;
+  const int o = vals[s];
+  #pragma unroll
+  for (int f = 0; f < D_NDERIVED; f++) a.dst[f][s] = a.src[f][o];
 }
 
 __device__ __forceinline__ int delta(const uint64_t* __restrict__ key, int N, int i, long long j) {
@@ -468,9 +484,17 @@ int tree_build(gdf_ctx* c) {
   // 3. permute persistent particle state into key order
   PermArgs pa;
   for (int f = 0; f < F_NPERSIST; f++) { pa.src[f] = c->pf[f]; pa.dst[f] = c->pf_alt[f]; }
+  for (int f = 0; f < I_NINT; f++) { pa.isrc[f] = c->pi[f]; pa.idst[f] = c->pi_alt[f]; }
   k_permute<<<nblocks(N, BS), BS, 0, s>>>(N, c->vals, pa, c->iorig, c->iorig_alt);
   for (int f = 0; f < F_NPERSIST; f++) std::swap(c->pf[f], c->pf_alt[f]);
+  for (int f = 0; f < I_NINT; f++) std::swap(c->pi[f], c->pi_alt[f]);
   std::swap(c->iorig, c->iorig_alt);
+  if (c->df_alt[0]) {
+    PermDArgs pd;
+    for (int f = 0; f < D_NDERIVED; f++) { pd.src[f] = c->df[f]; pd.dst[f] = c->df_alt[f]; }
+    k_permute_d<<<nblocks(N, BS), BS, 0, s>>>(N, c->vals, pd);
+    for (int f = 0; f < D_NDERIVED; f++) std::swap(c->df[f], c->df_alt[f]);
+  }
   // 4. topology
   if (N > 1)
     k_lbvh<<<nblocks(N - 1, BS), BS, 0, s>>>(N, c->keys, c->left, c->right, c->first, c->last,

@@ -40,16 +40,25 @@ int GradhSphTreeGpu<ndim>::UpdateAllSphProperties(bool nopbc) {
   return gdf::sph_density(ctx, nopbc ? 1 : 0);
 }
 
+// Sph::ZeroAccelerations (Sph.cpp:126-140): every particle with global timesteps, the active
+// ones (and their neighbour levels) with block timesteps; the thermal state of every particle
+// is refreshed as SphSimulation::MainLoop does (SphSimulation.cpp:667-676, nradstep = 1)
+static int zero_accelerations(gdf_ctx* c) {
+  if (c->nlevels == 1) return gdf::integ_zero_forces(c);
+  GDF_TRY(gdf::block_zero_active(c));
+  return gdf::block_eos_all(c);
+}
+
 template <int ndim>
 int GradhSphTreeGpu<ndim>::UpdateAllSphHydroForces() {
-  GDF_TRY(gdf::integ_zero_forces(ctx));
+  GDF_TRY(zero_accelerations(ctx));
   GDF_TRY(gdf::sph_build_sg(ctx));
   return gdf::sph_hydro(ctx, 0);
 }
 
 template <int ndim>
 int GradhSphTreeGpu<ndim>::UpdateAllSphForces() {
-  GDF_TRY(gdf::integ_zero_forces(ctx));
+  GDF_TRY(zero_accelerations(ctx));
   if (ctx->phys.hydro_forces) {
     GDF_TRY(gdf::sph_build_sg(ctx));
     GDF_TRY(gdf::sph_hydro(ctx, 1));
@@ -107,6 +116,9 @@ int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
   const double hg = initial_h_guess(r, c->phys.kern.range, c->phys.h_fac);
   std::vector<double> h((size_t) c->N, hg);
   GDF_TRY(gdf_set_field(c, "h", h.data()));
+  c->t = 0.0;
+  // every particle active, level 0, tlast = t (SphSimulation.cpp:225-330)
+  if (c->nlevels > 1) GDF_TRY(gdf::block_init(c));
   GDF_TRY(sphneib.BuildTree(true, 0, 1, 1, 0.0));
   GDF_TRY(sphneib.UpdateAllSphProperties(c->phys.periodic != 0));
   GDF_TRY(sphneib.BuildTree(true, 0, 1, 1, 0.0));
@@ -116,9 +128,16 @@ int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
   GDF_TRY(ComputeForces());
   c->t = 0.0;
   c->tlast = 0.0;
-  GDF_TRY(ComputeGlobalTimestep(&c->timestep));
-  // EndTimestep with the particles' initial dt = 0 sets r0, v0, a0, u0, dudt0
-  GDF_TRY(hydroint.EndTimestep(0, c->t, 0.0));
+  if (c->nlevels > 1) {
+    // the first ComputeBlockTimesteps resynchronises (n = nresync = 0); EndTimestep with the
+    // particles' initial dt = 0 sets r0, v0, a0, u0, dudt0 and each particle's first step
+    GDF_TRY(gdf::block_timesteps(c));
+    GDF_TRY(gdf::block_end_step(c));
+  } else {
+    GDF_TRY(ComputeGlobalTimestep(&c->timestep));
+    // EndTimestep with the particles' initial dt = 0 sets r0, v0, a0, u0, dudt0
+    GDF_TRY(hydroint.EndTimestep(0, c->t, 0.0));
+  }
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   c->nsteps = 0;
   c->setup_done = true;
@@ -129,6 +148,7 @@ template <int ndim>
 int GradhSphSimulationGpu<ndim>::MainLoop() {
   gdf_ctx* c = ctx;
   if (!c->setup_done) return gdf::fail(c, GDF_ERR_INVALID, "MainLoop before setup");
+  if (c->nlevels > 1) return MainLoopBlock();
   GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
   record(c, 0);
   c->nsteps++;
@@ -153,6 +173,14 @@ int GradhSphSimulationGpu<ndim>::MainLoop() {
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   const long long* ic = reinterpret_cast<const long long*>(c->h_red + 16000);
   for (int k = 0; k < 6; k++) c->icounts_last[k] = ic[k];
+  c->finished_last = c->N;
+  FinishStepStats();
+  return GDF_OK;
+}
+
+template <int ndim>
+void GradhSphSimulationGpu<ndim>::FinishStepStats() {
+  gdf_ctx* c = ctx;
   if (c->timing) {
     float ms = 0.f;
     // tree, density, hydro, gravity, integrate
@@ -165,6 +193,43 @@ int GradhSphSimulationGpu<ndim>::MainLoop() {
     hipEventElapsedTime(&b, c->ev[4], c->ev[6]);
     c->pass_ms[4] = a + b;
   }
+}
+
+template <int ndim>
+int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
+  gdf_ctx* c = ctx;
+  GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
+  record(c, 0);
+  c->n_int += 1;
+  c->nsteps++;
+  c->t = c->t + c->timestep;
+  GDF_TRY(gdf::block_advance(c));
+  GDF_TRY(hydroint.CheckBoundaries());
+  record(c, 1);
+  GDF_TRY(sphneib.BuildTree(true, (int) c->nsteps, 1, 1, c->timestep));
+  record(c, 2);
+  int activecount = 0, rounds = 0;
+  do {
+    GDF_TRY(sphneib.UpdateAllSphProperties(false));
+    if (rounds == 0) record(c, 5);
+    GDF_TRY(ComputeForces());
+    if (rounds == 0) record(c, 4);
+    // SphLeapfrogKDK::CheckTimesteps: the active flags are rewritten, set only for the
+    // particles whose step a neighbour's level ends early
+    GDF_TRY(gdf::block_check_timesteps(c, &activecount));
+    if (++rounds > 64) return gdf::fail(c, GDF_ERR_TIMESTEP, "CheckTimesteps did not settle");
+  } while (activecount > 0);
+  GDF_TRY(gdf::block_timesteps(c));
+  GDF_TRY(gdf::block_end_step(c));
+  c->tlast = c->t;
+  record(c, 6);
+  GDF_HIP(c, hipMemcpyAsync(c->h_red + 16000, c->icount, sizeof(long long)*7,
+                            hipMemcpyDeviceToHost, c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  const long long* ic = reinterpret_cast<const long long*>(c->h_red + 16000);
+  for (int k = 0; k < 6; k++) c->icounts_last[k] = ic[k];
+  c->finished_last = ic[6];
+  FinishStepStats();
   return GDF_OK;
 }
 

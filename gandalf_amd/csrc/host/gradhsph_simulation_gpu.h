@@ -56,13 +56,19 @@ class GradhSphSimulationGpu {
   explicit GradhSphSimulationGpu(gdf_ctx* c) : ctx(c), sphneib(c), hydroint(c) {}
   // SphSimulation::PostInitialConditionsSetup (SphSimulation.cpp:204-565), file ICs
   int PostInitialConditionsSetup();
-  // SphSimulation::MainLoop (SphSimulation.cpp:574-880) with Nlevels = 1
+  // SphSimulation::MainLoop (SphSimulation.cpp:574-880): global timesteps for Nlevels = 1,
+  // block timesteps otherwise
   int MainLoop();
   // Simulation::ComputeGlobalTimestep (Simulation.cpp:1669-1754)
   int ComputeGlobalTimestep(double* dt);
 
  private:
   int ComputeForces();
+  // MainLoop with block timesteps: per-particle drift, forces of the active particles, the
+  // CheckTimesteps loop, Simulation::ComputeBlockTimesteps (Simulation.cpp:1764-2197) and
+  // EndTimestep of the particles whose step ended
+  int MainLoopBlock();
+  void FinishStepStats();
   gdf_ctx* ctx;
   GradhSphTreeGpu<ndim> sphneib;
   SphLeapfrogKDKGpu<ndim> hydroint;

@@ -37,6 +37,7 @@ class Config(C.Structure):
         ("precision", C.c_int), ("group_max", C.c_int),
         ("nbody", C.c_int), ("nbody_direct", C.c_int), ("nbody_eps", C.c_double),
         ("nbody_mult", C.c_double), ("fixed_dt", C.c_double),
+        ("Nlevels", C.c_int), ("level_diff_max", C.c_int),
     ]
 
 
@@ -66,6 +67,7 @@ PROTOTYPES = {
     "gdf_setup_simulation": (_i, [_vp]),
     "gdf_main_loop": (_i, [_vp, _i]),
     "gdf_time": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)]),
+    "gdf_block_state": (_i, [_vp, _vp, _vp]),
     "gdf_neighbour_sets": (_i64, [_vp, _i, _vp, _vp, _i64]),
     "gdf_tree_arrays": (_i, [_vp] + [_vp] * 15 + [C.POINTER(_i)]),
     "gdf_interaction_counts": (_i, [_vp, _vp]),
@@ -127,6 +129,8 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
         c.nbody_direct = 2 if int(prm.get("nbody_softening", 0)) else 1
     c.nbody_mult = float(prm.get("nbody_mult", 0.1))
     c.fixed_dt = float(prm.get("fixed_dt", 0.0))
+    c.Nlevels = int(prm.get("Nlevels", 1))
+    c.level_diff_max = int(prm.get("level_diff_max", 1))
     return c
 
 
@@ -223,6 +227,13 @@ class GpuSim:
         t, dt, n = C.c_double(), C.c_double(), C.c_int64()
         self._chk(lib().gdf_time(self._h, C.byref(t), C.byref(dt), C.byref(n)))
         return t.value, dt.value, n.value
+
+    def block_state(self):
+        """(n, level_max, level_step, nresync, dt_max, particles that ended a step last)"""
+        iv = np.zeros(5, dtype=np.int64)
+        dv = np.zeros(1)
+        self._chk(lib().gdf_block_state(self._h, _p(iv), _p(dv)))
+        return int(iv[0]), int(iv[1]), int(iv[2]), int(iv[3]), float(dv[0]), int(iv[4])
 
     def set_timing(self, on=True):
         self._chk(lib().gdf_set_timing(self._h, int(on)))

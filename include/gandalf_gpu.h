@@ -56,6 +56,10 @@ typedef struct gdf_config {
   double nbody_eps;
   double nbody_mult;          /* "nbody_mult" timestep multiplier */
   double fixed_dt;            /* > 0: every step uses this timestep */
+  /* block timesteps (Simulation.cpp:1764-2197): "Nlevels" (1 = global timestep) and
+   * "level_diff_max" (max. level difference to neighbours) */
+  int Nlevels;
+  int level_diff_max;
 } gdf_config;
 
 typedef struct gdf_ctx gdf_ctx;
@@ -75,7 +79,9 @@ int gdf_set_particles(gdf_ctx* ctx, int N, const double* r, const double* v, con
                       const double* h, const double* u);
 /* read a per-particle field back in import order; replaces the host particle array reads of
  * SphSnapshot::CopyDataFromSimulation (src/Common/SphSnapshot.cpp:340-372).  Names:
- * r v a atree r0 v0 a0 m h rho u u0 dudt dudt0 div_v gpot invomega zeta sound pressure hfactor */
+ * r v a atree r0 v0 a0 m h rho u u0 dudt dudt0 div_v gpot invomega zeta sound pressure hfactor
+ * dt (timestep criterion) adot; block-timestep state: tlast dtstep (the particle's step
+ * length, Particle.h dt) level levelneib nstep nlast (integers returned as doubles) */
 int gdf_get_field(gdf_ctx* ctx, const char* name, double* out);
 int gdf_set_field(gdf_ctx* ctx, const char* name, const double* in);
 int gdf_num_particles(const gdf_ctx* ctx);
@@ -112,8 +118,12 @@ int gdf_calculate_diagnostics(gdf_ctx* ctx, double* out);
 /* ---- whole-step drivers: SphSimulation<3> (src/Hydrodynamics/SphSimulation.cpp) ---- */
 /* PostInitialConditionsSetup (:204-565) for initial conditions without smoothing lengths */
 int gdf_setup_simulation(gdf_ctx* ctx);
-/* MainLoop (:574-880) with global timesteps (Nlevels = 1); nsteps steps */
+/* MainLoop (:574-880), nsteps steps; global timesteps for Nlevels = 1, else block timesteps
+ * (Simulation::ComputeBlockTimesteps, src/Common/Simulation.cpp:1764-2197) */
 int gdf_main_loop(gdf_ctx* ctx, int nsteps);
+/* block-timestep state (Simulation.h:187-215): iout = {n, level_max, level_step, nresync,
+ * particles that ended a step in the last MainLoop}; dout = {dt_max} */
+int gdf_block_state(const gdf_ctx* ctx, int64_t* iout5, double* dout1);
 /* simulation time, current timestep and step count */
 int gdf_time(const gdf_ctx* ctx, double* t, double* timestep, int64_t* nsteps);
 
