@@ -3,12 +3,16 @@
 Workload: the rotating m=2-perturbed cloud of 1,048,576 SPH particles (uniform radius-1 sphere,
 rho = rho0 (1 + 0.1 cos 2 phi), solid-body rotation E_rot/|W| = 0.16, thermal ratio 0.26,
 barotropic EOS, quintic kernel, self-gravity with quadrupole cells, opening angle 0.5), seeded
-synthetic initial conditions, global timesteps.  A "step" is one full MainLoop: kick-drift,
-tree build, h-rho iteration, hydro forces, gravity walk, timestep reduction, end-of-step kick.
+synthetic initial conditions, block timesteps with up to 8 levels.  A "step" is one full
+MainLoop at the base timestep: drift, tree build, h-rho iteration and forces of the active
+particles (repeated for particles whose step CheckTimesteps ends early), block-timestep
+update, end-of-step kick.  Particle-steps count the particles that complete a step (the
+active ones), so steps where few particles are active count little.
 
 Both precision paths are timed: the FP32 mixed path (the headline value) and the plain FP64
-path.  Inputs are resident on the device before the timed region; the timed region brackets
-exactly --steps steps with device synchronisation on both sides.
+path; the same workload with global timesteps is timed beside them for reference.  Inputs are
+resident on the device before the timed region; the timed region brackets exactly --steps
+steps with device synchronisation on both sides.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
 """
@@ -30,6 +34,7 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}   # MI355X dense vector peaks (FLOP/s
 # only), hydro pair 75, hydro pair with the kernel-softened gravity of neighbours 135,
 # direct/monopole gravity term 22, quadrupole particle-cell term 70
 W_DENS, W_HYDRO, W_HYDRO_GRAV, W_PP, W_PC = 45, 75, 135, 22, 70
+NLEVELS = 8          # block timestep levels of the workload
 
 
 def log(msg):
@@ -50,6 +55,8 @@ def run_path(ic, precision, steps, warmup):
                                 "k_gravity")}
     counts = None
     per_step = []
+    finished = []
+    launches = 0
     t_start = time.perf_counter()
     for k in range(steps):
         ts = time.perf_counter()
@@ -58,6 +65,8 @@ def run_path(ic, precision, steps, warmup):
         pt = s.pass_times()
         for key in passes:
             passes[key] += pt[key]
+        launches += int(pt["k_gravity_launches"])
+        finished.append(s.block_state()[5])
         c = s.interaction_counts()
         counts = c if counts is None else {k2: counts[k2] + c[k2] for k2 in c}
     wall = time.perf_counter() - t_start
@@ -66,7 +75,8 @@ def run_path(ic, precision, steps, warmup):
     s.close()
     N = ic["r"].shape[0]
     return dict(N=N, wall=wall, steps=steps, per_step=per_step, passes=passes, counts=counts,
-                t=t, dt=dt, E=float(diag[0]), precision=precision)
+                t=t, dt=dt, E=float(diag[0]), precision=precision, finished=finished,
+                launches=launches, particle_steps=int(sum(finished)))
 
 
 def roofline(res, dtype):
@@ -77,6 +87,7 @@ def roofline(res, dtype):
     fraction (density 45, hydro with neighbour gravity 135, plus the walk) over the step-loop
     wall time is reported beside it."""
     c, p, steps = res["counts"], res["passes"], res["steps"]
+    nl = max(res["launches"], 1)
     direct = c["grav_pp"] - c["sg_pairs"]
     flops = W_PP * direct + W_PC * c["grav_pc"]
     ms = p["k_gravity"]
@@ -87,8 +98,8 @@ def roofline(res, dtype):
     return {"kernel": "k_gravity", "bound": "valu", "achieved": achieved / 1e12,
             "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": measured_traffic() if dtype == "fp32" else None,
-            "flop_per_launch": flops / steps,
-            "launch_ms": ms / steps,
+            "flop_per_launch": flops / nl,
+            "launch_ms": ms / nl, "launches": nl,
             "loop": {"achieved": loop_rate / 1e12, "frac": loop_rate / peak,
                      "flop_per_step": loop_flops / steps}}
 
@@ -117,35 +128,40 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}), flush=True)
 
 
-def cpu_baseline(nsample=131072, steps=3):
+def cpu_baseline(nsample=131072, steps=4):
     """The CPU oracle (OpenMP restatement of the reference path) on a bounded sample of the
-    same workload: the same generator at nsample particles, one step after setup."""
+    same workload: the same generator and block timesteps at nsample particles, `steps` base
+    steps after setup, counting the particles that complete a step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from gandalf_amd import ic as ICS
-    g = ICS.rotating_m2_cloud(N=nsample, seed=3, rho_bary=10.0)
+    g = ICS.rotating_m2_cloud(N=nsample, seed=3, rho_bary=10.0, nlevels=NLEVELS)
     p = O.params_from_dict(g["params"])
     s = O.Sim(p, g["r"], g["v"], g["m"], g["u"])
     s.setup()
     t0 = time.perf_counter()
+    done = 0
     for _ in range(steps):
         s.step()
+        done += s.block_state()[5]
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": nsample * steps / dt, "unit": "particle-steps/s", "cores": cores,
+    return {"value": done / dt, "unit": "particle-steps/s", "cores": cores,
             "kind": "port",
-            "sample": f"rotating m=2 cloud at N={nsample}, {steps} step(s) after setup, "
+            "sample": f"rotating m=2 cloud at N={nsample}, block timesteps ({NLEVELS} levels), "
+                      f"{steps} base steps after setup ({done} particle-steps), "
                       f"oracle/gdf_oracle.c OpenMP"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-global", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -158,15 +174,21 @@ def main():
         sys.exit(2)
     from gandalf_amd import ic as ICS
     log(f"generating rotating m=2 cloud N={args.n}")
-    g = ICS.rotating_m2_cloud(N=args.n, seed=3, rho_bary=10.0)
+    g = ICS.rotating_m2_cloud(N=args.n, seed=3, rho_bary=10.0, nlevels=NLEVELS)
     mixed = run_path(g, "mixed", args.steps, args.warmup)
-    log(f"[mixed] {mixed['wall']:.3f} s for {args.steps} steps; passes(ms) {mixed['passes']}")
+    log(f"[mixed] {mixed['wall']:.3f} s for {args.steps} steps, particle-steps per step "
+        f"{mixed['finished']}; passes(ms) {mixed['passes']}")
     fp64 = None
     if not args.no_fp64:
         fp64 = run_path(g, "fp64", args.steps, args.warmup)
         log(f"[fp64] {fp64['wall']:.3f} s for {args.steps} steps; passes(ms) {fp64['passes']}")
+    glob = None
+    if not args.no_global:
+        gg = ICS.rotating_m2_cloud(N=args.n, seed=3, rho_bary=10.0, nlevels=1)
+        glob = run_path(gg, "mixed", 3, 1)
+        log(f"[mixed, global timesteps] {glob['wall']:.3f} s for 3 steps")
     N = args.n
-    value = N * args.steps / mixed["wall"]
+    value = mixed["particle_steps"] / mixed["wall"]
     out = {
         "metric": "SPH+grav particle-steps/s at 1M (FP32 vs FP64); ensemble p50/p99 step latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -174,17 +196,23 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "rotating m=2 perturbed cloud, quintic kernel, barotropic EOS, "
-                               "self-gravity theta=0.5 quadrupole, global timesteps",
-                   "n_particles": N, "precision": "fp32 pair arithmetic + fp64 integration"},
+                               "self-gravity theta=0.5 quadrupole, block timesteps up to "
+                               f"{NLEVELS} levels",
+                   "n_particles": N, "precision": "fp32 pair arithmetic + fp64 integration",
+                   "particle_steps_per_step": mixed["finished"]},
         "roofline": roofline(mixed, "fp32"),
         "passes_ms_per_step": {k: v / args.steps for k, v in mixed["passes"].items()},
         "interactions_per_step": {k: v // args.steps for k, v in mixed["counts"].items()},
     }
     if fp64 is not None:
-        out["fp64"] = {"value": N * args.steps / fp64["wall"],
+        out["fp64"] = {"value": fp64["particle_steps"] / fp64["wall"],
                        "ms_per_step": 1e3 * fp64["wall"] / args.steps,
                        "roofline": roofline(fp64, "fp64"),
                        "passes_ms_per_step": {k: v / args.steps for k, v in fp64["passes"].items()}}
+    if glob is not None:
+        out["global_timesteps"] = {"value": glob["particle_steps"] / glob["wall"],
+                                   "ms_per_step": 1e3 * glob["wall"] / glob["steps"],
+                                   "precision": "mixed"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

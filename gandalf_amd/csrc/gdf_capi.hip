@@ -486,9 +486,10 @@ int gdf_gravity_group_counts(gdf_ctx* c, int32_t* ncell, int32_t* nnear) {
   return GDF_OK;
 }
 
-int gdf_pass_times(gdf_ctx* c, double* out6) {
-  if (!c || !out6) return GDF_ERR_INVALID;
-  for (int k = 0; k < 6; k++) out6[k] = c->pass_ms[k];
+int gdf_pass_times(gdf_ctx* c, double* out7) {
+  if (!c || !out7) return GDF_ERR_INVALID;
+  for (int k = 0; k < 6; k++) out7[k] = c->pass_ms[k];
+  out7[6] = (double) c->grav_launches;
   return GDF_OK;
 }
 

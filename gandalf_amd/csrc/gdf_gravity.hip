@@ -519,7 +519,8 @@ int gravity_walk(gdf_ctx* c) {
   if (c->timing) {
     float ms = 0.f;
     hipEventElapsedTime(&ms, c->ev[7], c->ev[8]);
-    c->pass_ms[5] = ms;
+    c->pass_ms[5] += ms;             // summed over the passes of one step
+    c->grav_launches++;
   }
   if (c->h_counters[2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in gravity pass");
   return GDF_OK;

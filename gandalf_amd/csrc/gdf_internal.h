@@ -154,6 +154,8 @@ struct gdf_ctx {
   bool timing = false;
   hipEvent_t ev[12] = {};
   double pass_ms[6] = {};
+  bool mark_passes = true;
+  int grav_launches = 0;       // gravity walk launches in the current step     // record the pass-boundary events (first force pass of a step)
   long long icounts_last[6] = {};
 };
 

@@ -25,7 +25,7 @@ double initial_h_guess(const std::vector<double>& r, double kernrange, double h_
 }
 
 void record(gdf_ctx* c, int e) {
-  if (c->timing) hipEventRecord(c->ev[e], c->stream);
+  if (c->timing && c->mark_passes) hipEventRecord(c->ev[e], c->stream);
 }
 
 }  // namespace
@@ -151,6 +151,8 @@ int GradhSphSimulationGpu<ndim>::MainLoop() {
   if (c->nlevels > 1) return MainLoopBlock();
   GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
   record(c, 0);
+  c->pass_ms[5] = 0.0;
+  c->grav_launches = 0;
   c->nsteps++;
   const double dtstep = c->timestep;
   c->t = c->t + c->timestep;
@@ -200,6 +202,8 @@ int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
   gdf_ctx* c = ctx;
   GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
   record(c, 0);
+  c->pass_ms[5] = 0.0;
+  c->grav_launches = 0;
   c->n_int += 1;
   c->nsteps++;
   c->t = c->t + c->timestep;
@@ -210,10 +214,12 @@ int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
   record(c, 2);
   int activecount = 0, rounds = 0;
   do {
+    c->mark_passes = rounds == 0;
     GDF_TRY(sphneib.UpdateAllSphProperties(false));
-    if (rounds == 0) record(c, 5);
+    record(c, 5);
     GDF_TRY(ComputeForces());
-    if (rounds == 0) record(c, 4);
+    record(c, 4);
+    c->mark_passes = true;
     // SphLeapfrogKDK::CheckTimesteps: the active flags are rewritten, set only for the
     // particles whose step a neighbour's level ends early
     GDF_TRY(gdf::block_check_timesteps(c, &activecount));
@@ -271,6 +277,8 @@ int NbodySimulationGpu<ndim>::MainLoop() {
   if (!c->setup_done) return gdf::fail(c, GDF_ERR_INVALID, "MainLoop before setup");
   GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
   record(c, 0);
+  c->pass_ms[5] = 0.0;
+  c->grav_launches = 0;
   c->nsteps++;
   const double dtstep = c->timestep;
   c->t = c->t + c->timestep;

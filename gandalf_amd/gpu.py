@@ -239,9 +239,10 @@ class GpuSim:
         self._chk(lib().gdf_set_timing(self._h, int(on)))
 
     def pass_times(self):
-        out = np.zeros(6)
+        out = np.zeros(7)
         self._chk(lib().gdf_pass_times(self._h, _p(out)))
-        return dict(zip(("tree", "density", "hydro", "gravity", "integrate", "k_gravity"), out))
+        return dict(zip(("tree", "density", "hydro", "gravity", "integrate", "k_gravity",
+                         "k_gravity_launches"), out))
 
     def interaction_counts(self):
         out = np.zeros(6, dtype=np.int64)

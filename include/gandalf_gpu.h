@@ -158,8 +158,10 @@ int gdf_gravity_group_counts(gdf_ctx* ctx, int32_t* ncell, int32_t* nnear);
 int gdf_nbody_direct_forces(gdf_ctx* ctx, int smoothed);
 
 /* event timing of the named passes of the last step, milliseconds:
- * {tree, density, hydro, gravity, integrate, gravity walk kernel alone} */
-int gdf_pass_times(gdf_ctx* ctx, double* out6);
+ * {tree, density, hydro, gravity, integrate, gravity walk kernel alone (summed over its
+ * launches)}, then the number of gravity walk launches in the step (block timesteps repeat the
+ * force pass for particles whose step CheckTimesteps ends early) */
+int gdf_pass_times(gdf_ctx* ctx, double* out7);
 int gdf_set_timing(gdf_ctx* ctx, int enable);
 
 #ifdef __cplusplus

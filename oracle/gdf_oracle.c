@@ -649,6 +649,37 @@ static void scatter_gather_lists(const orc_params* p, const obox* b, int N, cons
   *out = l;
 }
 
+/* the same sets for the rows with act[i] only (the others empty): a search of radius
+ * k*max_j(h_j) around i filtered by the test above, r2 < k^2 h_i^2 or r2 < k^2 h_j^2; cheaper
+ * than the transposition when few particles are active */
+static void scatter_gather_lists_active(const orc_params* p, const obox* b, int N,
+                                        const double* r, const double* h,
+                                        const unsigned char* act, ivec** out) {
+  const double kr = orc_kernel_range(p);
+  const double krsqd = kr*kr;
+  double hmax = 0.0;
+  for (int i=0; i<N; i++) if (h[i] > hmax) hmax = h[i];
+  ogrid g; grid_build(&g, b, p, N, r, kr*hmax);
+  ivec* l = (ivec*) calloc(N, sizeof(ivec));
+  #pragma omp parallel for schedule(dynamic, 16)
+  for (int i=0; i<N; i++) {
+    if (!act[i]) continue;
+    ivec cand = {0, 0, 0};
+    gather_one(&g, b, r, i, krsqd*hmax*hmax, 1, &cand);
+    const double* xi = &r[3*i];
+    for (int t=0; t<cand.n; t++) {
+      const int j = cand.data[t];
+      double dr[3] = {r[3*j] - xi[0], r[3*j+1] - xi[1], r[3*j+2] - xi[2]};
+      nearest(b, dr);
+      const double drsqd = dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2];
+      if (drsqd < krsqd*h[i]*h[i] || drsqd < krsqd*h[j]*h[j]) iv_push(&l[i], j);
+    }
+    free(cand.data);
+  }
+  grid_free(&g);
+  *out = l;
+}
+
 /* act (optional): forces of the active particles only; level/levelneib (optional): the
  * neighbour-level bookkeeping of GradhSph.cpp:445-446 merged as GradhSphTree.cpp:375-417:
  * for an active i and every j of its scatter-gather set (i itself included),
@@ -662,7 +693,10 @@ static void hydro_forces_masked(const orc_params* p, int variant, int N, const d
   okern kern; kern_init(&kern, p);
   obox b; box_init(&b, p, 0);
   ivec* lists;
-  scatter_gather_lists(p, &b, N, r, h, &lists);
+  int nact = N;
+  if (act) { nact = 0; for (int i=0; i<N; i++) nact += act[i] != 0; }
+  if (act && nact < N/4) scatter_gather_lists_active(p, &b, N, r, h, act, &lists);
+  else scatter_gather_lists(p, &b, N, r, h, &lists);
   const double alpha_visc = p->alpha_visc, beta_visc = p->beta_visc;
   const double gammam1 = p->gamma - 1.0;
   #pragma omp parallel for schedule(dynamic, 32)
