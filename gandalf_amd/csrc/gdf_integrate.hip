@@ -467,7 +467,10 @@ __global__ void k_block_update(BlkArgs b) {
     lv = level;
   }
   lv = wave_max_int(lv);
-  if (__lane_id() == 0) atomicMax(b.counter, lv);
+  // nearly every wave holds the same maximum: read before the atomic so that only a wave that
+  // raises the maximum contends for the counter (the value only grows)
+  if (__lane_id() == 0 && __hip_atomic_load(b.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < lv)
+    atomicMax(b.counter, lv);
 }
 
 // integer time rescaled when level_max changes: mul = 1 multiplies nstep and nlast by f, else
