@@ -97,7 +97,7 @@ void free_particles(gdf_ctx* c) {
                   c->first, c->last, c->parent, c->visit, c->front, c->bbmin, c->bbmax, c->geo_c,
                   c->geo_h, c->mp, c->gstart, c->grp_first, c->grp_count, c->grp_node, c->nsel,
                   c->sel_tmp, c->stacks, c->grav_nc,
-                  c->grav_nn, c->tmpd};
+                  c->grav_nn, c->tmpd, c->grp_root};
   for (void* p : ptrs) if (p) hipFree(p);
   c->iorig = c->iorig_alt = nullptr; c->posm = c->hrecA = c->hrecB = c->grec = nullptr;
   c->status = nullptr; c->keys = c->keys_alt = nullptr; c->vals = c->vals_alt = nullptr;
@@ -106,7 +106,7 @@ void free_particles(gdf_ctx* c) {
   c->gstart = c->grp_first = c->grp_count = c->grp_node = c->nsel = nullptr; c->sel_tmp = nullptr;
   c->stacks = nullptr;
   sph_free_lists(c);
-  c->grav_nc = c->grav_nn = nullptr; c->tmpd = nullptr;
+  c->grav_nc = c->grav_nn = nullptr; c->tmpd = nullptr; c->grp_root = nullptr;
   c->Ncap = 0; c->N = 0;
 }
 
@@ -204,6 +204,32 @@ int upload_kernel_tables(gdf_ctx* c) {
 }
 
 }  // namespace
+
+// context helpers for the serving pool (gdf_pool.hip)
+int ctx_alloc_particles(gdf_ctx* c, int ncap) { return alloc_particles(c, ncap); }
+
+// a field of particles [i0, i0 + n) whose iorig entries number them 0..n-1 (one segment of a
+// serving batch), scattered into that local order
+int ctx_get_field_range(gdf_ctx* c, const char* name, int i0, int n, double* out) {
+  double* p[3] = {nullptr, nullptr, nullptr};
+  int w = 0;
+  if (!field_ptrs(c, name, p, &w)) return fail(c, GDF_ERR_INVALID, std::string("unknown field ") + name);
+  if (n <= 0) return GDF_OK;
+  k_scatter_out<<<nblocks(n, BS), BS, 0, c->stream>>>(n, w, p[0] + i0, p[1] ? p[1] + i0 : nullptr,
+                                                      p[2] ? p[2] + i0 : nullptr, c->iorig + i0,
+                                                      c->tmpd);
+  GDF_HIP(c, hipGetLastError());
+  GDF_HIP(c, hipMemcpyAsync(out, c->tmpd, sizeof(double)*w*(size_t) n, hipMemcpyDeviceToHost, c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  return GDF_OK;
+}
+
+int ctx_field_width(gdf_ctx* c, const char* name) {
+  double* p[3] = {nullptr, nullptr, nullptr};
+  int w = 0;
+  return field_ptrs(c, name, p, &w) ? w : 0;
+}
+
 }  // namespace gdf
 
 using namespace gdf;
@@ -309,6 +335,8 @@ void gdf_destroy(gdf_ctx* c) {
   if (c->red) hipFree(c->red);
   if (c->ktab) hipFree(c->ktab);
   if (c->ktabf) hipFree(c->ktabf);
+  void* segp[] = {c->d_seg_off, c->seg_root, c->seg_dt, c->seg_bnd};
+  for (void* q : segp) if (q) hipFree(q);
   if (c->h_counters) hipHostFree(c->h_counters);
   if (c->h_red) hipHostFree(c->h_red);
   for (int e = 0; e < 12; e++) if (c->ev[e]) hipEventDestroy(c->ev[e]);

@@ -44,6 +44,7 @@ struct GravArgs {
   int* stacks; int* counter; int* flags; long long* icount;
   int32_t* gnc; int32_t* gnn;
   const int* act;         // block timesteps: only active targets (NULL = all)
+  const int* grp_root;    // serving batches: walk from the group's segment subtree (NULL = root)
   PhysParams p;
 };
 
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
     int sp = 1, phead = 0, ptail = 0, chead = 0, ctail = 0;
     int nnear = 0, ncell = 0;
     bool overflow = false;
-    if (lane == 0) stk[0] = 0;
+    if (lane == 0) stk[0] = a.grp_root ? a.grp_root[g] : 0;   // the group's segment subtree
     __threadfence_block();
     while (sp > 0) {
       const int b = min(sp, kWave);
@@ -496,6 +497,7 @@ int gravity_walk(gdf_ctx* c) {
   a.stacks = c->stacks; a.counter = c->counters + 4; a.flags = c->counters;
   a.icount = c->icount; a.gnc = c->grav_nc; a.gnn = c->grav_nn;
   a.act = c->nlevels > 1 ? c->pi[I_ACT] : nullptr;
+  a.grp_root = c->nseg > 0 ? c->grp_root : nullptr;
   a.p = c->phys;
   k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
   k_zero8<<<1, 64, 0, s>>>(c->counters);

@@ -21,6 +21,7 @@ constexpr int RED_BLOCKS = 1024;
 struct AdvArgs {
   int N, energy;
   double dt;
+  const double* segdt; const int* seg;    // serving batches: the step of the particle's segment
   const double* r0[3]; const double* v0[3]; const double* a0[3];
   double* r[3]; double* v[3];
   const double* u0; const double* dudt0; double* u;
@@ -30,7 +31,7 @@ __global__ void k_advance(AdvArgs a) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x*BS + threadIdx.x;
   if (i >= a.N) return;
-  const double dt = a.dt;
+  const double dt = a.segdt ? a.segdt[a.seg[i]] : a.dt;
   for (int k = 0; k < 3; k++) a.r[k][i] = a.r0[k][i] + a.v0[k][i]*dt + 0.5*a.a0[k][i]*dt*dt;
   for (int k = 0; k < 3; k++) a.v[k][i] = a.v0[k][i] + a.a0[k][i]*dt;
   if (a.energy) a.u[i] = a.u0[i] + a.dudt0[i]*dt;
@@ -53,6 +54,7 @@ __global__ void k_wrap(WrapArgs a) {
 struct EndArgs {
   int N, energy;
   double dt;
+  const double* segdt; const int* seg;
   const double* a[3]; double* v[3]; double* r0[3]; double* v0[3]; double* a0[3];
   const double* r[3];
   const double* dudt; double* u; double* u0; double* dudt0;
@@ -62,7 +64,7 @@ __global__ void k_end_step(EndArgs e) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x*BS + threadIdx.x;
   if (i >= e.N) return;
-  const double dt = e.dt;
+  const double dt = e.segdt ? e.segdt[e.seg[i]] : e.dt;
   for (int k = 0; k < 3; k++) {
     const double a = e.a[k][i], a0 = e.a0[k][i];
     const double v = e.v[k][i] + 0.5*dt*(a - a0);
@@ -133,6 +135,22 @@ __global__ void k_min_final(int nb, const double* __restrict__ part, double* __r
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = s[0];
+}
+
+// per-segment minimum of the per-particle timesteps: one block per segment
+__global__ void k_seg_min(const int* __restrict__ off, const double* __restrict__ v,
+                          double* __restrict__ out) {
+  __shared__ double sh[BS];
+  const int sg = blockIdx.x;
+  double best = 9.9e50;
+  for (int i = off[sg] + threadIdx.x; i < off[sg + 1]; i += BS) best = fmin(best, v[i]);
+  sh[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = BS/2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[sg] = sh[0];
 }
 
 __global__ void k_max_h(int N, const double* __restrict__ h, double* __restrict__ part) {
@@ -214,6 +232,7 @@ int integ_advance(gdf_ctx* c, double dt) {
     a.r[k] = c->pf[F_X + k]; a.v[k] = c->pf[F_VX + k];
   }
   a.u0 = c->pf[F_U0]; a.dudt0 = c->pf[F_DUDT0]; a.u = c->pf[F_U];
+  a.segdt = nullptr; a.seg = nullptr;
   k_advance<<<nblocks(c->N, BS), BS, 0, c->stream>>>(a);
   GDF_HIP(c, hipGetLastError());
   c->tree_valid = false;
@@ -241,8 +260,62 @@ int integ_end_step(gdf_ctx* c, double dtstep) {
     e.v0[k] = c->pf[F_VX0 + k]; e.a0[k] = c->pf[F_AX0 + k]; e.r[k] = c->pf[F_X + k];
   }
   e.dudt = c->df[D_DUDT]; e.u = c->pf[F_U]; e.u0 = c->pf[F_U0]; e.dudt0 = c->pf[F_DUDT0];
+  e.segdt = nullptr; e.seg = nullptr;
   k_end_step<<<nblocks(c->N, BS), BS, 0, c->stream>>>(e);
   GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int integ_seg_advance(gdf_ctx* c) {
+  AdvArgs a;
+  a.N = c->N; a.energy = c->phys.energy_integration; a.dt = 0.0;
+  for (int k = 0; k < 3; k++) {
+    a.r0[k] = c->pf[F_X0 + k]; a.v0[k] = c->pf[F_VX0 + k]; a.a0[k] = c->pf[F_AX0 + k];
+    a.r[k] = c->pf[F_X + k]; a.v[k] = c->pf[F_VX + k];
+  }
+  a.u0 = c->pf[F_U0]; a.dudt0 = c->pf[F_DUDT0]; a.u = c->pf[F_U];
+  a.segdt = c->seg_dt; a.seg = c->pi[I_SEG];
+  k_advance<<<nblocks(c->N, BS), BS, 0, c->stream>>>(a);
+  GDF_HIP(c, hipGetLastError());
+  c->tree_valid = false;
+  return GDF_OK;
+}
+
+int integ_seg_end_step(gdf_ctx* c) {
+  EndArgs e;
+  e.N = c->N; e.energy = c->phys.energy_integration; e.dt = 0.0;
+  for (int k = 0; k < 3; k++) {
+    e.a[k] = c->df[D_AX + k]; e.v[k] = c->pf[F_VX + k]; e.r0[k] = c->pf[F_X0 + k];
+    e.v0[k] = c->pf[F_VX0 + k]; e.a0[k] = c->pf[F_AX0 + k]; e.r[k] = c->pf[F_X + k];
+  }
+  e.dudt = c->df[D_DUDT]; e.u = c->pf[F_U]; e.u0 = c->pf[F_U0]; e.dudt0 = c->pf[F_DUDT0];
+  e.segdt = c->seg_dt; e.seg = c->pi[I_SEG];
+  k_end_step<<<nblocks(c->N, BS), BS, 0, c->stream>>>(e);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+// SphIntegration::Timestep per particle and Simulation::ComputeGlobalTimestep per segment
+int integ_seg_timesteps(gdf_ctx* c, double* out) {
+  TsArgs t;
+  t.N = c->N; t.hydro = c->phys.hydro_forces; t.energy = c->phys.energy_integration;
+  t.nbody = c->phys.nbody;
+  t.courant = c->phys.courant_mult; t.emult = c->phys.energy_mult;
+  t.accel = c->phys.nbody ? c->phys.nbody_mult : c->phys.accel_mult;
+  t.h = c->pf[F_H]; t.sound = c->df[D_SOUND]; t.div_v = c->df[D_DIVV];
+  for (int k = 0; k < 3; k++) t.a[k] = c->df[D_AX + k];
+  t.u = c->pf[F_U]; t.dudt = c->df[D_DUDT]; t.dt_out = c->df[D_DT]; t.part = c->red;
+  const int nb = std::min(nblocks(c->N, BS), RED_BLOCKS);
+  k_timestep<<<nb, BS, 0, c->stream>>>(t);
+  k_seg_min<<<c->nseg, BS, 0, c->stream>>>(c->d_seg_off, c->df[D_DT], c->red + RED_BLOCKS*ND);
+  GDF_HIP(c, hipMemcpyAsync(c->h_red, c->red + RED_BLOCKS*ND, sizeof(double)*c->nseg,
+                            hipMemcpyDeviceToHost, c->stream));
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  for (int sg = 0; sg < c->nseg; sg++) {
+    out[sg] = c->h_red[sg];
+    if (!(out[sg] > 0.0) || out[sg] > 1.0e20)
+      return fail(c, GDF_ERR_TIMESTEP, "Error : Hydro timestep too large");
+  }
   return GDF_OK;
 }
 

@@ -29,7 +29,11 @@ enum PField {
 // Per-particle integer state, permuted with the particles: block-timestep level, neighbour
 // level, integer step length and start (Particle.h:140-143), and the active / end-of-step
 // flags (Flags.h active, end_timestep) as 0/1.
-enum IField { I_LEVEL, I_LEVELNEIB, I_NSTEP, I_NLAST, I_ACT, I_END, I_NINT };
+enum IField { I_LEVEL, I_LEVELNEIB, I_NSTEP, I_NLAST, I_ACT, I_END, I_SEG, I_NINT };
+// Serving batches: up to kMaxSeg independent simulations share one context, each a contiguous
+// particle segment; the segment id prefixes the Morton key (7 bits + 19 bits per axis), so the
+// sort keeps segments apart and each segment is one subtree of the LBVH.
+constexpr int kMaxSeg = 128;
 // Derived per-particle fields, recomputed every step after the permutation.
 enum DField {
   D_AX, D_AY, D_AZ, D_ATX, D_ATY, D_ATZ, D_RHO, D_DUDT, D_DIVV, D_GPOT, D_INVOMEGA,
@@ -149,6 +153,15 @@ struct gdf_ctx {
   int n_int = 0, nresync = 0, level_max = 0, level_step = 0;
   double dt_max = 0.0;
   long long finished_last = 0;   // particles that ended a step in the last MainLoop
+  // serving batch (nseg > 0): segment s holds particles [seg_off[s], seg_off[s+1]); per-segment
+  // bounds, subtree roots and step lengths live on the device
+  int nseg = 0;
+  std::vector<int> seg_off;
+  int* d_seg_off = nullptr;      // [kMaxSeg + 1]
+  int* seg_root = nullptr;       // [kMaxSeg]
+  int* grp_root = nullptr;       // [Ncap]: subtree root of each walk group's segment
+  double* seg_dt = nullptr;      // [kMaxSeg]: drift / kick length of the segment's step
+  double* seg_bnd = nullptr;     // [4 kMaxSeg]: lower corner and scale of its Morton grid
 
   // timing
   bool timing = false;
@@ -168,6 +181,12 @@ int check_hip(gdf_ctx* c, hipError_t e, const char* what);
 
 // tree (gdf_tree.hip)
 int tree_alloc(gdf_ctx* c);
+int seg_alloc(gdf_ctx* c);
+// context helpers (gdf_capi.hip)
+int ctx_alloc_particles(gdf_ctx* c, int ncap);
+int ctx_get_field_range(gdf_ctx* c, const char* name, int i0, int n, double* out);
+int ctx_field_width(gdf_ctx* c, const char* name);
+int seg_upload(gdf_ctx* c);
 int tree_build(gdf_ctx* c);
 int tree_refit_h(gdf_ctx* c);
 TreeView tree_view(const gdf_ctx* c);
@@ -202,6 +221,10 @@ int integ_max_h(gdf_ctx* c, double* hmax);
 int integ_zero_forces(gdf_ctx* c);
 int integ_add_atree(gdf_ctx* c);
 int integ_pack_posm(gdf_ctx* c);
+// serving batches: per-segment timestep minima into out[nseg]; drift/kick with seg_dt
+int integ_seg_timesteps(gdf_ctx* c, double* out);
+int integ_seg_advance(gdf_ctx* c);
+int integ_seg_end_step(gdf_ctx* c);
 int nbody_direct(gdf_ctx* c, int smoothed);
 
 inline int nblocks(long long n, int bs) { return (int) ((n + bs - 1) / bs); }

@@ -41,6 +41,7 @@ struct CandWalk {
   const double4* geo_c; const double4* geo_h;
   const double* x; const double* y; const double* z; const double* hcand; const double* htarget;
   const int* status;                 // gather walks: only targets with status 1 take part
+  const int* grp_root;               // serving batches: the group's segment subtree (NULL = root)
   int* idx; unsigned long long* mask; int* ncand;
   int* stacks; int* counter; int* flags;
   PhysParams p;
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
     int nbat = 0;
     int sp = 1, head = 0, tail = 0;
     bool overflow = false;
-    if (lane == 0) stk[0] = 0;
+    if (lane == 0) stk[0] = a.grp_root ? a.grp_root[g] : 0;   // the group's segment subtree
     __threadfence_block();
     while (sp > 0) {
       const int b = min(sp, kWave);
@@ -680,6 +681,7 @@ int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status)
   w.hcand = c->pf[F_H];
   w.htarget = mode == 0 ? c->df[D_HSEARCH] : c->pf[F_H];
   w.status = status;
+  w.grp_root = c->nseg > 0 ? c->grp_root : nullptr;
   w.idx = cs.idx; w.mask = cs.mask; w.ncand = cs.ncand;
   w.stacks = c->stacks; w.counter = c->counters + 4; w.flags = c->counters;
   w.p = c->phys;

@@ -72,6 +72,55 @@ __global__ __launch_bounds__(BS) void k_bounds_final(int nb, const double* __res
   out[3] = 2097152.0/span;
 }
 
+// ---- serving batches: per-segment bounds and segment-prefixed keys ----
+// partial-reduction blocks per segment: kMaxSeg x 16 x 6 partials fit the reduction buffer
+constexpr int kSegBlocks = 16;
+
+// grid (kSegBlocks, nseg): block (b, s) folds a strided share of segment s
+__global__ void k_seg_bounds_partial(const int* __restrict__ off, const double* __restrict__ x,
+                                     const double* __restrict__ y, const double* __restrict__ z,
+                                     double* __restrict__ part) {
+  __shared__ double sh[6][BS];
+  const int sg = blockIdx.y, b = blockIdx.x;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = off[sg] + b*BS + threadIdx.x; i < off[sg + 1]; i += kSegBlocks*BS) {
+    const double v[3] = {x[i], y[i], z[i]};
+    for (int k = 0; k < 3; k++) { lo[k] = fmin(lo[k], v[k]); hi[k] = fmax(hi[k], v[k]); }
+  }
+  for (int k = 0; k < 3; k++) { sh[k][threadIdx.x] = lo[k]; sh[3+k][threadIdx.x] = hi[k]; }
+  __syncthreads();
+  for (int w = BS/2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 3; k++) {
+        sh[k][threadIdx.x] = fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + w]);
+        sh[3+k][threadIdx.x] = fmax(sh[3+k][threadIdx.x], sh[3+k][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[(sg*kSegBlocks + b)*6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// one 64-thread block per segment: lower corner and scale of a 2^19 grid per axis
+__global__ void k_seg_bounds_final(const double* __restrict__ part, double* __restrict__ bnd) {
+  const int sg = blockIdx.x, t = threadIdx.x;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  if (t < kSegBlocks)
+    for (int k = 0; k < 3; k++) {
+      lo[k] = part[(sg*kSegBlocks + t)*6 + k];
+      hi[k] = part[(sg*kSegBlocks + t)*6 + 3 + k];
+    }
+  for (int k = 0; k < 3; k++)
+    for (int d = 32; d >= 1; d >>= 1) {
+      lo[k] = fmin(lo[k], __shfl_xor(lo[k], d, 64));
+      hi[k] = fmax(hi[k], __shfl_xor(hi[k], d, 64));
+    }
+  if (t != 0) return;
+  double span = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+  if (!(span > 0.0)) span = 1.0;
+  bnd[4*sg] = lo[0]; bnd[4*sg + 1] = lo[1]; bnd[4*sg + 2] = lo[2];
+  bnd[4*sg + 3] = 524288.0/span;
+}
+
 __device__ __forceinline__ uint64_t spread3(uint64_t x) {
   x &= 0x1fffffULL;
   x = (x | (x << 32)) & 0x1f00000000ffffULL;
@@ -131,6 +180,47 @@ __device__ __forceinline__ int delta(const uint64_t* __restrict__ key, int N, in
   const uint64_t a = key[i], b = key[j];
   if (a != b) return __clzll((long long) (a ^ b));
   return 64 + __clz((int) (i ^ (int) j));
+}
+
+// segment-prefixed key: segment id in bits 57-63, 19 bits per axis below
+__global__ void k_morton_seg(int N, const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, const int* __restrict__ seg,
+                             const double* __restrict__ bnd, uint64_t* __restrict__ keys,
+                             int* __restrict__ vals) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= N) return;
+  const int sg = seg[i];
+  const double scale = bnd[4*sg + 3];
+  const double v[3] = {x[i], y[i], z[i]};
+  uint64_t q[3];
+  for (int k = 0; k < 3; k++) {
+    const double f = __dmul_rn(__dadd_rn(v[k], -bnd[4*sg + k]), scale);
+    q[k] = f >= 524287.0 ? 524287ULL : (uint64_t) f;
+  }
+  keys[i] = ((uint64_t) sg << 57) | (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+  vals[i] = i;
+}
+
+// the node spanning exactly segment s is its subtree root (the keys of a segment share the
+// segment prefix and no other key does)
+__global__ void k_seg_roots(int N, int nseg, const int* __restrict__ off,
+                            const int* __restrict__ first, const int* __restrict__ last,
+                            const int* __restrict__ seg, int* __restrict__ root) {
+  const int n = blockIdx.x*BS + threadIdx.x;
+  if (n < N - 1) {
+    const int f = first[n], l = last[n], sg = seg[f];
+    if (f == off[sg] && l == off[sg + 1] - 1) root[sg] = n;
+  } else if (n < 2*N - 1) {
+    const int leaf = n - (N - 1), sg = seg[leaf];
+    if (off[sg + 1] - off[sg] == 1) root[sg] = n;
+  }
+}
+
+__global__ void k_grp_roots(int ngroups, const int* __restrict__ grp_first,
+                            const int* __restrict__ seg, const int* __restrict__ root,
+                            int* __restrict__ grp_root) {
+  const int g = blockIdx.x*BS + threadIdx.x;
+  if (g < ngroups) grp_root[g] = root[seg[grp_first[g]]];
 }
 
 // Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees, and k-d trees"
@@ -427,6 +517,36 @@ static NodeArgs node_args(gdf_ctx* c) {
   return a;
 }
 
+int seg_alloc(gdf_ctx* c) {
+  if (c->d_seg_off) return GDF_OK;
+  GDF_HIP(c, hipMalloc(&c->d_seg_off, sizeof(int)*(kMaxSeg + 1)));
+  GDF_HIP(c, hipMalloc(&c->seg_root, sizeof(int)*kMaxSeg));
+  GDF_HIP(c, hipMalloc(&c->seg_dt, sizeof(double)*kMaxSeg));
+  GDF_HIP(c, hipMalloc(&c->seg_bnd, sizeof(double)*4*kMaxSeg));
+  return GDF_OK;
+}
+
+// segment offsets to the device and the per-particle segment ids
+__global__ void k_fill_seg(int n0, int n1, int sg, int* __restrict__ seg) {
+  const int i = n0 + blockIdx.x*BS + threadIdx.x;
+  if (i < n1) seg[i] = sg;
+}
+
+int seg_upload(gdf_ctx* c) {
+  GDF_TRY(seg_alloc(c));
+  if (c->nseg < 1 || c->nseg > kMaxSeg || (int) c->seg_off.size() != c->nseg + 1)
+    return fail(c, GDF_ERR_INVALID, "bad segment table");
+  GDF_HIP(c, hipMemcpyAsync(c->d_seg_off, c->seg_off.data(), sizeof(int)*(c->nseg + 1),
+                            hipMemcpyHostToDevice, c->stream));
+  for (int sg = 0; sg < c->nseg; sg++) {
+    const int n0 = c->seg_off[sg], n1 = c->seg_off[sg + 1];
+    if (n1 > n0) k_fill_seg<<<nblocks(n1 - n0, BS), BS, 0, c->stream>>>(n0, n1, sg, c->pi[I_SEG]);
+  }
+  GDF_HIP(c, hipGetLastError());
+  c->tree_valid = false;
+  return GDF_OK;
+}
+
 int tree_alloc(gdf_ctx* c) {
   const size_t N = (size_t) c->Ncap;
   const size_t nn = 2*N;  // >= 2N-1
@@ -451,11 +571,12 @@ int tree_alloc(gdf_ctx* c) {
   GDF_HIP(c, hipMalloc(&c->grp_first, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->grp_count, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->grp_node, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->grp_root, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->nsel, sizeof(int)*4));
   // temporary storage of the sort and the group compaction
   size_t b1 = 0, b2 = 0;
   GDF_HIP(c, rocprim::radix_sort_pairs(nullptr, b1, c->keys, c->keys_alt, c->vals, c->vals_alt,
-                                       (unsigned int) N, 0, 63, c->stream));
+                                       (unsigned int) N, 0, 64, c->stream));
   GDF_HIP(c, rocprim::select(nullptr, b2, rocprim::counting_iterator<int>(0),
                              (unsigned char*) c->visit, c->grp_first, c->nsel, N, c->stream));
   c->sort_tmp_bytes = b1;
@@ -471,14 +592,23 @@ int tree_build(gdf_ctx* c) {
   if (N <= 0) return fail(c, GDF_ERR_INVALID, "tree build without particles");
   const int nbp = std::min(nblocks(N, BS), 1024);
   // 1. bounds and Morton keys
-  k_bounds_partial<<<nbp, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->red);
-  k_bounds_final<<<1, BS, 0, s>>>(nbp, c->red, c->red + 6*1024);
-  k_morton<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->red + 6*1024,
-                                         c->keys, c->vals);
+  if (c->nseg > 0) {
+    k_seg_bounds_partial<<<dim3(kSegBlocks, c->nseg), BS, 0, s>>>(c->d_seg_off, c->pf[F_X],
+                                                                  c->pf[F_Y], c->pf[F_Z], c->red);
+    k_seg_bounds_final<<<c->nseg, 64, 0, s>>>(c->red, c->seg_bnd);
+    k_morton_seg<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z],
+                                               c->pi[I_SEG], c->seg_bnd, c->keys, c->vals);
+  } else {
+    k_bounds_partial<<<nbp, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->red);
+    k_bounds_final<<<1, BS, 0, s>>>(nbp, c->red, c->red + 6*1024);
+    k_morton<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z],
+                                           c->red + 6*1024, c->keys, c->vals);
+  }
   // 2. stable radix sort of (key, index)
   size_t bytes = c->sort_tmp_bytes;
   GDF_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp, bytes, c->keys, c->keys_alt, c->vals,
-                                       c->vals_alt, (unsigned int) N, 0, 63, s));
+                                       c->vals_alt, (unsigned int) N, 0,
+                                       c->nseg > 0 ? 64 : 63, s));
   std::swap(c->keys, c->keys_alt);
   std::swap(c->vals, c->vals_alt);
   // 3. permute persistent particle state into key order
@@ -519,6 +649,12 @@ int tree_build(gdf_ctx* c) {
   GDF_HIP(c, hipStreamSynchronize(s));
   c->ngroups = c->h_counters[0];
   if (c->ngroups <= 0 || c->ngroups > N) return fail(c, GDF_ERR_INVALID, "walk group build failed");
+  if (c->nseg > 0) {
+    k_seg_roots<<<nblocks(2*N - 1, BS), BS, 0, s>>>(N, c->nseg, c->d_seg_off, c->first, c->last,
+                                                    c->pi[I_SEG], c->seg_root);
+    k_grp_roots<<<nblocks(c->ngroups, BS), BS, 0, s>>>(c->ngroups, c->grp_first, c->pi[I_SEG],
+                                                       c->seg_root, c->grp_root);
+  }
   c->tree_valid = true;
   c->sg_valid = false;
   GDF_HIP(c, hipGetLastError());

@@ -19,6 +19,9 @@ PREC = {"fp64": 0, "mixed": 1}
 FIELDS3 = ("r", "v", "a", "atree", "r0", "v0", "a0", "adot")
 
 
+GDF_ERR_CAPACITY = -5       # status of the C ABI: capacity exhausted / request does not fit
+
+
 class GdfError(RuntimeError):
     """Non-zero status from the C ABI (the reference raises ExceptionHandler errors)."""
 
@@ -68,6 +71,15 @@ PROTOTYPES = {
     "gdf_main_loop": (_i, [_vp, _i]),
     "gdf_time": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)]),
     "gdf_block_state": (_i, [_vp, _vp, _vp]),
+    "gdf_pool_create": (_i, [C.POINTER(Config), _i64, C.POINTER(_vp)]),
+    "gdf_pool_destroy": (None, [_vp]),
+    "gdf_pool_last_error": (C.c_char_p, [_vp]),
+    "gdf_pool_admit": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, C.POINTER(_i)]),
+    "gdf_pool_step": (_i, [_vp, _vp]),
+    "gdf_pool_status": (_i, [_vp, _i, _vp, C.POINTER(_d)]),
+    "gdf_pool_get_field": (_i, [_vp, _i, C.c_char_p, _vp]),
+    "gdf_pool_release": (_i, [_vp, _i]),
+    "gdf_pool_resident_particles": (_i64, [_vp]),
     "gdf_neighbour_sets": (_i64, [_vp, _i, _vp, _vp, _i64]),
     "gdf_tree_arrays": (_i, [_vp] + [_vp] * 15 + [C.POINTER(_i)]),
     "gdf_interaction_counts": (_i, [_vp, _vp]),
@@ -289,3 +301,73 @@ class GpuSim:
         nn = np.zeros(g, np.int32)
         self._chk(lib().gdf_gravity_group_counts(self._h, _p(nc), _p(nn)))
         return nc, nn
+
+
+class GpuPool:
+    """A serving batch (gdf_pool): independent requests of one configuration class resident in
+    one device context, advanced together one step at a time."""
+
+    def __init__(self, params: dict, precision: str = "mixed", max_particles: int = 1 << 22,
+                 group_max: int = 64):
+        self.params = dict(params)
+        cfg = config_from_params(self.params, precision, group_max)
+        h = C.c_void_p()
+        rc = lib().gdf_pool_create(C.byref(cfg), int(max_particles), C.byref(h))
+        if rc != 0:
+            raise GdfError(f"gdf_pool_create failed ({rc})")
+        self._h = h
+        self.max_particles = int(max_particles)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gdf_pool_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise GdfError(f"status {rc}: {lib().gdf_pool_last_error(self._h).decode()}")
+        return rc
+
+    def admit(self, r, v, m, u, nsteps):
+        """Request id, or None when it does not fit beside the resident requests."""
+        r, v, m, u = _d(r), _d(v), _d(m), _d(u)
+        rid = C.c_int(-1)
+        rc = lib().gdf_pool_admit(self._h, r.shape[0], _p(r), _p(v), _p(m), _p(u), int(nsteps),
+                                  C.byref(rid))
+        if rc == GDF_ERR_CAPACITY:
+            return None
+        self._chk(rc)
+        return rid.value
+
+    def step(self):
+        """One step of every resident request: (requests stepped, particle-steps)."""
+        out = np.zeros(2, dtype=np.int64)
+        self._chk(lib().gdf_pool_step(self._h, _p(out)))
+        return int(out[0]), int(out[1])
+
+    def status(self, rid):
+        """(state 0 unknown / 1 resident / 2 finished, steps done, particles, time)"""
+        out = np.zeros(3, dtype=np.int64)
+        t = C.c_double()
+        self._chk(lib().gdf_pool_status(self._h, int(rid), _p(out), C.byref(t)))
+        return int(out[0]), int(out[1]), int(out[2]), t.value
+
+    def get(self, rid, name):
+        n = self.status(rid)[2]
+        w = 3 if name in FIELDS3 else 1
+        out = np.empty(n * w)
+        self._chk(lib().gdf_pool_get_field(self._h, int(rid), name.encode(), _p(out)))
+        return out.reshape(n, 3) if w == 3 else out
+
+    def release(self, rid):
+        self._chk(lib().gdf_pool_release(self._h, int(rid)))
+
+    @property
+    def resident_particles(self):
+        return int(lib().gdf_pool_resident_particles(self._h))

@@ -157,6 +157,32 @@ int gdf_gravity_group_counts(gdf_ctx* ctx, int32_t* ncell, int32_t* nnear);
  * (src/Nbody/NbodyLeapfrogKDK.cpp:78-150) */
 int gdf_nbody_direct_forces(gdf_ctx* ctx, int smoothed);
 
+/* ---- serving pool: no counterpart in the reference, which runs one simulation per process.
+ * Independent requests of one configuration class (one gdf_config) share a device context:
+ * each is set up alone, then joins a batch whose steps advance every resident request by one
+ * of its own global timesteps with one launch of each pass over all of them.  max_particles
+ * bounds the resident particles (the admission limit on HBM footprint; see
+ * gdf_pool_bytes_per_particle).  Requests need Nlevels = 1 and no direct N-body sums. */
+typedef struct gdf_pool gdf_pool;
+int gdf_pool_create(const gdf_config* cfg, int64_t max_particles, gdf_pool** out);
+void gdf_pool_destroy(gdf_pool* pool);
+const char* gdf_pool_last_error(const gdf_pool* pool);
+/* admission: GDF_ERR_CAPACITY (nothing changed) when the request does not fit beside the
+ * resident ones; the caller keeps it queued.  *id receives the request id. */
+int gdf_pool_admit(gdf_pool* pool, int N, const double* r, const double* v, const double* m,
+                   const double* u, int nsteps, int* id);
+/* one step of every resident request; requests that complete their steps leave the batch with
+ * their final fields kept.  out2 = {requests stepped, particle-steps} */
+int gdf_pool_step(gdf_pool* pool, int64_t* out2);
+/* out3 = {0 unknown, 1 resident, 2 finished; steps done; particles}; *t its simulation time */
+int gdf_pool_status(const gdf_pool* pool, int id, int64_t* out3, double* t);
+/* a per-particle field of a resident request (any gdf_get_field name) or of a finished one
+ * (r v a atree m h rho u dudt div_v gpot), in the request's import order */
+int gdf_pool_get_field(gdf_pool* pool, int id, const char* name, double* out);
+/* forget a finished request's fields */
+int gdf_pool_release(gdf_pool* pool, int id);
+int64_t gdf_pool_resident_particles(const gdf_pool* pool);
+
 /* event timing of the named passes of the last step, milliseconds:
  * {tree, density, hydro, gravity, integrate, gravity walk kernel alone (summed over its
  * launches)}, then the number of gravity walk launches in the step (block timesteps repeat the
