@@ -153,6 +153,42 @@ def cpu_baseline(nsample=131072, steps=4):
                       f"oracle/gdf_oracle.c OpenMP"}
 
 
+def ensemble(nreq=256):
+    """The serving workload: 256 requests arriving as a Poisson process (20/s), sizes
+    log-uniform in 4096-262144, the three generators in rotation, 20 mixed-precision steps
+    each.  The stream is served in real time (aggregate particle-steps/s over the run, per-step
+    latency, queueing before admission); the same requests are then served with every request
+    queued at once (the batches' capacity) and one after another alone on the device (the
+    sequential baseline, end to end: set-up and steps)."""
+    from gandalf_amd import serving as S
+    reqs = S.request_stream(nreq=nreq)
+    srv = S.Server()
+    live = srv.run(reqs)
+    srv.close()
+    log(f"[ensemble] poisson stream: {live}")
+    sat_reqs = S.request_stream(nreq=nreq)
+    for q in sat_reqs:
+        q.arrival = 0.0
+    srv = S.Server()
+    sat = srv.run(sat_reqs)
+    srv.close()
+    log(f"[ensemble] all queued at once: {sat}")
+    seq = S.run_sequential(S.request_stream(nreq=nreq))
+    log(f"[ensemble] sequential: {seq}")
+    return {"workload": f"{nreq} requests, Poisson 20/s, N log-uniform 4096-262144, box/sphere/"
+                        "plummer in rotation, 20 mixed-precision steps each",
+            "value": live["value"], "unit": "particle-steps/s",
+            "step_latency_ms": live["step_latency_ms"],
+            "admission_wait_ms": live["admission_wait_ms"], "wall_s": live["wall_s"],
+            "particle_steps": live["particle_steps"],
+            "saturated": {"value": sat["value"], "wall_s": sat["wall_s"],
+                          "step_latency_ms": sat["step_latency_ms"]},
+            "sequential": {"value": seq["value"], "wall_s": seq["wall_s"],
+                           "steps_only_value": seq["steps_only_value"],
+                           "step_latency_ms": seq["step_latency_ms"]},
+            "saturated_over_sequential": sat["value"] / seq["value"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +198,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-global", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -213,6 +250,8 @@ def main():
         out["global_timesteps"] = {"value": glob["particle_steps"] / glob["wall"],
                                    "ms_per_step": 1e3 * glob["wall"] / glob["steps"],
                                    "precision": "mixed"}
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

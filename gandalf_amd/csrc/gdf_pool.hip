@@ -44,6 +44,8 @@ const char* kKeep[] = {"r", "v", "a", "m", "h", "rho", "u", "dudt", "div_v", "gp
 struct gdf_pool {
   gdf_config cfg;
   gdf_ctx* batch = nullptr;
+  gdf_ctx* scratch = nullptr;               // where requests are set up alone (kept between
+                                            // admissions: its arrays only grow)
   int64_t max_particles = 0;
   std::vector<gdf::PoolReq> res;            // resident, in segment order
   std::map<int, gdf::PoolResult> fin;       // finished
@@ -76,6 +78,9 @@ int batch_reserve(gdf_ctx* c, int need, int64_t maxp) {
   c->iorig = nullptr;
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   const int rc = ctx_alloc_particles(c, ncap);
+  if (rc == GDF_OK)        // derived fields a class never computes (density of N-body) read 0
+    for (int f = 0; f < D_NDERIVED; f++)
+      GDF_HIP(c, hipMemsetAsync(c->df[f], 0, sizeof(double)*ncap, c->stream));
   if (rc == GDF_OK && N > 0) {
     for (int f = 0; f < F_NPERSIST; f++)
       GDF_HIP(c, hipMemcpyAsync(c->pf[f], keep[f], sizeof(double)*N, hipMemcpyDeviceToDevice,
@@ -175,6 +180,7 @@ int gdf_pool_create(const gdf_config* cfg, int64_t max_particles, gdf_pool** out
 
 void gdf_pool_destroy(gdf_pool* P) {
   if (!P) return;
+  gdf_destroy(P->scratch);
   gdf_destroy(P->batch);
   delete P;
 }
@@ -190,9 +196,12 @@ int gdf_pool_admit(gdf_pool* P, int N, const double* r, const double* v, const d
   if ((int64_t) c->N + N > P->max_particles || (int) P->res.size() >= kMaxSeg)
     return pool_fail(P, GDF_ERR_CAPACITY, "pool full: the request waits for resident ones to finish");
   // set the request up alone, with the batch's key layout (one segment)
-  gdf_ctx* sc = nullptr;
-  int rc = gdf_create(&P->cfg, &sc);
-  if (rc != GDF_OK) return pool_fail(P, rc, "scratch context creation failed");
+  int rc = GDF_OK;
+  if (!P->scratch) {
+    rc = gdf_create(&P->cfg, &P->scratch);
+    if (rc != GDF_OK) { P->scratch = nullptr; return pool_fail(P, rc, "scratch context creation failed"); }
+  }
+  gdf_ctx* sc = P->scratch;
   rc = gdf_set_particles(sc, N, r, v, m, nullptr, u);
   if (rc == GDF_OK) {
     sc->nseg = 1;
@@ -200,11 +209,7 @@ int gdf_pool_admit(gdf_pool* P, int N, const double* r, const double* v, const d
     rc = seg_upload(sc);
   }
   if (rc == GDF_OK) rc = gdf_setup_simulation(sc);
-  if (rc != GDF_OK) {
-    pool_ctx_fail(P, sc, rc);
-    gdf_destroy(sc);
-    return rc;
-  }
+  if (rc != GDF_OK) return pool_ctx_fail(P, sc, rc);
   // join the batch as the last segment
   const int N0 = c->N;
   rc = batch_reserve(c, N0 + N, P->max_particles);
@@ -220,7 +225,6 @@ int gdf_pool_admit(gdf_pool* P, int N, const double* r, const double* v, const d
   q.id = P->next_id; q.N = N; q.nsteps = nsteps; q.done = 0;
   q.t = sc->t; q.timestep = sc->timestep; q.tlast = sc->tlast;
   if (rc == GDF_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = GDF_ERR_HIP;
-  gdf_destroy(sc);
   if (rc != GDF_OK) return pool_ctx_fail(P, c, rc);
   c->N = N0 + N;
   if (c->seg_off.empty()) c->seg_off.push_back(0);
