@@ -97,7 +97,7 @@ void free_particles(gdf_ctx* c) {
                   c->first, c->last, c->parent, c->visit, c->front, c->bbmin, c->bbmax, c->geo_c,
                   c->geo_h, c->mp, c->gstart, c->grp_first, c->grp_count, c->grp_node, c->nsel,
                   c->sel_tmp, c->stacks, c->grav_nc,
-                  c->grav_nn, c->tmpd, c->grp_root};
+                  c->grav_nn, c->tmpd, c->grp_root, c->stacks2};
   for (void* p : ptrs) if (p) hipFree(p);
   c->iorig = c->iorig_alt = nullptr; c->posm = c->hrecA = c->hrecB = c->grec = nullptr;
   c->status = nullptr; c->keys = c->keys_alt = nullptr; c->vals = c->vals_alt = nullptr;
@@ -107,6 +107,7 @@ void free_particles(gdf_ctx* c) {
   c->stacks = nullptr;
   sph_free_lists(c);
   c->grav_nc = c->grav_nn = nullptr; c->tmpd = nullptr; c->grp_root = nullptr;
+  c->stacks2 = nullptr;
   c->Ncap = 0; c->N = 0;
 }
 
@@ -309,15 +310,19 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
   c->nlevels = cfg->Nlevels; c->level_diff_max = cfg->level_diff_max;
   int rc = upload_kernel_tables(c);
   if (rc == GDF_OK) {
-    if (hipMalloc(&c->counters, sizeof(int)*16) != hipSuccess ||
+    if (hipMalloc(&c->counters, sizeof(int)*32) != hipSuccess ||
         hipMalloc(&c->icount, sizeof(long long)*8) != hipSuccess ||
         hipMalloc(&c->red, sizeof(double)*16384) != hipSuccess ||
-        hipHostMalloc(&c->h_counters, sizeof(int)*16) != hipSuccess ||
+        hipHostMalloc(&c->h_counters, sizeof(int)*32) != hipSuccess ||
         hipHostMalloc(&c->h_red, sizeof(double)*16384) != hipSuccess)
       rc = GDF_ERR_NOMEM;
   }
   if (rc == GDF_OK) {
-    hipMemset(c->counters, 0, sizeof(int)*16);
+    hipMemset(c->counters, 0, sizeof(int)*32);
+    if (hipStreamCreate(&c->stream2) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_dens, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_grav, hipEventDisableTiming) != hipSuccess)
+      rc = GDF_ERR_HIP;
     hipMemset(c->icount, 0, sizeof(long long)*8);
     for (int e = 0; e < 12; e++) hipEventCreate(&c->ev[e]);
   }
@@ -340,6 +345,10 @@ void gdf_destroy(gdf_ctx* c) {
   if (c->h_counters) hipHostFree(c->h_counters);
   if (c->h_red) hipHostFree(c->h_red);
   for (int e = 0; e < 12; e++) if (c->ev[e]) hipEventDestroy(c->ev[e]);
+  if (c->stream2) hipStreamSynchronize(c->stream2);
+  if (c->ev_dens) hipEventDestroy(c->ev_dens);
+  if (c->ev_grav) hipEventDestroy(c->ev_grav);
+  if (c->stream2) hipStreamDestroy(c->stream2);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -444,8 +453,10 @@ int gdf_update_all_sph_forces(gdf_ctx* c) {
   if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "forces before the density pass");
   GDF_TRY(integ_zero_forces(c));
   if (c->phys.hydro_forces) {
+    GDF_TRY(gravity_launch(c));        // beside the hydro pass, on the second stream
     GDF_TRY(sph_build_sg(c));
     GDF_TRY(sph_hydro(c, 1));
+    return gravity_join(c);
   }
   GDF_TRY(gravity_walk(c));
   return integ_add_atree(c);

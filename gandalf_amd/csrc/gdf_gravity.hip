@@ -45,6 +45,9 @@ struct GravArgs {
   int32_t* gnc; int32_t* gnn;
   const int* act;         // block timesteps: only active targets (NULL = all)
   const int* grp_root;    // serving batches: walk from the group's segment subtree (NULL = root)
+  // non-NULL: the walk runs beside the hydro pass and writes its own sums here (acceleration,
+  // potential, self term) instead of adding to the hydro pass's atx/gpot
+  double* gwx; double* gwy; double* gwz; double* gwp; double* gwp0;
   PhysParams p;
 };
 
@@ -357,6 +360,9 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
       if (SOFT) {
         a.atx[i] = ax; a.aty[i] = ay; a.atz[i] = az;
         a.gpot[i] = pot0 + pot;
+      } else if (a.gwx) {
+        a.gwx[i] = ax; a.gwy[i] = ay; a.gwz[i] = az;
+        a.gwp[i] = pot; a.gwp0[i] = pot0;
       } else {
         a.atx[i] += ax; a.aty[i] += ay; a.atz[i] += az;
         a.gpot[i] = pot0 + a.gpot[i] + pot;
@@ -460,6 +466,25 @@ __global__ void k_pack_grav(int N, const double* __restrict__ h, const double* _
 
 __global__ void k_zero8(int* p) { if (threadIdx.x < 8) p[threadIdx.x] = 0; }
 
+// the walk's sums joined to the hydro pass's in the order of the single-stream update
+// (atx += walk; gpot = self + gpot + walk), then a += atree (GradhSphTree.cpp:615-617)
+struct JoinArgs {
+  int N;
+  const int* act;
+  const double* gw[3]; const double* gwp; const double* gwp0;
+  double* at[3]; double* gpot; double* a[3];
+};
+__global__ void k_grav_join(JoinArgs j) {
+  const int i = blockIdx.x*256 + threadIdx.x;
+  if (i >= j.N || (j.act && !j.act[i])) return;
+  for (int k = 0; k < 3; k++) {
+    const double t = j.at[k][i] + j.gw[k][i];
+    j.at[k][i] = t;
+    j.a[k][i] += t;
+  }
+  j.gpot[i] = j.gwp0[i] + j.gpot[i] + j.gwp[i];
+}
+
 }  // namespace
 
 int nbody_direct(gdf_ctx* c, int smoothed) {
@@ -481,14 +506,11 @@ int nbody_direct(gdf_ctx* c, int smoothed) {
   return GDF_OK;
 }
 
-int gravity_walk(gdf_ctx* c) {
-  const int N = c->N;
-  hipStream_t s = c->stream;
-  if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "gravity pass before tree build");
-  k_pack_grav<<<nblocks(N, 256), 256, 0, s>>>(N, c->pf[F_H], c->df[D_ZETA], c->df[D_HFACTOR],
-                                                c->grec);
+namespace {
+
+GravArgs grav_args(gdf_ctx* c) {
   GravArgs a;
-  a.N = N; a.ngroups = c->ngroups; a.nleaf = c->phys.nleaf;
+  a.N = c->N; a.ngroups = c->ngroups; a.nleaf = c->phys.nleaf;
   a.grp_node = c->grp_node; a.grp_first = c->grp_first; a.grp_count = c->grp_count;
   a.first = c->first; a.last = c->last; a.left = c->left; a.right = c->right;
   a.geo_c = c->geo_c; a.geo_h = c->geo_h; a.mp = c->mp;
@@ -498,7 +520,69 @@ int gravity_walk(gdf_ctx* c) {
   a.icount = c->icount; a.gnc = c->grav_nc; a.gnn = c->grav_nn;
   a.act = c->nlevels > 1 ? c->pi[I_ACT] : nullptr;
   a.grp_root = c->nseg > 0 ? c->grp_root : nullptr;
+  a.gwx = a.gwy = a.gwz = a.gwp = a.gwp0 = nullptr;
   a.p = c->phys;
+  return a;
+}
+
+}  // namespace
+
+int gravity_launch(gdf_ctx* c) {
+  if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "gravity pass before tree build");
+  if (!c->stacks2 || !c->phys.hydro_forces || c->phys.nbody)
+    return fail(c, GDF_ERR_INVALID, "gravity walk beside the hydro pass needs hydro + self-gravity");
+  const int N = c->N;
+  hipStream_t s2 = c->stream2;
+  GDF_HIP(c, hipEventRecord(c->ev_dens, c->stream));        // h, zeta, tree, posm are ready
+  GDF_HIP(c, hipStreamWaitEvent(s2, c->ev_dens, 0));
+  k_pack_grav<<<nblocks(N, 256), 256, 0, s2>>>(N, c->pf[F_H], c->df[D_ZETA], c->df[D_HFACTOR],
+                                                 c->grec);
+  GravArgs a = grav_args(c);
+  a.stacks = c->stacks2; a.counter = c->counters + 24; a.flags = c->counters + 16;
+  a.gwx = c->df[D_GWX]; a.gwy = c->df[D_GWY]; a.gwz = c->df[D_GWZ];
+  a.gwp = c->df[D_GWP]; a.gwp0 = c->df[D_GWP0];
+  k_zero8<<<1, 64, 0, s2>>>(c->counters + 24);
+  k_zero8<<<1, 64, 0, s2>>>(c->counters + 16);
+  const int grid = std::min(c->ngroups, kWalkSlots);
+  if (c->timing) hipEventRecord(c->ev[7], s2);
+  if (c->cfg.precision == GDF_PREC_MIXED) k_gravity<true, false, false><<<grid, kWave, 0, s2>>>(a);
+  else k_gravity<false, false, false><<<grid, kWave, 0, s2>>>(a);
+  GDF_HIP(c, hipGetLastError());
+  if (c->timing) hipEventRecord(c->ev[8], s2);
+  GDF_HIP(c, hipMemcpyAsync(c->h_counters + 16, c->counters + 16, sizeof(int)*8,
+                            hipMemcpyDeviceToHost, s2));
+  GDF_HIP(c, hipEventRecord(c->ev_grav, s2));
+  return GDF_OK;
+}
+
+int gravity_join(gdf_ctx* c) {
+  GDF_HIP(c, hipStreamWaitEvent(c->stream, c->ev_grav, 0));
+  JoinArgs j;
+  j.N = c->N; j.act = c->nlevels > 1 ? c->pi[I_ACT] : nullptr;
+  for (int k = 0; k < 3; k++) {
+    j.gw[k] = c->df[D_GWX + k]; j.at[k] = c->df[D_ATX + k]; j.a[k] = c->df[D_AX + k];
+  }
+  j.gwp = c->df[D_GWP]; j.gwp0 = c->df[D_GWP0]; j.gpot = c->df[D_GPOT];
+  k_grav_join<<<nblocks(c->N, 256), 256, 0, c->stream>>>(j);
+  GDF_HIP(c, hipGetLastError());
+  GDF_HIP(c, hipEventSynchronize(c->ev_grav));
+  if (c->timing) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev[7], c->ev[8]);
+    c->pass_ms[5] += ms;
+    c->grav_launches++;
+  }
+  if (c->h_counters[16 + 2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in gravity pass");
+  return GDF_OK;
+}
+
+int gravity_walk(gdf_ctx* c) {
+  const int N = c->N;
+  hipStream_t s = c->stream;
+  if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "gravity pass before tree build");
+  k_pack_grav<<<nblocks(N, 256), 256, 0, s>>>(N, c->pf[F_H], c->df[D_ZETA], c->df[D_HFACTOR],
+                                                c->grec);
+  GravArgs a = grav_args(c);
   k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
   k_zero8<<<1, 64, 0, s>>>(c->counters);
   const int grid = std::min(c->ngroups, kWalkSlots);

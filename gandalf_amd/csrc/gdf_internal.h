@@ -38,6 +38,9 @@ constexpr int kMaxSeg = 128;
 enum DField {
   D_AX, D_AY, D_AZ, D_ATX, D_ATY, D_ATZ, D_RHO, D_DUDT, D_DIVV, D_GPOT, D_INVOMEGA,
   D_ZETA, D_SOUND, D_PRESSURE, D_HFACTOR, D_HSEARCH, D_DT, D_ADOTX, D_ADOTY, D_ADOTZ,
+  // the gravity walk's own sums when it runs beside the hydro pass: acceleration, potential,
+  // and the self term (m/h) wpot(0)
+  D_GWX, D_GWY, D_GWZ, D_GWP, D_GWP0,
   D_NDERIVED
 };
 
@@ -133,7 +136,12 @@ struct gdf_ctx {
 
   // walk scratch
   int* stacks = nullptr;
-  int* counters = nullptr;     // [0..7] device counters/flags
+  int* stacks2 = nullptr;      // the gravity walk's stacks when it runs beside the hydro pass
+  int* counters = nullptr;     // [0..7] flags, [4..11] walk range counters, [12..13] block
+                               // reductions; [16..23] flags and [24..31] range counters of the
+                               // gravity walk on the second stream
+  hipStream_t stream2 = nullptr;             // gravity walk beside the hydro pass
+  hipEvent_t ev_dens = nullptr, ev_grav = nullptr;
   int* h_counters = nullptr;   // pinned mirror
   long long* icount = nullptr; // [8] interaction counters
   int32_t* grav_nc = nullptr; int32_t* grav_nn = nullptr;
@@ -202,6 +210,10 @@ int sph_hydro(gdf_ctx* c, int variant);
 
 // gravity (gdf_gravity.hip)
 int gravity_walk(gdf_ctx* c);
+// hydro + self-gravity: the walk on the second stream (after the density pass), then joined:
+// atree += walk, gpot = (self + gpot) + walk, a += atree
+int gravity_launch(gdf_ctx* c);
+int gravity_join(gdf_ctx* c);
 
 // integration (gdf_integrate.hip)
 int integ_advance(gdf_ctx* c, double dt);

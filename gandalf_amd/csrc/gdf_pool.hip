@@ -137,8 +137,10 @@ int batch_forces(gdf_ctx* c) {
   GDF_TRY(integ_zero_forces(c));
   if (c->phys.self_gravity) {
     if (c->phys.hydro_forces) {
+      GDF_TRY(gravity_launch(c));          // beside the hydro pass, on the second stream
       GDF_TRY(sph_build_sg(c));
       GDF_TRY(sph_hydro(c, 1));
+      return gravity_join(c);
     }
     GDF_TRY(gravity_walk(c));
     return integ_add_atree(c);

@@ -710,6 +710,8 @@ int sph_alloc_lists(gdf_ctx* c) {
   GDF_HIP(c, hipMalloc(&c->hrecA, sizeof(double4)*(size_t) c->Ncap));
   GDF_HIP(c, hipMalloc(&c->hrecB, sizeof(double4)*(size_t) c->Ncap));
   GDF_HIP(c, hipMalloc(&c->stacks, sizeof(int)*(size_t) kWalkSlots*kStackCap));
+  if (c->phys.self_gravity && c->phys.hydro_forces && !c->phys.nbody)
+    GDF_HIP(c, hipMalloc(&c->stacks2, sizeof(int)*(size_t) kWalkSlots*kStackCap));
   return GDF_OK;
 }
 

@@ -60,8 +60,13 @@ template <int ndim>
 int GradhSphTreeGpu<ndim>::UpdateAllSphForces() {
   GDF_TRY(zero_accelerations(ctx));
   if (ctx->phys.hydro_forces) {
+    // the gravity walk needs only h and zeta of the density pass: it runs on the second stream
+    // while the scatter-gather walk and the hydro pass run on the first
+    GDF_TRY(gdf::gravity_launch(ctx));
     GDF_TRY(gdf::sph_build_sg(ctx));
     GDF_TRY(gdf::sph_hydro(ctx, 1));
+    record(ctx, 3);
+    return gdf::gravity_join(ctx);
   }
   record(ctx, 3);
   GDF_TRY(gdf::gravity_walk(ctx));
