@@ -52,11 +52,12 @@ def run_path(ic, precision, steps, warmup):
         s.step(1)
     s.set_timing(True)
     passes = {k: 0.0 for k in ("tree", "density", "hydro", "gravity", "integrate",
-                                "k_gravity")}
+                                "k_gravity", "k_hydro")}
     counts = None
     per_step = []
     finished = []
     launches = 0
+    hlaunches = 0
     t_start = time.perf_counter()
     for k in range(steps):
         ts = time.perf_counter()
@@ -66,6 +67,7 @@ def run_path(ic, precision, steps, warmup):
         for key in passes:
             passes[key] += pt[key]
         launches += int(pt["k_gravity_launches"])
+        hlaunches += int(pt["k_hydro_launches"])
         finished.append(s.block_state()[5])
         c = s.interaction_counts()
         counts = c if counts is None else {k2: counts[k2] + c[k2] for k2 in c}
@@ -76,43 +78,52 @@ def run_path(ic, precision, steps, warmup):
     N = ic["r"].shape[0]
     return dict(N=N, wall=wall, steps=steps, per_step=per_step, passes=passes, counts=counts,
                 t=t, dt=dt, E=float(diag[0]), precision=precision, finished=finished,
-                launches=launches, particle_steps=int(sum(finished)))
+                launches=launches, hlaunches=hlaunches, particle_steps=int(sum(finished)))
 
 
-def roofline(res, dtype):
-    """The gravity walk kernel k_gravity dominates the step: its algorithmic FLOPs over its own
-    event-timed duration on the stream it runs on.  Algorithmic work: 22 FLOP per direct term
-    (walk particle pairs minus the scatter-gather pairs, whose softened gravity the hydro pass
-    sums) and 70 per quadrupole particle-cell term, both counted on the device.  The whole-loop
-    fraction (density 45, hydro with neighbour gravity 135, plus the walk) over the step-loop
-    wall time is reported beside it."""
-    c, p, steps = res["counts"], res["passes"], res["steps"]
-    nl = max(res["launches"], 1)
-    direct = c["grav_pp"] - c["sg_pairs"]
-    flops = W_PP * direct + W_PC * c["grav_pc"]
-    ms = p["k_gravity"]
-    achieved = flops / (ms * 1e-3) if ms > 0 else 0.0
+def kernel_rooflines(res, dtype):
+    """The two pair kernels that dominate the step, each as algorithmic FLOPs over its own
+    event-timed duration on the stream it runs on, summed over the timed steps' launches:
+    k_hydro (135 FLOP per scatter-gather pair: hydro terms plus the kernel-softened gravity of
+    the pair, counted on the device) and k_gravity (22 FLOP per direct term -- walk particle
+    pairs minus the scatter-gather pairs -- and 70 per quadrupole particle-cell term).  The
+    gravity walk runs on a second stream beside the scatter-gather walk and the hydro kernel,
+    so both durations include the other's share of the device.  Returns (dominant, other)."""
+    c, p = res["counts"], res["passes"]
     peak = PEAK[dtype]
-    loop_flops = (W_DENS * c["gather_pairs"] + W_HYDRO_GRAV * c["sg_pairs"] + flops)
+    out = {}
+    for name, flops, ms, nl in (
+            ("k_hydro", W_HYDRO_GRAV * c["sg_pairs"], p["k_hydro"], res["hlaunches"]),
+            ("k_gravity", W_PP * (c["grav_pp"] - c["sg_pairs"]) + W_PC * c["grav_pc"],
+             p["k_gravity"], res["launches"])):
+        nl = max(nl, 1)
+        achieved = flops / (ms * 1e-3) if ms > 0 else 0.0
+        out[name] = {"kernel": name, "bound": "valu", "achieved": achieved / 1e12,
+                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": measured_traffic(name) if dtype == "fp32" else None,
+                     "flop_per_launch": flops / nl, "launch_ms": ms / nl, "launches": nl,
+                     "total_ms": ms}
+    dom = max(out.values(), key=lambda r: r["total_ms"])
+    other = [r for r in out.values() if r is not dom][0]
+    loop_flops = (W_DENS * c["gather_pairs"] + W_HYDRO_GRAV * c["sg_pairs"] +
+                  W_PP * (c["grav_pp"] - c["sg_pairs"]) + W_PC * c["grav_pc"])
     loop_rate = loop_flops / res["wall"]
-    return {"kernel": "k_gravity", "bound": "valu", "achieved": achieved / 1e12,
-            "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": measured_traffic() if dtype == "fp32" else None,
-            "flop_per_launch": flops / nl,
-            "launch_ms": ms / nl, "launches": nl,
-            "loop": {"achieved": loop_rate / 1e12, "frac": loop_rate / peak,
-                     "flop_per_step": loop_flops / steps}}
+    dom = dict(dom)
+    dom["loop"] = {"achieved": loop_rate / 1e12, "frac": loop_rate / peak,
+                   "flop_per_step": loop_flops / res["steps"]}
+    return dom, other
 
 
-def measured_traffic():
-    """HBM bytes per k_gravity launch (mixed path) from the committed rocprofv3 --pmc
-    FETCH_SIZE / WRITE_SIZE runs, scaled by the same counters on a 1 GiB copy; None if the
-    summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_gravity_traffic.json")
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel (mixed path, timed steps of this benchmark) from the
+    committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, scaled by the same counters on a
+    1 GiB copy; None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return float(json.load(f)["bytes_per_launch"])
+        d = json.load(f)
+    return float(d[kernel]["bytes_per_launch"]) if kernel in d else None
 
 
 def copy_calibration(mib):
@@ -237,14 +248,16 @@ def main():
                                f"{NLEVELS} levels",
                    "n_particles": N, "precision": "fp32 pair arithmetic + fp64 integration",
                    "particle_steps_per_step": mixed["finished"]},
-        "roofline": roofline(mixed, "fp32"),
+        "roofline": kernel_rooflines(mixed, "fp32")[0],
+        "roofline_other_kernel": kernel_rooflines(mixed, "fp32")[1],
         "passes_ms_per_step": {k: v / args.steps for k, v in mixed["passes"].items()},
         "interactions_per_step": {k: v // args.steps for k, v in mixed["counts"].items()},
     }
     if fp64 is not None:
         out["fp64"] = {"value": fp64["particle_steps"] / fp64["wall"],
                        "ms_per_step": 1e3 * fp64["wall"] / args.steps,
-                       "roofline": roofline(fp64, "fp64"),
+                       "roofline": kernel_rooflines(fp64, "fp64")[0],
+                       "roofline_other_kernel": kernel_rooflines(fp64, "fp64")[1],
                        "passes_ms_per_step": {k: v / args.steps for k, v in fp64["passes"].items()}}
     if glob is not None:
         out["global_timesteps"] = {"value": glob["particle_steps"] / glob["wall"],

@@ -525,10 +525,12 @@ int gdf_gravity_group_counts(gdf_ctx* c, int32_t* ncell, int32_t* nnear) {
   return GDF_OK;
 }
 
-int gdf_pass_times(gdf_ctx* c, double* out7) {
-  if (!c || !out7) return GDF_ERR_INVALID;
-  for (int k = 0; k < 6; k++) out7[k] = c->pass_ms[k];
-  out7[6] = (double) c->grav_launches;
+int gdf_pass_times(gdf_ctx* c, double* out9) {
+  if (!c || !out9) return GDF_ERR_INVALID;
+  for (int k = 0; k < 6; k++) out9[k] = c->pass_ms[k];
+  out9[6] = (double) c->grav_launches;
+  out9[7] = c->pass_ms[6];
+  out9[8] = (double) c->hydro_launches;
   return GDF_OK;
 }
 

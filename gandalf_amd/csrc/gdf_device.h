@@ -198,6 +198,61 @@ template <typename T> __device__ __forceinline__ T kwpot(const KernelDesc& k, T 
   }
   return k.type == 0 ? m4_wpot<T>(s) : q_wpot<T>(s);
 }
+// ---- branch-free gravity kernels for the FP32 pair sums ----
+// With W written as truncated powers, W = norm sum_k c_k (a_k - s)^n_+ (M4: n = 3, a = 2, 1,
+// c = 1, -4; quintic: n = 5, a = 3, 2, 1, c = 1, -6, 15), the enclosed mass M(s) and the
+// outer potential integral P(s) = 4 pi int_s^inf u W du are polynomials in x_k = (a_k - s)_+:
+//   M = 1 - sum_k c_k x_k^(n+1) (a_k^2/(n+1) - 2 a_k x_k/(n+2) + x_k^2/(n+3)) * C,
+//   P = sum_k c_k x_k^(n+1) (a_k/(n+1) - x_k/(n+2)) * C      (C = 1 for M4, 1/30 quintic),
+// and wgrav = M/s^2, wpot = M/s + P, wzeta = wpot - s wgrav = P.  These are the reference's
+// piecewise polynomials (SmoothingKernel.h:132-408) summed without branches; the reference's
+// quintic wgrav and wpot carry 360/359 inside the kernel (its 12/359 in place of 1/30) and its
+// quintic wzeta is -30 P, both kept.  Below s = 1 the mass comes from the inner polynomial,
+// which avoids the cancellation of 1 - (...) at small s.  Agreement with the reference's
+// polynomials: ~1e-8 relative (wgrav), ~1e-12 (wpot), within the reference's own constants
+// (wzeta) -- far below FP32 rounding; the FP64 path keeps the piecewise forms.
+__device__ __forceinline__ void kgravpot_f(const KernelDesc& k, float s, float inv, float& wg,
+                                           float& wp) {
+  if (k.tabulated) { wg = kwgrav<float>(k, s); wp = kwpot<float>(k, s); return; }
+  const float x1 = fmaxf(1.f - s, 0.f), x2 = fmaxf(2.f - s, 0.f);
+  const float x1s = x1*x1, x2s = x2*x2, s2 = s*s;
+  if (k.type == 0) {
+    const float x1q = x1s*x1s, x2q = x2s*x2s;
+    const float D = fmaf(-4.f*x1q, fmaf(x1, fmaf(x1, 1.f/6.f, -0.4f), 0.25f),
+                         x2q*fmaf(x2, fmaf(x2, 1.f/6.f, -0.8f), 1.f));
+    const float P = fmaf(-4.f*x1q, fmaf(x1, -0.2f, 0.25f), x2q*fmaf(x2, -0.2f, 0.5f));
+    const float Min = s2*s*fmaf(s2, fmaf(s, 0.5f, -1.2f), 4.f/3.f);
+    const float M = s < 1.f ? Min : 1.f - D;
+    wg = M*inv*inv;
+    wp = fmaf(M, inv, P);
+    return;
+  }
+  const float x3 = fmaxf(3.f - s, 0.f), x3s = x3*x3;
+  const float x1h = x1s*x1s*x1s, x2h = x2s*x2s*x2s, x3h = x3s*x3s*x3s;
+  const float D = fmaf(15.f*x1h, fmaf(x1, fmaf(x1, 0.125f, -2.f/7.f), 1.f/6.f),
+                  fmaf(-6.f*x2h, fmaf(x2, fmaf(x2, 0.125f, -4.f/7.f), 2.f/3.f),
+                       x3h*fmaf(x3, fmaf(x3, 0.125f, -6.f/7.f), 1.5f)));
+  const float P30 = fmaf(15.f*x1h, fmaf(x1, -1.f/7.f, 1.f/6.f),
+                    fmaf(-6.f*x2h, fmaf(x2, -1.f/7.f, 1.f/3.f), x3h*fmaf(x3, -1.f/7.f, 0.5f)));
+  const float Min = s2*s*fmaf(s2, fmaf(s2, fmaf(s, -1.25f, 30.f/7.f), -12.f), 22.f)*(1.f/30.f);
+  const float M = s < 1.f ? Min : fmaf(D, -1.f/30.f, 1.f);
+  const float f = s < 3.f ? 360.f/359.f : 1.f;
+  wg = f*M*inv*inv;
+  wp = f*fmaf(M, inv, P30*(1.f/30.f));
+}
+
+// wzeta of the analytic kernels in FP32 without branches: P(s) for M4, -30 P(s) for quintic
+__device__ __forceinline__ float kwzeta_f(int type, float s) {
+  const float x1 = fmaxf(1.f - s, 0.f), x2 = fmaxf(2.f - s, 0.f);
+  const float x1s = x1*x1, x2s = x2*x2;
+  if (type == 0)
+    return fmaf(-4.f*x1s*x1s, fmaf(x1, -0.2f, 0.25f), x2s*x2s*fmaf(x2, -0.2f, 0.5f));
+  const float x3 = fmaxf(3.f - s, 0.f), x3s = x3*x3;
+  return -fmaf(15.f*x1s*x1s*x1s, fmaf(x1, -1.f/7.f, 1.f/6.f),
+               fmaf(-6.f*x2s*x2s*x2s, fmaf(x2, -1.f/7.f, 1.f/3.f),
+                    x3s*x3s*x3s*fmaf(x3, -1.f/7.f, 0.5f)));
+}
+
 // density triple w0, womega, wzeta at s^2 (GradhSph.cpp:197-204)
 template <typename T>
 __device__ __forceinline__ void kdens(const KernelDesc& k, T s2, T& w0, T& wom, T& wz) {
@@ -212,7 +267,8 @@ __device__ __forceinline__ void kdens(const KernelDesc& k, T s2, T& w0, T& wom, 
   T w1;
   kw01_an<T>(k.type, T(k.norm), s, w0, w1);
   wom = T(-3)*w0 - s*w1;
-  wz = k.type == 0 ? m4_wzeta<T>(s) : q_wzeta<T>(s);
+  if (sizeof(T) == 4) wz = (T) kwzeta_f(k.type, (float) s);
+  else wz = k.type == 0 ? m4_wzeta<T>(s) : q_wzeta<T>(s);
 }
 
 // ---- periodic nearest image (GhostNeighbours.hpp:119-138) ----

@@ -173,10 +173,11 @@ struct gdf_ctx {
 
   // timing
   bool timing = false;
-  hipEvent_t ev[12] = {};
-  double pass_ms[6] = {};
+  hipEvent_t ev[12] = {};      // [9], [10] bracket the hydro pair kernel
+  double pass_ms[7] = {};       // ... [5] gravity walk kernel, [6] hydro pair kernel
   bool mark_passes = true;
-  int grav_launches = 0;       // gravity walk launches in the current step     // record the pass-boundary events (first force pass of a step)
+  int grav_launches = 0;       // gravity walk launches in the current step
+  int hydro_launches = 0;      // hydro pair-kernel launches in the current step     // record the pass-boundary events (first force pass of a step)
   long long icounts_last[6] = {};
 };
 

@@ -605,10 +605,13 @@ __global__ __launch_bounds__(64) void k_hydro_mixed(HydroArgs a) {
         }
         bax += mj*dx*paux; bay += mj*dy*paux; baz += mj*dz*paux;
         if (VAR == 1) {
-          const float pg = 0.5f*(invhsqd_i*kwgrav<float>(p.kern, si) + zeta_i*wkerni +
-                                 invh_j*invh_j*kwgrav<float>(p.kern, sj) + fZ[q]*wkernj);
+          float wgi, wpi, wgj, wpj;
+          kgravpot_f(p.kern, si, rcp_f(si), wgi, wpi);
+          kgravpot_f(p.kern, sj, rcp_f(sj), wgj, wpj);
+          const float pg = 0.5f*(invhsqd_i*wgi + zeta_i*wkerni + invh_j*invh_j*wgj +
+                                 fZ[q]*wkernj);
           bgx += mj*dx*pg; bgy += mj*dy*pg; bgz += mj*dz*pg;
-          bgp += 0.5f*mj*(invh_i*kwpot<float>(p.kern, si) + invh_j*kwpot<float>(p.kern, sj));
+          bgp += 0.5f*mj*(invh_i*wpi + invh_j*wpj);
         }
       }
       ax.add(bax); ay.add(bay); az.add(baz); du.add(bdu); dv.add(bdv);
@@ -795,6 +798,7 @@ int sph_hydro(gdf_ctx* c, int variant) {
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
   const bool blk = c->nlevels > 1;
   const int G = c->ngroups;
+  if (c->timing) hipEventRecord(c->ev[9], s);
   if (variant == 0) {
     if (blk) {
       if (mixed) k_hydro_mixed<0, true><<<G, kWave, 0, s>>>(h);
@@ -813,6 +817,14 @@ int sph_hydro(gdf_ctx* c, int variant) {
     }
   }
   GDF_HIP(c, hipGetLastError());
+  if (c->timing) {
+    hipEventRecord(c->ev[10], s);
+    hipEventSynchronize(c->ev[10]);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev[9], c->ev[10]);
+    c->pass_ms[6] += ms;
+    c->hydro_launches++;
+  }
   return GDF_OK;
 }
 

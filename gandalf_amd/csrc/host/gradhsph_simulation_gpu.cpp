@@ -158,6 +158,8 @@ int GradhSphSimulationGpu<ndim>::MainLoop() {
   record(c, 0);
   c->pass_ms[5] = 0.0;
   c->grav_launches = 0;
+  c->pass_ms[6] = 0.0;
+  c->hydro_launches = 0;
   c->nsteps++;
   const double dtstep = c->timestep;
   c->t = c->t + c->timestep;
@@ -209,6 +211,8 @@ int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
   record(c, 0);
   c->pass_ms[5] = 0.0;
   c->grav_launches = 0;
+  c->pass_ms[6] = 0.0;
+  c->hydro_launches = 0;
   c->n_int += 1;
   c->nsteps++;
   c->t = c->t + c->timestep;
@@ -284,6 +288,8 @@ int NbodySimulationGpu<ndim>::MainLoop() {
   record(c, 0);
   c->pass_ms[5] = 0.0;
   c->grav_launches = 0;
+  c->pass_ms[6] = 0.0;
+  c->hydro_launches = 0;
   c->nsteps++;
   const double dtstep = c->timestep;
   c->t = c->t + c->timestep;

@@ -251,10 +251,10 @@ class GpuSim:
         self._chk(lib().gdf_set_timing(self._h, int(on)))
 
     def pass_times(self):
-        out = np.zeros(7)
+        out = np.zeros(9)
         self._chk(lib().gdf_pass_times(self._h, _p(out)))
         return dict(zip(("tree", "density", "hydro", "gravity", "integrate", "k_gravity",
-                         "k_gravity_launches"), out))
+                         "k_gravity_launches", "k_hydro", "k_hydro_launches"), out))
 
     def interaction_counts(self):
         out = np.zeros(6, dtype=np.int64)

@@ -185,9 +185,10 @@ int64_t gdf_pool_resident_particles(const gdf_pool* pool);
 
 /* event timing of the named passes of the last step, milliseconds:
  * {tree, density, hydro, gravity, integrate, gravity walk kernel alone (summed over its
- * launches)}, then the number of gravity walk launches in the step (block timesteps repeat the
- * force pass for particles whose step CheckTimesteps ends early) */
-int gdf_pass_times(gdf_ctx* ctx, double* out7);
+ * launches), number of gravity walk launches in the step (block timesteps repeat the force pass
+ * for particles whose step CheckTimesteps ends early), hydro pair kernel alone (summed), number
+ * of hydro pair-kernel launches} */
+int gdf_pass_times(gdf_ctx* ctx, double* out9);
 int gdf_set_timing(gdf_ctx* ctx, int enable);
 
 #ifdef __cplusplus
