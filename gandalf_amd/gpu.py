@@ -71,6 +71,9 @@ PROTOTYPES = {
     "gdf_main_loop": (_i, [_vp, _i]),
     "gdf_time": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)]),
     "gdf_block_state": (_i, [_vp, _vp, _vp]),
+    "gdf_write_snapshot": (_i, [_vp, C.c_char_p, C.c_char_p]),
+    "gdf_read_snapshot": (_i, [_vp, C.c_char_p, C.c_char_p]),
+    "gdf_format_column": (_i, [C.c_char_p, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gdf_pool_create": (_i, [C.POINTER(Config), _i64, C.POINTER(_vp)]),
     "gdf_pool_destroy": (None, [_vp]),
     "gdf_pool_last_error": (C.c_char_p, [_vp]),
@@ -239,6 +242,13 @@ class GpuSim:
         t, dt, n = C.c_double(), C.c_double(), C.c_int64()
         self._chk(lib().gdf_time(self._h, C.byref(t), C.byref(dt), C.byref(n)))
         return t.value, dt.value, n.value
+
+    def write_snapshot(self, path, fmt="column"):
+        self._chk(lib().gdf_write_snapshot(self._h, str(path).encode(), fmt.encode()))
+
+    def read_snapshot(self, path, fmt="column"):
+        self._chk(lib().gdf_read_snapshot(self._h, str(path).encode(), fmt.encode()))
+        self.N = lib().gdf_num_particles(self._h)
 
     def block_state(self):
         """(n, level_max, level_step, nresync, dt_max, particles that ended a step last)"""

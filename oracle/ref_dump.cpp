@@ -115,6 +115,8 @@ int main(int argc, char** argv)
       char name[64];
       snprintf(name, sizeof(name), "_s%d.bin", s);
       dump(sim, prefix + name);
+      snprintf(name, sizeof(name), "_s%d.col", s);
+      sim->WriteColumnSnapshotFile(prefix + name);
     }
   }
   fclose(dfp);
