@@ -442,12 +442,16 @@ __global__ void k_block_init(BlkArgs b) {
 __global__ void k_block_advance(BlkArgs b) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x*BS + threadIdx.x;
-  if (i >= b.N) return;
-  const double dt = b.t - b.tlast[i];
-  for (int k = 0; k < 3; k++) b.r[k][i] = b.r0[k][i] + b.v0[k][i]*dt + 0.5*b.a0[k][i]*dt*dt;
-  for (int k = 0; k < 3; k++) b.v[k][i] = b.v0[k][i] + b.a0[k][i]*dt;
-  if (b.energy) b.u[i] = b.u0[i] + b.dudt0[i]*dt;
-  b.act[i] = (b.n - b.nlast[i] == b.nstep[i]) ? 1 : 0;
+  long long a = 0;
+  if (i < b.N) {
+    const double dt = b.t - b.tlast[i];
+    for (int k = 0; k < 3; k++) b.r[k][i] = b.r0[k][i] + b.v0[k][i]*dt + 0.5*b.a0[k][i]*dt*dt;
+    for (int k = 0; k < 3; k++) b.v[k][i] = b.v0[k][i] + b.a0[k][i]*dt;
+    if (b.energy) b.u[i] = b.u0[i] + b.dudt0[i]*dt;
+    a = (b.n - b.nlast[i] == b.nstep[i]) ? 1 : 0;
+    b.act[i] = (int) a;
+  }
+  wave_add_ll(reinterpret_cast<long long*>(b.counter), a);   // active particles of the step
 }
 
 // Sph::ZeroAccelerations (Sph.cpp:126-140) for the active particles
@@ -642,6 +646,7 @@ int zero_counter(gdf_ctx* c) {
 
 int block_init(gdf_ctx* c) {
   c->n_int = 0; c->nresync = 0; c->level_max = 0; c->level_step = 0; c->dt_max = 0.0;
+  c->n_active = c->N;                  // the setup passes compute every particle
   BlkArgs b = blk_args(c);
   k_block_init<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
   GDF_HIP(c, hipGetLastError());
@@ -649,9 +654,14 @@ int block_init(gdf_ctx* c) {
 }
 
 int block_advance(gdf_ctx* c) {
+  GDF_TRY(zero_counter(c));
   BlkArgs b = blk_args(c);
   k_block_advance<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
   GDF_HIP(c, hipGetLastError());
+  // read with the tree build's synchronisation (n_active is used after it)
+  GDF_HIP(c, hipMemcpyAsync(c->h_counters + 12, c->counters + 12, sizeof(int)*2,
+                            hipMemcpyDeviceToHost, c->stream));
+  c->n_active_pending = true;
   c->tree_valid = false;
   return GDF_OK;
 }

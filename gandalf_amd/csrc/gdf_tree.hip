@@ -164,15 +164,21 @@ __global__ void k_permute(int N, const int* __restrict__ vals, PermArgs a,
   io_alt[s] = io[o];
 }
 
-// block timesteps: the derived fields persist for the particles that are not recomputed
-struct PermDArgs { const double* src[D_NDERIVED]; double* dst[D_NDERIVED]; };
+// block timesteps: the derived fields an inactive particle keeps until its step ends (its
+// neighbours read rho, P, Omega, zeta, c_s, hfactor; diagnostics read gpot; a, atree, du/dt and
+// div v are its visible state); the search radius, the timestep criterion, the N-body jerk and
+// the gravity walk's partial sums are rewritten before every use and are not moved
+constexpr int kPermD[] = {D_AX, D_AY, D_AZ, D_ATX, D_ATY, D_ATZ, D_RHO, D_DUDT, D_DIVV, D_GPOT,
+                          D_INVOMEGA, D_ZETA, D_SOUND, D_PRESSURE, D_HFACTOR};
+constexpr int kNPermD = sizeof(kPermD)/sizeof(kPermD[0]);
+struct PermDArgs { const double* src[kNPermD]; double* dst[kNPermD]; };
 
 __global__ void k_permute_d(int N, const int* __restrict__ vals, PermDArgs a) {
   const int s = blockIdx.x*BS + threadIdx.x;
   if (s >= N) return;
   const int o = vals[s];
   #pragma unroll
-  for (int f = 0; f < D_NDERIVED; f++) a.dst[f][s] = a.src[f][o];
+  for (int f = 0; f < kNPermD; f++) a.dst[f][s] = a.src[f][o];
 }
 
 __device__ __forceinline__ int delta(const uint64_t* __restrict__ key, int N, int i, long long j) {
@@ -621,9 +627,9 @@ int tree_build(gdf_ctx* c) {
   std::swap(c->iorig, c->iorig_alt);
   if (c->df_alt[0]) {
     PermDArgs pd;
-    for (int f = 0; f < D_NDERIVED; f++) { pd.src[f] = c->df[f]; pd.dst[f] = c->df_alt[f]; }
+    for (int k = 0; k < kNPermD; k++) { pd.src[k] = c->df[kPermD[k]]; pd.dst[k] = c->df_alt[kPermD[k]]; }
     k_permute_d<<<nblocks(N, BS), BS, 0, s>>>(N, c->vals, pd);
-    for (int f = 0; f < D_NDERIVED; f++) std::swap(c->df[f], c->df_alt[f]);
+    for (int k = 0; k < kNPermD; k++) std::swap(c->df[kPermD[k]], c->df_alt[kPermD[k]]);
   }
   // 4. topology
   if (N > 1)

@@ -221,6 +221,9 @@ int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
   record(c, 1);
   GDF_TRY(sphneib.BuildTree(true, (int) c->nsteps, 1, 1, c->timestep));
   record(c, 2);
+  // the tree build synchronised the stream: block_advance's count of active particles is in
+  c->n_active = c->h_counters[12];
+  c->n_active_pending = false;
   int activecount = 0, rounds = 0;
   do {
     c->mark_passes = rounds == 0;
@@ -232,6 +235,7 @@ int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
     // SphLeapfrogKDK::CheckTimesteps: the active flags are rewritten, set only for the
     // particles whose step a neighbour's level ends early
     GDF_TRY(gdf::block_check_timesteps(c, &activecount));
+    c->n_active = activecount;
     if (++rounds > 64) return gdf::fail(c, GDF_ERR_TIMESTEP, "CheckTimesteps did not settle");
   } while (activecount > 0);
   GDF_TRY(gdf::block_timesteps(c));
