@@ -658,10 +658,9 @@ int block_advance(gdf_ctx* c) {
   BlkArgs b = blk_args(c);
   k_block_advance<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
   GDF_HIP(c, hipGetLastError());
-  // read with the tree build's synchronisation (n_active is used after it)
+  // read with the tree build's synchronisation
   GDF_HIP(c, hipMemcpyAsync(c->h_counters + 12, c->counters + 12, sizeof(int)*2,
                             hipMemcpyDeviceToHost, c->stream));
-  c->n_active_pending = true;
   c->tree_valid = false;
   return GDF_OK;
 }

@@ -34,8 +34,7 @@ enum IField { I_LEVEL, I_LEVELNEIB, I_NSTEP, I_NLAST, I_ACT, I_END, I_SEG, I_NIN
 // particle segment; the segment id prefixes the Morton key (7 bits + 19 bits per axis), so the
 // sort keeps segments apart and each segment is one subtree of the LBVH.
 constexpr int kMaxSeg = 128;
-// force passes with fewer particles than this run the gravity walk beside the hydro pass
-constexpr long long kOverlapMaxActive = 1LL << 18;
+
 // Derived per-particle fields, recomputed every step after the permutation.
 enum DField {
   D_AX, D_AY, D_AZ, D_ATX, D_ATY, D_ATZ, D_RHO, D_DUDT, D_DIVV, D_GPOT, D_INVOMEGA,
@@ -179,11 +178,7 @@ struct gdf_ctx {
   double pass_ms[7] = {};       // ... [5] gravity walk kernel, [6] hydro pair kernel
   bool mark_passes = true;
   int grav_launches = 0;       // gravity walk launches in the current step
-  // particles whose forces the current force pass computes (all, or the active ones of a block
-  // step); small counts leave the device idle enough to run the gravity walk beside the hydro
-  // pass on the second stream
-  long long n_active = 0;
-  bool n_active_pending = false;   // block_advance's count is still in flight
+  long long n_active = 0;      // particles the current force pass computes (block steps: active)
   int hydro_launches = 0;      // hydro pair-kernel launches in the current step     // record the pass-boundary events (first force pass of a step)
   long long icounts_last[6] = {};
 };

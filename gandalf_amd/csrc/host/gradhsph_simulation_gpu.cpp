@@ -223,7 +223,6 @@ int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
   record(c, 2);
   // the tree build synchronised the stream: block_advance's count of active particles is in
   c->n_active = c->h_counters[12];
-  c->n_active_pending = false;
   int activecount = 0, rounds = 0;
   do {
     c->mark_passes = rounds == 0;
