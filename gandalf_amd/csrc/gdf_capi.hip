@@ -97,7 +97,7 @@ void free_particles(gdf_ctx* c) {
                   c->first, c->last, c->parent, c->visit, c->front, c->bbmin, c->bbmax, c->geo_c,
                   c->geo_h, c->mp, c->gstart, c->grp_first, c->grp_count, c->grp_node, c->nsel,
                   c->sel_tmp, c->stacks, c->grav_nc,
-                  c->grav_nn, c->tmpd, c->grp_root, c->stacks2};
+                  c->grav_nn, c->tmpd, c->grp_root, c->stacks2, c->glist};
   for (void* p : ptrs) if (p) hipFree(p);
   c->iorig = c->iorig_alt = nullptr; c->posm = c->hrecA = c->hrecB = c->grec = nullptr;
   c->status = nullptr; c->keys = c->keys_alt = nullptr; c->vals = c->vals_alt = nullptr;
@@ -107,7 +107,7 @@ void free_particles(gdf_ctx* c) {
   c->stacks = nullptr;
   sph_free_lists(c);
   c->grav_nc = c->grav_nn = nullptr; c->tmpd = nullptr; c->grp_root = nullptr;
-  c->stacks2 = nullptr;
+  c->stacks2 = nullptr; c->glist = nullptr;
   c->Ncap = 0; c->N = 0;
 }
 

@@ -389,6 +389,59 @@ __device__ __forceinline__ int wave_next_group(int* counters, int ngroups, int& 
   return ngroups;
 }
 
+// ---- tree walks ----
+// A walk's node stack: the first kLdsStack entries live in LDS, deeper ones in the wave's
+// global stack region (kStackCap entries), so the common case never waits on global memory.
+constexpr int kLdsStack = 512;
+
+__device__ __forceinline__ int stack_get(const int* lstk, const int* gstk, int s) {
+  return s < kLdsStack ? lstk[s] : gstk[s - kLdsStack];
+}
+__device__ __forceinline__ void stack_put(int* lstk, int* gstk, int s, int v) {
+  if (s < kLdsStack) lstk[s] = v;
+  else gstk[s - kLdsStack] = v;
+}
+
+// particle range and children of a node (unified index: leaf l is node N - 1 + l), loaded
+// together with the node's box so that a walk step waits on global memory once
+struct NodeLinks {
+  int first, count, left, right;
+};
+__device__ __forceinline__ void load_links(int n, int N, const int* __restrict__ first,
+                                           const int* __restrict__ last,
+                                           const int* __restrict__ left,
+                                           const int* __restrict__ right, NodeLinks& l) {
+  if (n >= N - 1) {
+    l.first = n - (N - 1); l.count = 1; l.left = -1; l.right = -1;
+  } else {
+    const int f = first[n], e = last[n];
+    l.first = f; l.count = e - f + 1; l.left = left[n]; l.right = right[n];
+  }
+}
+
+// Queue of walk groups for a grid of persistent wavefronts.  nq entries (all groups, or a
+// compacted list of the groups with targets): when they fit the grid, wavefront b takes entry b
+// and stops, with no atomics and the work spread over every XCD; otherwise the entries come from
+// the per-XCD ranged queues of wave_next_group.  Returns nq when the wavefront is done.
+struct GroupQueue {
+  int nq;
+  int range;
+  bool once;
+};
+__device__ __forceinline__ GroupQueue group_queue(int nq) {
+  GroupQueue q;
+  q.nq = nq; q.range = blockIdx.x % kXcds; q.once = false;
+  return q;
+}
+__device__ __forceinline__ int group_next(GroupQueue& q, int* counters) {
+  if (q.nq <= (int) gridDim.x) {
+    if (q.once) return q.nq;
+    q.once = true;
+    return (int) blockIdx.x < q.nq ? (int) blockIdx.x : q.nq;
+  }
+  return wave_next_group(counters, q.nq, q.range);
+}
+
 // Kahan-compensated accumulator for the mixed-precision pair sums
 struct KahanF {
   float s = 0.f, c = 0.f;

@@ -42,6 +42,7 @@ struct GravArgs {
   const double4* posm; const double4* grec;     // grec: (h, zeta, hfactor, 0)
   double* atx; double* aty; double* atz; double* gpot;
   int* stacks; int* counter; int* flags; long long* icount;
+  const int* glist; int ncap;   // groups to walk (count at glist[ncap]); NULL = all groups
   int32_t* gnc; int32_t* gnn;
   const int* act;         // block timesteps: only active targets (NULL = all)
   const int* grp_root;    // serving batches: walk from the group's segment subtree (NULL = root)
@@ -137,18 +138,20 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
   __shared__ int cbuf[CCAP];
   __shared__ int sidx[kWave];
   // staged batch: particles (x, y, z, m, r2, h, zeta, hfactor) or cells (com x3, m, q x5)
-  __shared__ double stage[9*kWave];
+  __shared__ double stage[(MIXED ? 5 : 9)*kWave];   // the FP32 path stages 9 floats per lane
   double (*sd)[kWave] = reinterpret_cast<double (*)[kWave]>(stage);
   float (*sf)[kWave] = reinterpret_cast<float (*)[kWave]>(stage);
+  __shared__ int lstk[kLdsStack];    // node stack; deeper entries go to the global region
   const int lane = threadIdx.x;
   int* stk = a.stacks + (size_t) blockIdx.x*kStackCap;
   const int N = a.N;
   const KernelDesc& K = a.p.kern;
   const double kr = K.range, krsqd = K.rangesqd;
-  int range = blockIdx.x % kXcds;
+  GroupQueue gq = group_queue(a.glist ? a.glist[a.ncap] : a.ngroups);
   for (;;) {
-    const int g = wave_next_group(a.counter, a.ngroups, range);
-    if (g >= a.ngroups) break;
+    const int k = group_next(gq, a.counter);
+    if (k >= gq.nq) break;
+    const int g = a.glist ? a.glist[k] : k;
     const int G = a.grp_node[g], gfirst = a.grp_first[g], gcnt = a.grp_count[g];
     const bool has = lane < gcnt && (!a.act || a.act[gfirst + lane]);
     const int i = gfirst + lane;
@@ -174,18 +177,21 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
     int sp = 1, phead = 0, ptail = 0, chead = 0, ctail = 0;
     int nnear = 0, ncell = 0;
     bool overflow = false;
-    if (lane == 0) stk[0] = a.grp_root ? a.grp_root[g] : 0;   // the group's segment subtree
-    __threadfence_block();
+    if (lane == 0) lstk[0] = a.grp_root ? a.grp_root[g] : 0;   // the group's segment subtree
+    __syncthreads();
     while (sp > 0) {
       const int b = min(sp, kWave);
       int node = -1;
-      if (lane < b) node = stk[sp - 1 - lane];
+      if (lane < b) node = stack_get(lstk, stk, sp - 1 - lane);
       sp -= b;
       bool open = false, near = false, cell = false;
       int ef = 0, ec = 0, cl = 0, cr = 0;
       if (node >= 0) {
 #pragma clang fp contract(off)
+        // box, range and children in one round of loads
         const double4 nc = a.geo_c[node], nh = a.geo_h[node];
+        NodeLinks nl;
+        load_links(node, N, a.first, a.last, a.left, a.right, nl);
         double dx = nc.x - gc.x, dy = nc.y - gc.y, dz = nc.z - gc.z;
         nearest_image<double>(a.p, dx, dy, dz);
         const double gx = fmax(fabs(dx) - nh.x - gh.x, 0.0);
@@ -193,8 +199,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
         const double gz = fmax(fabs(dz) - nh.z - gh.z, 0.0);
         const double dbox2 = gx*gx + gy*gy + gz*gz;
         const double hr = kr*fmax(hmaxG, nc.w);
-        if (node >= N - 1) { ef = node - (N - 1); ec = 1; }
-        else { ef = a.first[node]; ec = a.last[node] - a.first[node] + 1; }
+        ef = nl.first; ec = nl.count;
         const bool bucket = ec <= a.nleaf || node >= N - 1;
         if (dbox2 < hr*hr*kGravNeibSlack) {
           if (bucket) near = true; else open = true;
@@ -209,18 +214,19 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
             if (bucket) near = true; else open = true;
           }
         }
-        if (open) { cl = a.left[node]; cr = a.right[node]; }
+        if (open) { cl = nl.left; cr = nl.right; }
       }
       // children onto the stack
       const unsigned long long om = __ballot(open);
       const int nopen = __popcll(om);
-      if (sp + 2*nopen > kStackCap) { overflow = true; break; }
+      if (sp + 2*nopen > kLdsStack + kStackCap) { overflow = true; break; }
       if (open) {
         const int o = sp + 2*lanes_below(om);
-        stk[o] = cr;
-        stk[o + 1] = cl;
+        stack_put(lstk, stk, o, cr);
+        stack_put(lstk, stk, o + 1, cl);
       }
       sp += 2*nopen;
+      if (sp > kLdsStack) __threadfence_block();   // entries in the global overflow region
       // accepted cells
       const unsigned long long cm = __ballot(cell);
       if (cell) cbuf[(ctail + lanes_below(cm)) & (CCAP - 1)] = node;
@@ -349,7 +355,6 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
         phead += nb;
         __syncthreads();
       }
-      __threadfence_block();
     }
     if (overflow) {
       if (lane == 0) atomicOr(&a.flags[2], 1);
@@ -521,8 +526,18 @@ GravArgs grav_args(gdf_ctx* c) {
   a.act = c->nlevels > 1 ? c->pi[I_ACT] : nullptr;
   a.grp_root = c->nseg > 0 ? c->grp_root : nullptr;
   a.gwx = a.gwy = a.gwz = a.gwp = a.gwp0 = nullptr;
+  a.glist = nullptr; a.ncap = c->Ncap;
   a.p = c->phys;
   return a;
+}
+
+// in a block step, the compacted list of groups with active particles (built on the main stream)
+int grav_group_list(gdf_ctx* c, GravArgs& a) {
+  if (!a.act || !c->glist) return GDF_OK;
+  int* gl = c->glist + 2*((size_t) c->Ncap + 1);
+  GDF_TRY(group_list(c, a.act, gl, nullptr, c->grav_nc, c->grav_nn));
+  a.glist = gl;
+  return GDF_OK;
 }
 
 }  // namespace
@@ -535,11 +550,12 @@ int gravity_launch(gdf_ctx* c) {
   // on the second stream beside the scatter-gather walk and the hydro pass: measured faster
   // than running the two in turn, both at 1M active particles and in sparse block steps
   hipStream_t s2 = c->stream2;
+  GravArgs a = grav_args(c);
+  GDF_TRY(grav_group_list(c, a));
   GDF_HIP(c, hipEventRecord(c->ev_dens, c->stream));        // h, zeta, tree, posm are ready
   GDF_HIP(c, hipStreamWaitEvent(s2, c->ev_dens, 0));
   k_pack_grav<<<nblocks(N, 256), 256, 0, s2>>>(N, c->pf[F_H], c->df[D_ZETA], c->df[D_HFACTOR],
                                                  c->grec);
-  GravArgs a = grav_args(c);
   a.stacks = c->stacks2; a.counter = c->counters + 24; a.flags = c->counters + 16;
   a.gwx = c->df[D_GWX]; a.gwy = c->df[D_GWY]; a.gwz = c->df[D_GWZ];
   a.gwp = c->df[D_GWP]; a.gwp0 = c->df[D_GWP0];
@@ -585,6 +601,7 @@ int gravity_walk(gdf_ctx* c) {
   k_pack_grav<<<nblocks(N, 256), 256, 0, s>>>(N, c->pf[F_H], c->df[D_ZETA], c->df[D_HFACTOR],
                                                 c->grec);
   GravArgs a = grav_args(c);
+  GDF_TRY(grav_group_list(c, a));
   k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
   k_zero8<<<1, 64, 0, s>>>(c->counters);
   const int grid = std::min(c->ngroups, kWalkSlots);

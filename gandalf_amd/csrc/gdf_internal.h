@@ -138,6 +138,10 @@ struct gdf_ctx {
   // walk scratch
   int* stacks = nullptr;
   int* stacks2 = nullptr;      // the gravity walk's stacks when it runs beside the hydro pass
+  // walk groups with at least one target, in ascending order: [0, Ncap] for the gather walk
+  // of the h-rho rounds, [Ncap + 1, 2 Ncap + 1] for the active particles of a block step (the
+  // scatter-gather walk and the gravity walk); each list's count follows its Ncap entries
+  int* glist = nullptr;
   int* counters = nullptr;     // [0..7] flags, [4..11] walk range counters, [12..13] block
                                // reductions; [16..23] flags and [24..31] range counters of the
                                // gravity walk on the second stream
@@ -200,6 +204,9 @@ int ctx_field_width(gdf_ctx* c, const char* name);
 int seg_upload(gdf_ctx* c);
 int tree_build(gdf_ctx* c);
 int tree_refit_h(gdf_ctx* c);
+// compacted list of the walk groups holding a particle with status[i] == 1 (ascending, count at
+// list[Ncap]); the optional per-group outputs of every other group are cleared
+int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, int* gnn);
 TreeView tree_view(const gdf_ctx* c);
 
 // SPH passes (gdf_sph.hip)

@@ -28,12 +28,6 @@ constexpr int CANDCAP = 1024;              // ring buffer of candidate slots (po
 constexpr double NODE_SLACK = 1.0 + 1e-12; // node tests err on the inclusive side
 constexpr int kInitBatches = 16;           // initial candidate capacity per group, in batches
 
-__device__ __forceinline__ void node_range(int n, int N, const int* first, const int* last,
-                                           int& f, int& cnt) {
-  if (n >= N - 1) { f = n - (N - 1); cnt = 1; }
-  else { f = first[n]; cnt = last[n] - first[n] + 1; }
-}
-
 struct CandWalk {
   int N, ngroups, nleaf, capb;
   const int* grp_node; const int* grp_first; const int* grp_count;
@@ -42,6 +36,8 @@ struct CandWalk {
   const double* x; const double* y; const double* z; const double* hcand; const double* htarget;
   const int* status;                 // gather walks: only targets with status 1 take part
   const int* grp_root;               // serving batches: the group's segment subtree (NULL = root)
+  const int* glist;                  // groups to walk (count at glist[Ncap]); NULL = all groups
+  int ncap;
   int* idx; unsigned long long* mask; int* ncand;
   int* stacks; int* counter; int* flags;
   PhysParams p;
@@ -54,18 +50,20 @@ struct CandWalk {
 template <int MODE, bool PER>
 __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
   __shared__ int cand[CANDCAP];
-  __shared__ double4 sc[kWave];      // MODE 0/1: (x, y, z, (k h_j)^2)
-  __shared__ float4 fc[kWave];       // MODE 2: group-frame (x, y, z, (k h_j)^2)
+  __shared__ double4 sc[MODE == 2 ? 1 : kWave];   // MODE 0/1: (x, y, z, (k h_j)^2)
+  __shared__ float4 fc[MODE == 2 ? kWave : 1];    // MODE 2: group-frame (x, y, z, (k h_j)^2)
+  __shared__ int lstk[kLdsStack];    // node stack; deeper entries go to the global region
   const int lane = threadIdx.x;
   int* stk = a.stacks + (size_t) blockIdx.x*kStackCap;
   const int N = a.N;
   const double krsqd = a.p.kern.rangesqd, kr = a.p.kern.range;
   PhysParams pp = a.p;
   pp.periodic = PER ? 1 : 0;
-  int range = blockIdx.x % kXcds;
+  GroupQueue q = group_queue(a.glist ? a.glist[a.ncap] : a.ngroups);
   for (;;) {
-    const int g = wave_next_group(a.counter, a.ngroups, range);
-    if (g >= a.ngroups) break;
+    const int k = group_next(q, a.counter);
+    if (k >= q.nq) break;
+    const int g = a.glist ? a.glist[k] : k;
     const int G = a.grp_node[g], gfirst = a.grp_first[g], gcnt = a.grp_count[g];
     const int i = gfirst + lane;
     bool has = lane < gcnt;
@@ -88,17 +86,20 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
     int nbat = 0;
     int sp = 1, head = 0, tail = 0;
     bool overflow = false;
-    if (lane == 0) stk[0] = a.grp_root ? a.grp_root[g] : 0;   // the group's segment subtree
-    __threadfence_block();
+    if (lane == 0) lstk[0] = a.grp_root ? a.grp_root[g] : 0;   // the group's segment subtree
+    __syncthreads();
     while (sp > 0) {
       const int b = min(sp, kWave);
       int node = -1;
-      if (lane < b) node = stk[sp - 1 - lane];
+      if (lane < b) node = stack_get(lstk, stk, sp - 1 - lane);
       sp -= b;
       bool open = false, emit = false;
       int ef = 0, ec = 0, cl = 0, cr = 0;
       if (node >= 0) {
+        // box, range and children in one round of loads
         const double4 nc = a.geo_c[node], nh = a.geo_h[node];
+        NodeLinks nl;
+        load_links(node, N, a.first, a.last, a.left, a.right, nl);
         double dx = nc.x - gc.x, dy = nc.y - gc.y, dz = nc.z - gc.z;
         nearest_image<double>(pp, dx, dy, dz);
         const double gx = fmax(fabs(dx) - nh.x - gh.x, 0.0);
@@ -107,20 +108,21 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
         const double dbox2 = gx*gx + gy*gy + gz*gz;
         const double rr = MODE == 0 ? kr*hG : kr*fmax(hG, nc.w);
         if (dbox2 <= rr*rr*NODE_SLACK) {
-          node_range(node, N, a.first, a.last, ef, ec);
+          ef = nl.first; ec = nl.count;
           if (ec <= a.nleaf) emit = true;
-          else { open = true; cl = a.left[node]; cr = a.right[node]; }
+          else { open = true; cl = nl.left; cr = nl.right; }
         }
       }
       const unsigned long long om = __ballot(open);
       const int nopen = __popcll(om);
-      if (sp + 2*nopen > kStackCap) { overflow = true; break; }
+      if (sp + 2*nopen > kLdsStack + kStackCap) { overflow = true; break; }
       if (open) {
         const int o = sp + 2*lanes_below(om);
-        stk[o] = cr;
-        stk[o + 1] = cl;
+        stack_put(lstk, stk, o, cr);
+        stack_put(lstk, stk, o + 1, cl);
       }
       sp += 2*nopen;
+      if (sp > kLdsStack) __threadfence_block();   // entries in the global overflow region
       const int ecount = emit ? ec : 0;
       const int incl = wave_incl_scan(ecount);
       const int tot = __shfl(incl, 63, 64);
@@ -170,7 +172,6 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
         head += nb;
         __syncthreads();
       }
-      __threadfence_block();
     }
     if (overflow) {
       if (lane == 0) atomicOr(&a.flags[2], 1);
@@ -674,7 +675,9 @@ int grow_cand(gdf_ctx* c, CandSet& cs, int need) {
   return alloc_cand(c, cs, std::max(need + need/4 + 1, cs.capb + 1));
 }
 
-int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status) {
+// glist: the groups holding targets (from group_list), or NULL to walk every group
+int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status,
+                const int* glist) {
   CandWalk w;
   w.N = c->N; w.ngroups = c->ngroups; w.nleaf = c->phys.nleaf; w.capb = cs.capb;
   w.grp_node = c->grp_node; w.grp_first = c->grp_first; w.grp_count = c->grp_count;
@@ -685,6 +688,7 @@ int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status)
   w.htarget = mode == 0 ? c->df[D_HSEARCH] : c->pf[F_H];
   w.status = status;
   w.grp_root = c->nseg > 0 ? c->grp_root : nullptr;
+  w.glist = glist; w.ncap = c->Ncap;
   w.idx = cs.idx; w.mask = cs.mask; w.ncand = cs.ncand;
   w.stacks = c->stacks; w.counter = c->counters + 4; w.flags = c->counters;
   w.p = c->phys;
@@ -713,6 +717,7 @@ int sph_alloc_lists(gdf_ctx* c) {
   GDF_HIP(c, hipMalloc(&c->hrecA, sizeof(double4)*(size_t) c->Ncap));
   GDF_HIP(c, hipMalloc(&c->hrecB, sizeof(double4)*(size_t) c->Ncap));
   GDF_HIP(c, hipMalloc(&c->stacks, sizeof(int)*(size_t) kWalkSlots*kStackCap));
+  GDF_HIP(c, hipMalloc(&c->glist, sizeof(int)*3*((size_t) c->Ncap + 1)));
   if (c->phys.self_gravity && c->phys.hydro_forces && !c->phys.nbody)
     GDF_HIP(c, hipMalloc(&c->stacks2, sizeof(int)*(size_t) kWalkSlots*kStackCap));
   return GDF_OK;
@@ -742,7 +747,8 @@ int sph_density(gdf_ctx* c, int nopbc) {
   d.p = c->phys;
   for (int round = 0; round < 64; round++) {
     k_zero_ints<<<1, BS, 0, s>>>(4, c->counters);
-    GDF_TRY(launch_walk(c, 0, nopbc, c->cs_g, c->status));
+    GDF_TRY(group_list(c, c->status, c->glist, c->cs_g.ncand, nullptr, nullptr));
+    GDF_TRY(launch_walk(c, 0, nopbc, c->cs_g, c->status, c->glist));
     d.capb = c->cs_g.capb; d.idx = c->cs_g.idx; d.mask = c->cs_g.mask; d.ncand = c->cs_g.ncand;
     if (c->cfg.precision == GDF_PREC_MIXED) k_density<float><<<c->ngroups, kWave, 0, s>>>(d);
     else k_density<double><<<c->ngroups, kWave, 0, s>>>(d);
@@ -765,7 +771,10 @@ int sph_build_sg(gdf_ctx* c) {
   GDF_TRY(ensure_cand(c, c->cs_s));
   for (int round = 0; round < 8; round++) {
     k_zero_ints<<<1, BS, 0, c->stream>>>(4, c->counters);
-    GDF_TRY(launch_walk(c, 1, 0, c->cs_s, c->nlevels > 1 ? c->pi[I_ACT] : nullptr));
+    const int* act = c->nlevels > 1 ? c->pi[I_ACT] : nullptr;
+    int* sl = c->glist + (c->Ncap + 1);
+    if (act) GDF_TRY(group_list(c, act, sl, c->cs_s.ncand, nullptr, nullptr));
+    GDF_TRY(launch_walk(c, 1, 0, c->cs_s, act, act ? sl : nullptr));
     GDF_TRY(read_flags(c));
     if (c->h_counters[2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in force pass");
     if (c->h_counters[1] > 0) { GDF_TRY(grow_cand(c, c->cs_s, c->h_counters[1])); continue; }

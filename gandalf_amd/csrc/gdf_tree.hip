@@ -282,47 +282,67 @@ __global__ void k_leaves(int N, const double* __restrict__ x, const double* __re
   a.geo_h[n] = make_double4(0.0, 0.0, 0.0, __ddiv_rn(__dmul_rn(hh, hh), theta2));
 }
 
-// merge children (left, right) into internal node n; operation order of KDTree.cpp:950-1051
-__device__ void merge_node(int n, int L, int R, double theta2, const NodeArgs& a) {
-#pragma clang fp contract(off)
-  const double mL = a.mp[10*L+3], mR = a.mp[10*R+3];
-  const double m = mL + mR;
-  double com[3], lo[3], hi[3];
+// Moments of one node as a refit climb carries them: bounding box, centre of mass, mass,
+// quadrupole moment and the largest smoothing length below the node.
+struct NodeD {
+  double lo[3], hi[3], com[3], m, q[5], hmax;
+};
+
+__device__ __forceinline__ void load_node(int n, const NodeArgs& a, NodeD& d) {
   for (int k = 0; k < 3; k++) {
-    com[k] = m > 0 ? (mL*a.mp[10*L+k] + mR*a.mp[10*R+k])/m : 0.0;
-    lo[k] = fmin(a.bbmin[3*L+k], a.bbmin[3*R+k]);
-    hi[k] = fmax(a.bbmax[3*L+k], a.bbmax[3*R+k]);
+    d.lo[k] = a.bbmin[3*n+k]; d.hi[k] = a.bbmax[3*n+k]; d.com[k] = a.mp[10*n+k];
   }
-  const double hmax = fmax(a.geo_c[L].w, a.geo_c[R].w);
-  double q[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const int ch[2] = {L, R};
-  for (int c = 0; c < 2; c++) {
-    const int C = ch[c];
-    const double mi = a.mp[10*C+3];
-    if (!(mi > 0)) continue;
-    double dr[3];
-    for (int k = 0; k < 3; k++) dr[k] = a.mp[10*C+k] - com[k];
-    const double drsqd = dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2];
-    for (int k = 0; k < 5; k++) q[k] += a.mp[10*C+4+k];
-    q[0] += mi*(3.0*dr[0]*dr[0] - drsqd);
-    q[1] += mi*3.0*dr[0]*dr[1];
-    q[2] += mi*(3.0*dr[1]*dr[1] - drsqd);
-    q[3] += mi*3.0*dr[2]*dr[0];
-    q[4] += mi*3.0*dr[2]*dr[1];
-  }
-  for (int k = 0; k < 3; k++) { a.mp[10*n+k] = com[k]; a.bbmin[3*n+k] = lo[k]; a.bbmax[3*n+k] = hi[k]; }
-  a.mp[10*n+3] = m;
-  for (int k = 0; k < 5; k++) a.mp[10*n+4+k] = q[k];
-  double c[3], hf[3];
-  for (int k = 0; k < 3; k++) { c[k] = 0.5*(lo[k] + hi[k]); hf[k] = 0.5*(hi[k] - lo[k]); }
-  const double rmaxsqd = hf[0]*hf[0] + hf[1]*hf[1] + hf[2]*hf[2];
-  a.geo_c[n] = make_double4(c[0], c[1], c[2], hmax);
-  a.geo_h[n] = make_double4(hf[0], hf[1], hf[2], fmax(rmaxsqd, hmax*hmax)/theta2);
+  d.m = a.mp[10*n+3];
+  for (int k = 0; k < 5; k++) d.q[k] = a.mp[10*n+4+k];
+  d.hmax = a.geo_c[n].w;
 }
 
-__device__ void merge_h(int n, int L, int R, double theta2, const NodeArgs& a) {
+// merge children L (left) and R (right) into o; operation order of KDTree.cpp:950-1051
+__device__ __forceinline__ void merge_d(const NodeD& L, const NodeD& R, NodeD& o) {
 #pragma clang fp contract(off)
-  const double hmax = fmax(a.geo_c[L].w, a.geo_c[R].w);
+  const double m = L.m + R.m;
+  for (int k = 0; k < 3; k++) {
+    o.com[k] = m > 0 ? (L.m*L.com[k] + R.m*R.com[k])/m : 0.0;
+    o.lo[k] = fmin(L.lo[k], R.lo[k]);
+    o.hi[k] = fmax(L.hi[k], R.hi[k]);
+  }
+  o.m = m;
+  o.hmax = fmax(L.hmax, R.hmax);
+  for (int k = 0; k < 5; k++) o.q[k] = 0.0;
+  for (int c = 0; c < 2; c++) {
+    const NodeD& C = c == 0 ? L : R;
+    const double mi = C.m;
+    if (!(mi > 0)) continue;
+    double dr[3];
+    for (int k = 0; k < 3; k++) dr[k] = C.com[k] - o.com[k];
+    const double drsqd = dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2];
+    for (int k = 0; k < 5; k++) o.q[k] += C.q[k];
+    o.q[0] += mi*(3.0*dr[0]*dr[0] - drsqd);
+    o.q[1] += mi*3.0*dr[0]*dr[1];
+    o.q[2] += mi*(3.0*dr[1]*dr[1] - drsqd);
+    o.q[3] += mi*3.0*dr[2]*dr[0];
+    o.q[4] += mi*3.0*dr[2]*dr[1];
+  }
+}
+
+__device__ __forceinline__ void store_node(int n, const NodeD& d, double theta2,
+                                           const NodeArgs& a) {
+#pragma clang fp contract(off)
+  for (int k = 0; k < 3; k++) {
+    a.mp[10*n+k] = d.com[k]; a.bbmin[3*n+k] = d.lo[k]; a.bbmax[3*n+k] = d.hi[k];
+  }
+  a.mp[10*n+3] = d.m;
+  for (int k = 0; k < 5; k++) a.mp[10*n+4+k] = d.q[k];
+  double c[3], hf[3];
+  for (int k = 0; k < 3; k++) { c[k] = 0.5*(d.lo[k] + d.hi[k]); hf[k] = 0.5*(d.hi[k] - d.lo[k]); }
+  const double rmaxsqd = hf[0]*hf[0] + hf[1]*hf[1] + hf[2]*hf[2];
+  a.geo_c[n] = make_double4(c[0], c[1], c[2], d.hmax);
+  a.geo_h[n] = make_double4(hf[0], hf[1], hf[2], fmax(rmaxsqd, d.hmax*d.hmax)/theta2);
+}
+
+// new smoothing-length bound of internal node n whose box is unchanged
+__device__ __forceinline__ void store_h(int n, double hmax, double theta2, const NodeArgs& a) {
+#pragma clang fp contract(off)
   double4 gc = a.geo_c[n];
   double4 gh = a.geo_h[n];
   const double rmaxsqd = gh.x*gh.x + gh.y*gh.y + gh.z*gh.z;
@@ -349,60 +369,199 @@ __global__ void k_zero_visit(int n, int* __restrict__ visit) {
 constexpr int RB = 1024;   // leaves per phase-1 workgroup
 constexpr int RS = 32;     // phase-1 chunks per phase-2 workgroup
 
-template <bool HONLY>
-__device__ __forceinline__ void merge_any(int p, const int* __restrict__ left,
-                                          const int* __restrict__ right, double theta2,
-                                          const NodeArgs& a) {
-  if (HONLY) merge_h(p, left[p], right[p], theta2, a);
-  else merge_node(p, left[p], right[p], theta2, a);
-}
-
+// Phase 1, fused with the leaf moments.  The chunk's topology is staged in LDS, and each
+// climbing thread carries its node's moments in registers: before it counts its arrival at a
+// parent it leaves them in its own LDS slot, and the exchange that counts the arrival hands the
+// second arriver the first one's slot, so a merge reads nothing from global memory.  A slot is
+// read only after its owner has stopped climbing, hence is never overwritten while still needed.
 template <bool HONLY>
 __global__ __launch_bounds__(RB) void k_refit_chunks(
-    int N, const int* __restrict__ parent, const int* __restrict__ left,
-    const int* __restrict__ right, const int* __restrict__ first, const int* __restrict__ last,
-    double theta2, NodeArgs a, int* __restrict__ front1, int* __restrict__ cnt1) {
-  __shared__ int arrivals[RB];
+    int N, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ mass, const double* __restrict__ h,
+    const int* __restrict__ parent, const int* __restrict__ left, const int* __restrict__ first,
+    const int* __restrict__ last, double theta2, NodeArgs a, int* __restrict__ visit,
+    int* __restrict__ front1, int* __restrict__ cnt1) {
+#pragma clang fp contract(off)
+  constexpr int ND = HONLY ? 1 : 16;    // doubles carried per node
+  __shared__ double sd[ND*RB];
+  __shared__ int arr[RB];               // 0, or 1 + slot of the first arrival
+  __shared__ int lpar[RB];              // parent of leaf c0 + t
+  __shared__ int ipar[RB];              // parent of internal node c0 + t
+  __shared__ int lft[RB];               // left child of internal node c0 + t, -1 if not inside
   __shared__ int nfront;
+  const int t = threadIdx.x;
   const int c0 = blockIdx.x*RB;
   const int c1 = min(c0 + RB, N) - 1;
-  arrivals[threadIdx.x] = 0;
-  if (threadIdx.x == 0) nfront = 0;
+  const int l = c0 + t;
+  arr[t] = 0;
+  if (t == 0) nfront = 0;
+  NodeD d;
+  if (l <= c1) {
+    lpar[t] = parent[N - 1 + l];
+    if (l < N - 1) {
+      ipar[t] = parent[l];
+      lft[t] = (first[l] >= c0 && last[l] <= c1) ? left[l] : -1;
+      visit[l] = 0;
+    } else {
+      lft[t] = -1;
+    }
+    const int n = N - 1 + l;
+    const double hh = h[l];
+    d.hmax = hh;
+    if (HONLY) {
+      double4 gc = a.geo_c[n]; gc.w = hh; a.geo_c[n] = gc;
+      double4 gh = a.geo_h[n]; gh.w = hh*hh/theta2; a.geo_h[n] = gh;
+    } else {
+      const double r[3] = {x[l], y[l], z[l]};
+      for (int k = 0; k < 3; k++) { d.lo[k] = r[k]; d.hi[k] = r[k]; d.com[k] = r[k]; }
+      d.m = mass[l];
+      for (int k = 0; k < 5; k++) d.q[k] = 0.0;
+      for (int k = 0; k < 3; k++) { a.bbmin[3*n+k] = r[k]; a.bbmax[3*n+k] = r[k]; a.mp[10*n+k] = r[k]; }
+      a.mp[10*n+3] = d.m;
+      for (int k = 4; k < 10; k++) a.mp[10*n+k] = 0.0;
+      a.geo_c[n] = make_double4(r[0], r[1], r[2], hh);
+      a.geo_h[n] = make_double4(0.0, 0.0, 0.0, hh*hh/theta2);
+    }
+  }
   __syncthreads();
-  const int l = c0 + (int) threadIdx.x;
   if (l <= c1) {
     int n = N - 1 + l;
-    int p = parent[n];
+    int p = lpar[t];
     while (p >= 0) {
-      if (first[p] < c0 || last[p] > c1) {
+      const int lf = (p >= c0 && p <= c1) ? lft[p - c0] : -1;
+      if (lf < 0) {
         // n is a maximal node of this chunk: at most RB of them
         front1[c0 + atomicAdd(&nfront, 1)] = n;
         break;
       }
-      const int old = __hip_atomic_fetch_add(&arrivals[p - c0], 1, __ATOMIC_ACQ_REL,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (HONLY) {
+        sd[t] = d.hmax;
+      } else {
+        for (int k = 0; k < 3; k++) {
+          sd[k*RB + t] = d.lo[k]; sd[(3 + k)*RB + t] = d.hi[k]; sd[(6 + k)*RB + t] = d.com[k];
+        }
+        sd[9*RB + t] = d.m;
+        for (int k = 0; k < 5; k++) sd[(10 + k)*RB + t] = d.q[k];
+        sd[15*RB + t] = d.hmax;
+      }
+      const int old = __hip_atomic_exchange(&arr[p - c0], t + 1, __ATOMIC_ACQ_REL,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP);
       if (old == 0) break;
-      merge_any<HONLY>(p, left, right, theta2, a);
+      const int o = old - 1;
+      if (HONLY) {
+        d.hmax = fmax(d.hmax, sd[o]);
+        store_h(p, d.hmax, theta2, a);
+      } else {
+        NodeD e, r;
+        for (int k = 0; k < 3; k++) {
+          e.lo[k] = sd[k*RB + o]; e.hi[k] = sd[(3 + k)*RB + o]; e.com[k] = sd[(6 + k)*RB + o];
+        }
+        e.m = sd[9*RB + o];
+        for (int k = 0; k < 5; k++) e.q[k] = sd[(10 + k)*RB + o];
+        e.hmax = sd[15*RB + o];
+        if (lf == n) merge_d(d, e, r);
+        else merge_d(e, d, r);
+        store_node(p, r, theta2, a);
+        d = r;
+      }
       n = p;
-      p = parent[p];
+      p = ipar[p - c0];
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) cnt1[blockIdx.x] = nfront;
+  if (t == 0) cnt1[blockIdx.x] = nfront;
+}
+
+// Topology of the next node of a climb, loaded together with the moments of the current
+// merge so that each level of phases 2 and 3 waits on global memory about once.
+struct Topo {
+  int first, last, left, right, parent;
+  double rmax2;   // squared half-diagonal of the node's box (smoothing-length refit only)
+};
+
+template <bool HONLY>
+__device__ __forceinline__ void load_topo(int p, const int* __restrict__ parent,
+                                          const int* __restrict__ left,
+                                          const int* __restrict__ right,
+                                          const int* __restrict__ first,
+                                          const int* __restrict__ last, const NodeArgs& a,
+                                          Topo& q) {
+#pragma clang fp contract(off)
+  q.first = first[p]; q.last = last[p]; q.left = left[p]; q.right = right[p];
+  q.parent = parent[p];
+  if (HONLY) {
+    // the box only: another thread may be writing the node's .w bound
+    const double hx = a.geo_h[p].x, hy = a.geo_h[p].y, hz = a.geo_h[p].z;
+    q.rmax2 = hx*hx + hy*hy + hz*hz;
+  }
+}
+
+// Climb from node n through the nodes whose leaf range lies in [lo, hi] (phase 2) or up to
+// the root (phase 3, lo = 0 and hi = N - 1).  The second arrival at a node, counted in
+// cnt[p - cbase], merges it from its own moments (registers) and its sibling's (global memory,
+// written before the sibling's thread counted its arrival).  A node whose parent leaves the
+// range goes to the next frontier.
+template <bool HONLY>
+__device__ __forceinline__ void climb(int n, int lo, int hi, int* cnt, int cbase,
+                                      const int* __restrict__ parent,
+                                      const int* __restrict__ left,
+                                      const int* __restrict__ right,
+                                      const int* __restrict__ first,
+                                      const int* __restrict__ last, double theta2,
+                                      const NodeArgs& a, int* __restrict__ front_next,
+                                      int* __restrict__ front_count) {
+#pragma clang fp contract(off)
+  NodeD d;
+  if (HONLY) d.hmax = a.geo_c[n].w;
+  else load_node(n, a, d);
+  int p = parent[n];
+  Topo q;
+  if (p >= 0) load_topo<HONLY>(p, parent, left, right, first, last, a, q);
+  while (p >= 0) {
+    if (q.first < lo || q.last > hi) {
+      front_next[atomicAdd(front_count, 1)] = n;
+      break;
+    }
+    const int old = __hip_atomic_fetch_add(&cnt[p - cbase], 1, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0) break;
+    const bool isleft = q.left == n;
+    const int sib = isleft ? q.right : q.left;
+    const int pp = q.parent;
+    Topo qn;
+    if (pp >= 0) load_topo<HONLY>(pp, parent, left, right, first, last, a, qn);
+    if (HONLY) {
+      d.hmax = fmax(d.hmax, a.geo_c[sib].w);
+      a.geo_c[p].w = d.hmax;
+      a.geo_h[p].w = fmax(q.rmax2, d.hmax*d.hmax)/theta2;
+    } else {
+      NodeD e, r;
+      load_node(sib, a, e);
+      if (isleft) merge_d(d, e, r);
+      else merge_d(e, d, r);
+      store_node(p, r, theta2, a);
+      d = r;
+    }
+    n = p;
+    p = pp;
+    q = qn;
+  }
 }
 
 // Phase 2: one workgroup per RS consecutive phase-1 chunks climbs from their frontier nodes
-// through the nodes inside its leaf range (every arrival at such a node comes from this
-// workgroup, so workgroup-scope counters suffice); maximal nodes go to the phase-3 frontier.
+// through the nodes inside its leaf range.  Every arrival at such a node comes from this
+// workgroup, and the node's index lies in the range, so the arrival counters live in LDS.
 template <bool HONLY>
 __global__ __launch_bounds__(RB) void k_refit_mid(
     int N, int nchunks, const int* __restrict__ parent, const int* __restrict__ left,
     const int* __restrict__ right, const int* __restrict__ first, const int* __restrict__ last,
     double theta2, NodeArgs a, const int* __restrict__ front1, const int* __restrict__ cnt1,
-    int* __restrict__ visit, int* __restrict__ front2) {
+    int* __restrict__ front2) {
+  __shared__ int arr[RB*RS];
   __shared__ int pre[RS + 1];
   const int k0 = blockIdx.x*RS, k1 = min(k0 + RS, nchunks);
   const int lo = k0*RB, hi = min(k1*RB, N) - 1;
+  for (int i = threadIdx.x; i < RB*RS; i += RB) arr[i] = 0;
   if (threadIdx.x == 0) {
     pre[0] = 0;
     for (int c = 0; c < RS; c++) pre[c + 1] = pre[c] + (k0 + c < k1 ? cnt1[k0 + c] : 0);
@@ -412,40 +571,23 @@ __global__ __launch_bounds__(RB) void k_refit_mid(
   for (int e = threadIdx.x; e < total; e += RB) {
     int c = 0;
     while (pre[c + 1] <= e) c++;
-    int n = front1[(k0 + c)*RB + (e - pre[c])];
-    int p = parent[n];
-    while (p >= 0) {
-      if (first[p] < lo || last[p] > hi) {
-        front2[atomicAdd(&front2[N], 1)] = n;
-        break;
-      }
-      const int old = __hip_atomic_fetch_add(&visit[p], 1, __ATOMIC_ACQ_REL,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) break;
-      merge_any<HONLY>(p, left, right, theta2, a);
-      n = p;
-      p = parent[p];
-    }
+    climb<HONLY>(front1[(k0 + c)*RB + (e - pre[c])], lo, hi, arr, lo, parent, left, right,
+                 first, last, theta2, a, front2, &front2[N]);
   }
 }
 
-// Phase 3: one workgroup climbs from the phase-3 frontier to the root.
+// Phase 3: one workgroup climbs from the phase-3 frontier to the root, counting arrivals in
+// global memory (the nodes above the frontier may have any index).
 template <bool HONLY>
 __global__ __launch_bounds__(RB) void k_refit_top(
     int N, const int* __restrict__ parent, const int* __restrict__ left,
-    const int* __restrict__ right, double theta2, NodeArgs a, const int* __restrict__ front2,
-    int* __restrict__ visit) {
+    const int* __restrict__ right, const int* __restrict__ first, const int* __restrict__ last,
+    double theta2, NodeArgs a, const int* __restrict__ front2, int* __restrict__ visit) {
   const int nf = front2[N];
-  for (int k = threadIdx.x; k < nf; k += RB) {
-    int p = parent[front2[k]];
-    while (p >= 0) {
-      const int old = __hip_atomic_fetch_add(&visit[p], 1, __ATOMIC_ACQ_REL,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) break;
-      merge_any<HONLY>(p, left, right, theta2, a);
-      p = parent[p];
-    }
-  }
+  // every range lies in [0, N - 1], so the climb never reaches its frontier branch
+  for (int k = threadIdx.x; k < nf; k += RB)
+    climb<HONLY>(front2[k], 0, N - 1, visit, 0, parent, left, right, first, last, theta2, a,
+                 nullptr, nullptr);
 }
 
 template <bool HONLY>
@@ -457,15 +599,16 @@ void launch_refit(gdf_ctx* c, const NodeArgs& na, double theta2) {
   int* cnt1 = front1 + (size_t) nch*RB;            // nch counts
   int* front2 = cnt1 + nch;                        // N entries + count
   k_zero_visit<<<1, 64, 0, s>>>(1, front2 + N);
-  k_refit_chunks<HONLY><<<nch, RB, 0, s>>>(N, c->parent, c->left, c->right, c->first, c->last,
-                                           theta2, na, front1, cnt1);
+  k_refit_chunks<HONLY><<<nch, RB, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M],
+                                           c->pf[F_H], c->parent, c->left, c->first, c->last,
+                                           theta2, na, c->visit, front1, cnt1);
   if (nch > 1) {
     const int nmid = (nch + RS - 1)/RS;
     k_refit_mid<HONLY><<<nmid, RB, 0, s>>>(N, nch, c->parent, c->left, c->right, c->first,
-                                           c->last, theta2, na, front1, cnt1, c->visit, front2);
+                                           c->last, theta2, na, front1, cnt1, front2);
     if (nmid > 1)
-      k_refit_top<HONLY><<<1, RB, 0, s>>>(N, c->parent, c->left, c->right, theta2, na, front2,
-                                          c->visit);
+      k_refit_top<HONLY><<<1, RB, 0, s>>>(N, c->parent, c->left, c->right, c->first, c->last,
+                                          theta2, na, front2, c->visit);
   }
 }
 
@@ -506,6 +649,25 @@ __global__ void k_groups_fill(int N, const int* __restrict__ nsel, const int* __
   const int n = gstart[s];
   gnode[g] = n;
   gcount[g] = n >= N - 1 ? 1 : last[n] - first[n] + 1;
+}
+
+// one wavefront per walk group: does any of its particles have status 1?
+__global__ void k_group_flags(int ngroups, const int* __restrict__ grp_first,
+                              const int* __restrict__ grp_count, const int* __restrict__ status,
+                              unsigned char* __restrict__ flags, int* __restrict__ ncand,
+                              int* __restrict__ gnc, int* __restrict__ gnn) {
+  const int g = blockIdx.x*(BS/kWave) + threadIdx.x/kWave;
+  const int lane = threadIdx.x % kWave;
+  if (g >= ngroups) return;
+  const bool on = lane < grp_count[g] && status[grp_first[g] + lane] == 1;
+  const bool any = __ballot(on) != 0;
+  if (lane == 0) {
+    flags[g] = any ? 1 : 0;
+    if (!any) {
+      if (ncand) ncand[g] = 0;
+      if (gnc) { gnc[g] = 0; gnn[g] = 0; }
+    }
+  }
 }
 
 }  // namespace
@@ -638,9 +800,11 @@ int tree_build(gdf_ctx* c) {
   // 5. moments, bottom-up
   NodeArgs na = node_args(c);
   const double theta2 = c->phys.thetamaxsqd;
-  k_leaves<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M],
-                                         c->pf[F_H], theta2, na, c->visit, c->parent);
-  if (N > 1) launch_refit<false>(c, na, theta2);
+  if (N > 1)
+    launch_refit<false>(c, na, theta2);
+  else
+    k_leaves<<<1, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M], c->pf[F_H],
+                              theta2, na, c->visit, c->parent);
   // 6. walk groups in ascending first-particle order
   GDF_HIP(c, hipMemsetAsync(c->visit, 0, sizeof(int)*((size_t) N), s));
   k_groups_mark<<<nblocks(2*N - 1, BS), BS, 0, s>>>(N, c->cfg.group_max, c->parent, c->first,
@@ -667,13 +831,30 @@ int tree_build(gdf_ctx* c) {
   return integ_pack_posm(c);
 }
 
+int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, int* gnn) {
+  const int ng = c->ngroups;
+  hipStream_t s = c->stream;
+  // the refit's arrival counters are free between tree passes (phase 1 clears them)
+  unsigned char* flags = reinterpret_cast<unsigned char*>(c->visit);
+  constexpr int gpb = BS/kWave;   // groups per block
+  k_group_flags<<<(ng + gpb - 1)/gpb, BS, 0, s>>>(ng, c->grp_first, c->grp_count, status, flags,
+                                                  ncand, gnc, gnn);
+  size_t bytes = c->sel_tmp_bytes;
+  GDF_HIP(c, rocprim::select(c->sel_tmp, bytes, rocprim::counting_iterator<int>(0),
+                             (const unsigned char*) flags, list, list + c->Ncap, (size_t) ng, s));
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
 int tree_refit_h(gdf_ctx* c) {
   const int N = c->N;
   hipStream_t s = c->stream;
   NodeArgs na = node_args(c);
   const double theta2 = c->phys.thetamaxsqd;
-  k_hleaves<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_H], theta2, na, c->visit);
-  if (N > 1) launch_refit<true>(c, na, theta2);
+  if (N > 1)
+    launch_refit<true>(c, na, theta2);
+  else
+    k_hleaves<<<1, BS, 0, s>>>(N, c->pf[F_H], theta2, na, c->visit);
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
 }

@@ -45,6 +45,24 @@ def test_tree_matches_twin_large(n):
     assert relmax(a["quad"], t["quad"]) < 1e-14
 
 
+@pytest.mark.parametrize("n", [1025, 70000])
+def test_h_refit_matches_twin_large(n):
+    """After the h-rho pass, the smoothing-length refit (all three phases) leaves every node's
+    hmax equal to that of a tree built from scratch with the converged h; the moments are
+    untouched."""
+    g = cloud(n, 11)
+    rng = np.random.default_rng(6)
+    s = gpu_sim(g)
+    s.set("h", 0.02 + 0.03 * rng.random(n))
+    s.build_tree()
+    s.density()
+    a = s.tree_arrays()
+    p = O.params_from_dict(g["params"])
+    t = O.TreeTwin(p, g["r"], g["m"], s.get("h")).arrays()
+    for k in ("perm", "left", "right", "mass", "com", "bbmin", "bbmax", "hmax"):
+        assert np.array_equal(a[k], t[k]), k
+
+
 def test_gravity_matches_twin_large():
     """65536 particles after setup: per-group interaction counts bit-exact, tree forces and
     potential to rounding (the hydro pass's softened neighbour terms plus the walk's direct
