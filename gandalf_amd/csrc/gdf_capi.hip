@@ -604,14 +604,16 @@ int64_t gdf_neighbour_sets(gdf_ctx* c, int mode, int64_t* offsets, int32_t* ids,
   for (int g = 0; g < G; g++) {
     const size_t base = (size_t) g*capb*64;
     const int nc = ncand[g];
+    const int nbat = (nc + 63)/64;
     for (int l = 0; l < gcount[g]; l++) {
       const int s = gfirst[g] + l;
       const double r2i = (krsqd*hh[s])*hh[s];
       auto& out = sets[io[s]];
+      // candidate k (walk order) sits in batch k % nbat at bit k / nbat
       for (int k = 0; k < nc; k++) {
-        const int b = k/64, q = k%64;
+        const int b = k%nbat, q = k/nbat;
         if (!((cmask[base + (size_t) b*64 + l] >> q) & 1ULL)) continue;
-        const int j = cidx[base + (size_t) b*64 + q];
+        const int j = cidx[base + (size_t) k];
         if (mode == 0) {
           double dx = x[j] - x[s], dy = y[j] - y[s], dz = z[j] - z[s];
           if (c->phys.periodic) {

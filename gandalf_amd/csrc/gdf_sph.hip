@@ -7,14 +7,17 @@
 //
 // Work mapping: one wavefront per walk group (a tree node of <= 64 Morton-consecutive
 // particles, one target per lane).  A walk pass resolves the group's candidate particles from
-// the tree once (all particles of buckets that come within the group's search range) and, for
-// every batch of 64 candidates, records one 64-bit mask per target holding the candidates that
-// are inside that target's own range (the exact neighbour test).  The pair passes stage each
-// batch's candidate records in LDS with one coalesced load per lane, shared by all 64 targets,
-// and every lane walks the set bits of its own mask.  The h-rho pass re-sweeps the same batches
-// until every target of the group has converged.  Set bits are visited in candidate order, so
-// each target sums its neighbours in the order the reference's neighbour list would hold them
-// after the same walk.
+// the tree once (all particles of buckets that come within the group's search range, in walk
+// order) and, for every batch of up to 64 candidates, records one 64-bit mask per target
+// holding the candidates that are inside that target's own range (the exact neighbour test).
+// Batch b of nbat takes the candidates at walk positions b, b + nbat, b + 2 nbat, ...: every
+// batch then samples the whole candidate region, each target's neighbours spread evenly over
+// the batches, and the per-target mask loops below leave fewer lanes idle than with batches of
+// consecutive (spatially clustered) candidates.  The pair passes stage each batch's candidate
+// records in LDS with one load per lane, shared by all 64 targets, and every lane walks the set
+// bits of its own mask.  The h-rho pass re-sweeps the same batches until every target of the
+// group has converged.  A target sums its neighbours batch by batch, in bit order within a
+// batch.
 #include <hip/hip_runtime.h>
 
 #include "gdf_internal.h"
@@ -24,7 +27,6 @@ namespace gdf {
 namespace {
 
 constexpr int BS = 256;
-constexpr int CANDCAP = 1024;              // ring buffer of candidate slots (power of two)
 constexpr double NODE_SLACK = 1.0 + 1e-12; // node tests err on the inclusive side
 constexpr int kInitBatches = 16;           // initial candidate capacity per group, in batches
 
@@ -49,7 +51,6 @@ struct CandWalk {
 // decides it.  PER: periodic nearest images.
 template <int MODE, bool PER>
 __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
-  __shared__ int cand[CANDCAP];
   __shared__ double4 sc[MODE == 2 ? 1 : kWave];   // MODE 0/1: (x, y, z, (k h_j)^2)
   __shared__ float4 fc[MODE == 2 ? kWave : 1];    // MODE 2: group-frame (x, y, z, (k h_j)^2)
   __shared__ int lstk[kLdsStack];    // node stack; deeper entries go to the global region
@@ -83,11 +84,12 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
     const float r2if = (float) r2i;
     int* idx = a.idx + (size_t) g*a.capb*kWave;
     unsigned long long* msk = a.mask + (size_t) g*a.capb*kWave;
-    int nbat = 0;
-    int sp = 1, head = 0, tail = 0;
+    const int cap = a.capb*kWave;
+    int sp = 1, tail = 0;
     bool overflow = false;
     if (lane == 0) lstk[0] = a.grp_root ? a.grp_root[g] : 0;   // the group's segment subtree
     __syncthreads();
+    // 1. traversal: the particles of every bucket within range, in walk order, into idx
     while (sp > 0) {
       const int b = min(sp, kWave);
       int node = -1;
@@ -126,61 +128,62 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
       const int ecount = emit ? ec : 0;
       const int incl = wave_incl_scan(ecount);
       const int tot = __shfl(incl, 63, 64);
-      for (int k = 0; k < ecount; k++) cand[(tail + incl - ecount + k) & (CANDCAP - 1)] = ef + k;
-      tail += tot;
-      __syncthreads();
-      // one batch of 64 candidates: stage, test every target, record index and masks
-      while (tail - head >= kWave || (sp == 0 && tail > head)) {
-        const int nb = min(tail - head, kWave);
-        int j = 0;
-        if (lane < nb) {
-          j = cand[(head + lane) & (CANDCAP - 1)];
-          const double r2j = MODE == 0 ? 0.0 : range2_rn(krsqd, a.hcand[j]);
-          if (MODE == 2) {
-            double dx = a.x[j] - gc.x, dy = a.y[j] - gc.y, dz = a.z[j] - gc.z;
-            nearest_image<double>(pp, dx, dy, dz);
-            fc[lane] = make_float4((float) dx, (float) dy, (float) dz, (float) r2j);
-          } else {
-            sc[lane] = make_double4(a.x[j], a.y[j], a.z[j], r2j);
-          }
-        }
-        __syncthreads();
-        unsigned long long m = 0;
-        if (has) {
-          for (int q = 0; q < nb; q++) {
-            bool in;
-            if (MODE == 2) {
-              const float4 c = fc[q];
-              const float d2 = dot3_f(c.x - rxi, c.y - ryi, c.z - rzi);
-              in = (d2 < r2if) | (d2 < c.w);
-            } else {
-              const double4 c = sc[q];
-              double dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
-              nearest_image<double>(pp, dx, dy, dz);
-              const double d2 = dot3_rn(dx, dy, dz);
-              if (MODE == 0) in = d2 <= r2i;
-              else in = (d2 < r2i) | (d2 < c.w);
-            }
-            m |= (unsigned long long) in << q;
-          }
-        }
-        if (nbat < a.capb) {
-          if (lane < nb) idx[nbat*kWave + lane] = j;
-          msk[nbat*kWave + lane] = m;
-        }
-        nbat++;
-        head += nb;
-        __syncthreads();
+      for (int k = 0; k < ecount; k++) {
+        const int slot = tail + incl - ecount + k;
+        if (slot < cap) idx[slot] = ef + k;
       }
+      tail += tot;
     }
     if (overflow) {
       if (lane == 0) atomicOr(&a.flags[2], 1);
       continue;
     }
-    if (lane == 0) {
-      a.ncand[g] = tail;
-      if (nbat > a.capb) atomicMax(&a.flags[1], nbat);
+    const int nbat = (tail + kWave - 1)/kWave;
+    if (tail > cap) {                // candidates beyond the set's capacity: grow and repeat
+      if (lane == 0) { a.ncand[g] = tail; atomicMax(&a.flags[1], nbat); }
+      continue;
     }
+    __threadfence_block();
+    // 2. masks: batch b holds the candidates at walk positions b, b + nbat, b + 2 nbat, ...,
+    // so every batch samples the whole candidate region and each target's neighbours spread
+    // evenly over the batches (fewer idle lanes in the per-target mask loops of the pair passes)
+    for (int b = 0; b < nbat; b++) {
+      const int nb = (tail - b + nbat - 1)/nbat;
+      if (lane < nb) {
+        const int j = idx[b + lane*nbat];
+        const double r2j = MODE == 0 ? 0.0 : range2_rn(krsqd, a.hcand[j]);
+        if (MODE == 2) {
+          double dx = a.x[j] - gc.x, dy = a.y[j] - gc.y, dz = a.z[j] - gc.z;
+          nearest_image<double>(pp, dx, dy, dz);
+          fc[lane] = make_float4((float) dx, (float) dy, (float) dz, (float) r2j);
+        } else {
+          sc[lane] = make_double4(a.x[j], a.y[j], a.z[j], r2j);
+        }
+      }
+      __syncthreads();
+      unsigned long long m = 0;
+      if (has) {
+        for (int q = 0; q < nb; q++) {
+          bool in;
+          if (MODE == 2) {
+            const float4 c = fc[q];
+            const float d2 = dot3_f(c.x - rxi, c.y - ryi, c.z - rzi);
+            in = (d2 < r2if) | (d2 < c.w);
+          } else {
+            const double4 c = sc[q];
+            double dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+            nearest_image<double>(pp, dx, dy, dz);
+            const double d2 = dot3_rn(dx, dy, dz);
+            if (MODE == 0) in = d2 <= r2i;
+            else in = (d2 < r2i) | (d2 < c.w);
+          }
+          m |= (unsigned long long) in << q;
+        }
+      }
+      msk[b*kWave + lane] = m;
+      __syncthreads();
+    }
+    if (lane == 0) a.ncand[g] = tail;
   }
 }
 
@@ -273,9 +276,9 @@ __global__ __launch_bounds__(64) void k_density(DensArgs a) {
     double r_ = 0.0, om_ = 0.0, z_ = 0.0;
     KahanF kr, kom, kz;
     for (int b = 0; b < nbat; b++) {
-      const int nb = min(nc - b*kWave, kWave);
+      const int nb = (nc - b + nbat - 1)/nbat;   // candidates b, b + nbat, ...
       if (lane < nb) {
-        const double4 pj = a.posm[idx[b*kWave + lane]];
+        const double4 pj = a.posm[idx[b + lane*nbat]];
         if (sizeof(T) == 8) {
           sd[lane] = pj;
         } else {
@@ -438,9 +441,9 @@ __global__ __launch_bounds__(64) void k_hydro(HydroArgs a) {
   const int* idx = a.idx + (size_t) g*a.capb*kWave;
   const unsigned long long* msk = a.mask + (size_t) g*a.capb*kWave;
   for (int b = 0; b < nbat; b++) {
-    const int nb = min(nc - b*kWave, kWave);
+    const int nb = (nc - b + nbat - 1)/nbat;   // candidates b, b + nbat, ...
     if (lane < nb) {
-      const int j = idx[b*kWave + lane];
+      const int j = idx[b + lane*nbat];
       sP[lane] = a.posm[j]; sA[lane] = a.A[j]; sB[lane] = a.B[j];
       if (VAR == 1) sZ[lane] = a.zeta[j];
       if (BLK) { sJ[lane] = j; sL[lane] = blk_record(a, j); }
@@ -550,9 +553,9 @@ __global__ __launch_bounds__(64) void k_hydro_mixed(HydroArgs a) {
   const int* idx = a.idx + (size_t) g*a.capb*kWave;
   const unsigned long long* msk = a.mask + (size_t) g*a.capb*kWave;
   for (int b = 0; b < nbat; b++) {
-    const int nb = min(nc - b*kWave, kWave);
+    const int nb = (nc - b + nbat - 1)/nbat;   // candidates b, b + nbat, ...
     if (lane < nb) {
-      const int j = idx[b*kWave + lane];
+      const int j = idx[b + lane*nbat];
       const double4 pj = a.posm[j], Aj = a.A[j], Bj = a.B[j];
       double dx = pj.x - gc.x, dy = pj.y - gc.y, dz = pj.z - gc.z;
       nearest_image<double>(p, dx, dy, dz);

@@ -651,6 +651,37 @@ __global__ void k_groups_fill(int N, const int* __restrict__ nsel, const int* __
   gcount[g] = n >= N - 1 ? 1 : last[n] - first[n] + 1;
 }
 
+// Order-preserving compaction of n byte flags into the indices of the set ones (count at
+// out[cap]) by one workgroup.  No look-back between workgroups: a look-back scan launched while
+// another stream's kernels fill the device stalls until its predecessor workgroups find a
+// compute unit (milliseconds for a few thousand elements), while this pass needs one CU.
+constexpr int kCompactThreads = 1024;
+__global__ __launch_bounds__(kCompactThreads) void k_compact_flags(
+    int n, const unsigned char* __restrict__ flags, int* __restrict__ out, int cap) {
+  constexpr int nw = kCompactThreads/kWave;
+  __shared__ int wbase[nw];
+  __shared__ int total;
+  const int t = threadIdx.x, lane = t % kWave, w = t/kWave;
+  const int per = (n + kCompactThreads - 1)/kCompactThreads;
+  const int i0 = min(t*per, n), i1 = min(i0 + per, n);
+  int cnt = 0;
+  for (int i = i0; i < i1; i++) cnt += flags[i] != 0;
+  const int incl = wave_incl_scan(cnt);
+  if (lane == kWave - 1) wbase[w] = incl;
+  __syncthreads();
+  if (w == 0) {
+    const int v = lane < nw ? wbase[lane] : 0;
+    const int vi = wave_incl_scan(v);
+    if (lane < nw) wbase[lane] = vi - v;
+    if (lane == nw - 1) total = vi;
+  }
+  __syncthreads();
+  int o = wbase[w] + incl - cnt;
+  for (int i = i0; i < i1; i++)
+    if (flags[i]) out[o++] = i;
+  if (t == 0) out[cap] = total;
+}
+
 // one wavefront per walk group: does any of its particles have status 1?
 __global__ void k_group_flags(int ngroups, const int* __restrict__ grp_first,
                               const int* __restrict__ grp_count, const int* __restrict__ status,
@@ -839,9 +870,7 @@ int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, i
   constexpr int gpb = BS/kWave;   // groups per block
   k_group_flags<<<(ng + gpb - 1)/gpb, BS, 0, s>>>(ng, c->grp_first, c->grp_count, status, flags,
                                                   ncand, gnc, gnn);
-  size_t bytes = c->sel_tmp_bytes;
-  GDF_HIP(c, rocprim::select(c->sel_tmp, bytes, rocprim::counting_iterator<int>(0),
-                             (const unsigned char*) flags, list, list + c->Ncap, (size_t) ng, s));
+  k_compact_flags<<<1, kCompactThreads, 0, s>>>(ng, flags, list, c->Ncap);
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
 }
