@@ -337,6 +337,22 @@ __device__ __forceinline__ void wave_add_ll(long long* dst, long long v) {
   if (__lane_id() == 0 && v != 0) atomicAdd((unsigned long long*) dst, (unsigned long long) v);
 }
 
+// The same over a whole workgroup (blockDim.x a multiple of 64, at most 1024; every thread
+// calls it): one atomic per workgroup.  Atomics on one address from every wavefront of a
+// million-particle pass serialise at the memory side (about 0.2 ms for 16384 of them).
+__device__ __forceinline__ void block_add_ll(long long* dst, long long v) {
+  __shared__ long long part[16];
+  #pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if (__lane_id() == 0) part[threadIdx.x/64] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int w = 0; w < (int) blockDim.x/64; w++) t += part[w];
+    if (t != 0) atomicAdd((unsigned long long*) dst, (unsigned long long) t);
+  }
+}
+
 // GradhSph::ComputeThermalProperties (GradhSph.cpp:335-347) with the adiabatic
 // (AdiabaticEOS.cpp), isothermal (IsothermalEOS.cpp) and barotropic (BarotropicEOS.cpp) closures:
 // u (replaced for the last two), sound speed and pressure from rho and u

@@ -441,17 +441,17 @@ __global__ void k_block_init(BlkArgs b) {
 // drift every particle from the start of its own step; active = at the end of its step
 __global__ void k_block_advance(BlkArgs b) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x*BS + threadIdx.x;
-  long long a = 0;
-  if (i < b.N) {
+  long long na = 0;
+  for (int i = blockIdx.x*BS + threadIdx.x; i < b.N; i += gridDim.x*BS) {
     const double dt = b.t - b.tlast[i];
     for (int k = 0; k < 3; k++) b.r[k][i] = b.r0[k][i] + b.v0[k][i]*dt + 0.5*b.a0[k][i]*dt*dt;
     for (int k = 0; k < 3; k++) b.v[k][i] = b.v0[k][i] + b.a0[k][i]*dt;
     if (b.energy) b.u[i] = b.u0[i] + b.dudt0[i]*dt;
-    a = (b.n - b.nlast[i] == b.nstep[i]) ? 1 : 0;
-    b.act[i] = (int) a;
+    const int a = (b.n - b.nlast[i] == b.nstep[i]) ? 1 : 0;
+    b.act[i] = a;
+    na += a;
   }
-  wave_add_ll(reinterpret_cast<long long*>(b.counter), a);   // active particles of the step
+  block_add_ll(reinterpret_cast<long long*>(b.counter), na);   // active particles of the step
 }
 
 // Sph::ZeroAccelerations (Sph.cpp:126-140) for the active particles
@@ -478,9 +478,8 @@ __global__ void k_eos_all(int N, PhysParams p, const double* __restrict__ rho,
 // SphLeapfrogKDK::CheckTimesteps: end a step early where a neighbour's level is too far above;
 // every particle's active flag is rewritten (set only for these), counter += their number
 __global__ void k_check_timesteps(BlkArgs b) {
-  const int i = blockIdx.x*BS + threadIdx.x;
-  int newly = 0;
-  if (i < b.N) {
+  long long newly = 0;
+  for (int i = blockIdx.x*BS + threadIdx.x; i < b.N; i += gridDim.x*BS) {
     int a = 0;
     const int dn = b.n - b.nlast[i];
     const int level = b.level[i], ln = b.levelneib[i];
@@ -494,10 +493,9 @@ __global__ void k_check_timesteps(BlkArgs b) {
       }
     }
     b.act[i] = a;
-    newly = a;
+    newly += a;
   }
-  const long long tot = newly;
-  wave_add_ll(reinterpret_cast<long long*>(b.counter), tot);
+  block_add_ll(reinterpret_cast<long long*>(b.counter), newly);
 }
 
 // resynchronisation branch: every particle gets a fresh level from its timestep
@@ -516,9 +514,8 @@ __global__ void k_block_resync(BlkArgs b) {
 
 // particles at the end of their step choose their next level; counter = max level of all
 __global__ void k_block_update(BlkArgs b) {
-  const int i = blockIdx.x*BS + threadIdx.x;
   int lv = 0;
-  if (i < b.N) {
+  for (int i = blockIdx.x*BS + threadIdx.x; i < b.N; i += gridDim.x*BS) {
     const int n = b.n;
     int level = b.level[i];
     const int nstep = b.nstep[i], nlast = b.nlast[i];
@@ -541,13 +538,17 @@ __global__ void k_block_update(BlkArgs b) {
       b.dtnext[i] = ns*b.timestep;
       b.end[i] = 1;
     }
-    lv = level;
+    lv = max(lv, level);
   }
+  // one atomic per workgroup
+  __shared__ int part[BS/64];
   lv = wave_max_int(lv);
-  // nearly every wave holds the same maximum: read before the atomic so that only a wave that
-  // raises the maximum contends for the counter (the value only grows)
-  if (__lane_id() == 0 && __hip_atomic_load(b.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < lv)
+  if (__lane_id() == 0) part[threadIdx.x/64] = lv;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BS/64; w++) lv = max(lv, part[w]);
     atomicMax(b.counter, lv);
+  }
 }
 
 // integer time rescaled when level_max changes: mul = 1 multiplies nstep and nlast by f, else
@@ -579,10 +580,10 @@ struct BlkEndArgs {
 
 __global__ void k_block_end_step(BlkEndArgs e) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x*BS + threadIdx.x;
   long long fin = 0;
-  if (i < e.N && e.end[i]) {
-    fin = 1;
+  for (int i = blockIdx.x*BS + threadIdx.x; i < e.N; i += gridDim.x*BS) {
+    if (!e.end[i]) continue;
+    fin++;
     const double dt = e.pdt[i];
     for (int k = 0; k < 3; k++) {
       const double a = e.a[k][i], a0 = e.a0[k][i];
@@ -607,8 +608,11 @@ __global__ void k_block_end_step(BlkEndArgs e) {
     e.act[i] = 0;
     e.end[i] = 0;
   }
-  wave_add_ll(e.finished, fin);
+  block_add_ll(e.finished, fin);
 }
+
+// grid of the passes that count into one counter: at most 1024 workgroups, one atomic each
+int count_grid(int n) { return std::min(nblocks(n, BS), 1024); }
 
 BlkArgs blk_args(gdf_ctx* c) {
   BlkArgs b;
@@ -656,7 +660,7 @@ int block_init(gdf_ctx* c) {
 int block_advance(gdf_ctx* c) {
   GDF_TRY(zero_counter(c));
   BlkArgs b = blk_args(c);
-  k_block_advance<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
+  k_block_advance<<<count_grid(c->N), BS, 0, c->stream>>>(b);
   GDF_HIP(c, hipGetLastError());
   // read with the tree build's synchronisation
   GDF_HIP(c, hipMemcpyAsync(c->h_counters + 12, c->counters + 12, sizeof(int)*2,
@@ -685,7 +689,7 @@ int block_eos_all(gdf_ctx* c) {
 int block_check_timesteps(gdf_ctx* c, int* activecount) {
   GDF_TRY(zero_counter(c));
   BlkArgs b = blk_args(c);
-  k_check_timesteps<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
+  k_check_timesteps<<<count_grid(c->N), BS, 0, c->stream>>>(b);
   GDF_HIP(c, hipGetLastError());
   return read_counter(c, activecount);
 }
@@ -709,7 +713,7 @@ int block_timesteps(gdf_ctx* c) {
   const int level_max_old = c->level_max;
   GDF_TRY(zero_counter(c));
   BlkArgs b = blk_args(c);
-  k_block_update<<<nblocks(c->N, BS), BS, 0, c->stream>>>(b);
+  k_block_update<<<count_grid(c->N), BS, 0, c->stream>>>(b);
   GDF_HIP(c, hipGetLastError());
   int level_max = 0;
   GDF_TRY(read_counter(c, &level_max));
@@ -748,7 +752,7 @@ int block_end_step(gdf_ctx* c) {
   e.tlast = c->pf[F_TLAST]; e.pdt = c->pf[F_PDT]; e.dtnext = c->pf[F_DTNEXT];
   e.nlast = c->pi[I_NLAST]; e.act = c->pi[I_ACT]; e.end = c->pi[I_END];
   e.finished = c->icount + 6;
-  k_block_end_step<<<nblocks(c->N, BS), BS, 0, c->stream>>>(e);
+  k_block_end_step<<<count_grid(c->N), BS, 0, c->stream>>>(e);
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
 }
