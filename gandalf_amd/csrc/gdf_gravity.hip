@@ -548,7 +548,7 @@ int gravity_launch(gdf_ctx* c) {
     return fail(c, GDF_ERR_INVALID, "gravity walk beside the hydro pass needs hydro + self-gravity");
   const int N = c->N;
   // on the second stream beside the scatter-gather walk and the hydro pass: measured faster
-  // than running the two in turn, both at 1M active particles and in sparse block steps
+  // than running them in turn, both at 1M active particles and in sparse block steps
   hipStream_t s2 = c->stream2;
   GravArgs a = grav_args(c);
   GDF_TRY(grav_group_list(c, a));
@@ -561,7 +561,9 @@ int gravity_launch(gdf_ctx* c) {
   a.gwp = c->df[D_GWP]; a.gwp0 = c->df[D_GWP0];
   k_zero8<<<1, 64, 0, s2>>>(c->counters + 24);
   k_zero8<<<1, 64, 0, s2>>>(c->counters + 16);
-  const int grid = std::min(c->ngroups, kWalkSlots);
+  // half the persistent wavefronts of a lone walk: at the full count they fill every CU's LDS
+  // and the scatter-gather walk beside them waits for the gravity walk to drain
+  const int grid = std::min(c->ngroups, kWalkSlots/2);
   if (c->timing) hipEventRecord(c->ev[7], s2);
   if (c->cfg.precision == GDF_PREC_MIXED) k_gravity<true, false, false><<<grid, kWave, 0, s2>>>(a);
   else k_gravity<false, false, false><<<grid, kWave, 0, s2>>>(a);

@@ -651,35 +651,30 @@ __global__ void k_groups_fill(int N, const int* __restrict__ nsel, const int* __
   gcount[g] = n >= N - 1 ? 1 : last[n] - first[n] + 1;
 }
 
-// Order-preserving compaction of n byte flags into the indices of the set ones (count at
-// out[cap]) by one workgroup.  No look-back between workgroups: a look-back scan launched while
-// another stream's kernels fill the device stalls until its predecessor workgroups find a
-// compute unit (milliseconds for a few thousand elements), while this pass needs one CU.
-constexpr int kCompactThreads = 1024;
-__global__ __launch_bounds__(kCompactThreads) void k_compact_flags(
-    int n, const unsigned char* __restrict__ flags, int* __restrict__ out, int cap) {
-  constexpr int nw = kCompactThreads/kWave;
-  __shared__ int wbase[nw];
-  __shared__ int total;
-  const int t = threadIdx.x, lane = t % kWave, w = t/kWave;
-  const int per = (n + kCompactThreads - 1)/kCompactThreads;
-  const int i0 = min(t*per, n), i1 = min(i0 + per, n);
+// Order-preserving compaction of n byte flags (0 or 1) into the indices of the set ones
+// (count at out[cap]) by a single wavefront without LDS.  It may run while another stream's
+// kernels fill the device: a look-back scan then stalls until its predecessor workgroups find
+// a compute unit, and any workgroup that needs LDS waits until enough LDS is free on one CU
+// (both measured at up to 2-4 ms beside the gravity walk); one LDS-free wavefront starts at
+// once.  Each lane takes a contiguous run of 4-byte words (the flag buffer is padded to a
+// multiple of 4 bytes).
+__global__ __launch_bounds__(64) void k_compact_flags(int n, const unsigned* __restrict__ words,
+                                                      int* __restrict__ out, int cap) {
+  const int lane = threadIdx.x;
+  const int nword = (n + 3)/4;
+  const int per = (nword + kWave - 1)/kWave;
+  const int w0 = min(lane*per, nword), w1 = min(w0 + per, nword);
   int cnt = 0;
-  for (int i = i0; i < i1; i++) cnt += flags[i] != 0;
+  for (int w = w0; w < w1; w++) cnt += __popc(words[w]);   // bytes are 0 or 1
   const int incl = wave_incl_scan(cnt);
-  if (lane == kWave - 1) wbase[w] = incl;
-  __syncthreads();
-  if (w == 0) {
-    const int v = lane < nw ? wbase[lane] : 0;
-    const int vi = wave_incl_scan(v);
-    if (lane < nw) wbase[lane] = vi - v;
-    if (lane == nw - 1) total = vi;
+  int o = incl - cnt;
+  for (int w = w0; w < w1; w++) {
+    const unsigned x = words[w];
+    if (!x) continue;
+    for (int k = 0; k < 4; k++)
+      if ((x >> (8*k)) & 1u) out[o++] = 4*w + k;
   }
-  __syncthreads();
-  int o = wbase[w] + incl - cnt;
-  for (int i = i0; i < i1; i++)
-    if (flags[i]) out[o++] = i;
-  if (t == 0) out[cap] = total;
+  if (lane == kWave - 1) out[cap] = incl;
 }
 
 // one wavefront per walk group: does any of its particles have status 1?
@@ -694,6 +689,9 @@ __global__ void k_group_flags(int ngroups, const int* __restrict__ grp_first,
   const bool any = __ballot(on) != 0;
   if (lane == 0) {
     flags[g] = any ? 1 : 0;
+    // the compaction reads whole 4-byte words: the bytes after the last flag read 0
+    if (g == ngroups - 1)
+      for (int k = ngroups; k & 3; k++) flags[k] = 0;
     if (!any) {
       if (ncand) ncand[g] = 0;
       if (gnc) { gnc[g] = 0; gnn[g] = 0; }
@@ -870,7 +868,8 @@ int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, i
   constexpr int gpb = BS/kWave;   // groups per block
   k_group_flags<<<(ng + gpb - 1)/gpb, BS, 0, s>>>(ng, c->grp_first, c->grp_count, status, flags,
                                                   ncand, gnc, gnn);
-  k_compact_flags<<<1, kCompactThreads, 0, s>>>(ng, flags, list, c->Ncap);
+  k_compact_flags<<<1, kWave, 0, s>>>(ng, reinterpret_cast<const unsigned*>(flags), list,
+                                      c->Ncap);
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
 }
