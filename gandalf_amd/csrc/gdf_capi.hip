@@ -235,6 +235,35 @@ int ctx_field_width(gdf_ctx* c, const char* name) {
 
 using namespace gdf;
 
+namespace gdf {
+
+int ktime_begin(gdf_ctx* c, hipStream_t s, int kind) {
+  if (!c->timing) return -1;
+  if (c->nkev == gdf_ctx::kKevPairs) ktime_collect(c);
+  const int k = c->nkev++;
+  c->kev_kind[k] = kind;
+  hipEventRecord(c->kev[2*k], s);
+  return k;
+}
+
+void ktime_end(gdf_ctx* c, hipStream_t s, int slot) {
+  if (slot >= 0) hipEventRecord(c->kev[2*slot + 1], s);
+}
+
+void ktime_collect(gdf_ctx* c) {
+  for (int k = 0; k < c->nkev; k++) {
+    float ms = 0.f;
+    if (hipEventSynchronize(c->kev[2*k + 1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, c->kev[2*k], c->kev[2*k + 1]) == hipSuccess) {
+      if (c->kev_kind[k] == 5) { c->pass_ms[5] += ms; c->grav_launches++; }
+      else { c->pass_ms[6] += ms; c->hydro_launches++; }
+    }
+  }
+  c->nkev = 0;
+}
+
+}  // namespace gdf
+
 extern "C" {
 
 int gdf_config_default(gdf_config* cfg) {
@@ -325,6 +354,7 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
       rc = GDF_ERR_HIP;
     hipMemset(c->icount, 0, sizeof(long long)*8);
     for (int e = 0; e < 12; e++) hipEventCreate(&c->ev[e]);
+    for (int e = 0; e < 2*gdf_ctx::kKevPairs; e++) hipEventCreate(&c->kev[e]);
   }
   if (rc != GDF_OK) { gdf_destroy(c); return rc; }
   *out = c;
@@ -345,6 +375,7 @@ void gdf_destroy(gdf_ctx* c) {
   if (c->h_counters) hipHostFree(c->h_counters);
   if (c->h_red) hipHostFree(c->h_red);
   for (int e = 0; e < 12; e++) if (c->ev[e]) hipEventDestroy(c->ev[e]);
+  for (int e = 0; e < 2*gdf_ctx::kKevPairs; e++) if (c->kev[e]) hipEventDestroy(c->kev[e]);
   if (c->stream2) hipStreamSynchronize(c->stream2);
   if (c->ev_dens) hipEventDestroy(c->ev_dens);
   if (c->ev_grav) hipEventDestroy(c->ev_grav);
@@ -527,6 +558,7 @@ int gdf_gravity_group_counts(gdf_ctx* c, int32_t* ncell, int32_t* nnear) {
 
 int gdf_pass_times(gdf_ctx* c, double* out9) {
   if (!c || !out9) return GDF_ERR_INVALID;
+  gdf::ktime_collect(c);
   for (int k = 0; k < 6; k++) out9[k] = c->pass_ms[k];
   out9[6] = (double) c->grav_launches;
   out9[7] = c->pass_ms[6];
@@ -536,6 +568,7 @@ int gdf_pass_times(gdf_ctx* c, double* out9) {
 
 int gdf_set_timing(gdf_ctx* c, int enable) {
   if (!c) return GDF_ERR_INVALID;
+  gdf::ktime_collect(c);
   c->timing = enable != 0;
   return GDF_OK;
 }

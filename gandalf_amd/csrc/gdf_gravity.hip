@@ -564,11 +564,11 @@ int gravity_launch(gdf_ctx* c) {
   // half the persistent wavefronts of a lone walk: at the full count they fill every CU's LDS
   // and the scatter-gather walk beside them waits for the gravity walk to drain
   const int grid = std::min(c->ngroups, kWalkSlots/2);
-  if (c->timing) hipEventRecord(c->ev[7], s2);
+  const int kt = ktime_begin(c, s2, 5);
   if (c->cfg.precision == GDF_PREC_MIXED) k_gravity<true, false, false><<<grid, kWave, 0, s2>>>(a);
   else k_gravity<false, false, false><<<grid, kWave, 0, s2>>>(a);
   GDF_HIP(c, hipGetLastError());
-  if (c->timing) hipEventRecord(c->ev[8], s2);
+  ktime_end(c, s2, kt);
   GDF_HIP(c, hipMemcpyAsync(c->h_counters + 16, c->counters + 16, sizeof(int)*8,
                             hipMemcpyDeviceToHost, s2));
   GDF_HIP(c, hipEventRecord(c->ev_grav, s2));
@@ -586,12 +586,6 @@ int gravity_join(gdf_ctx* c) {
   k_grav_join<<<nblocks(c->N, 256), 256, 0, c->stream>>>(j);
   GDF_HIP(c, hipGetLastError());
   GDF_HIP(c, hipEventSynchronize(c->ev_grav));
-  if (c->timing) {
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, c->ev[7], c->ev[8]);
-    c->pass_ms[5] += ms;
-    c->grav_launches++;
-  }
   if (c->h_counters[16 + 2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in gravity pass");
   return GDF_OK;
 }
@@ -607,7 +601,7 @@ int gravity_walk(gdf_ctx* c) {
   k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
   k_zero8<<<1, 64, 0, s>>>(c->counters);
   const int grid = std::min(c->ngroups, kWalkSlots);
-  if (c->timing) hipEventRecord(c->ev[7], s);
+  const int kt = ktime_begin(c, s, 5);
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
   if (c->phys.nbody) {
     if (mixed) k_gravity<true, true, true><<<grid, kWave, 0, s>>>(a);
@@ -620,15 +614,9 @@ int gravity_walk(gdf_ctx* c) {
     else k_gravity<false, true, false><<<grid, kWave, 0, s>>>(a);
   }
   GDF_HIP(c, hipGetLastError());
-  if (c->timing) hipEventRecord(c->ev[8], s);
+  ktime_end(c, s, kt);
   GDF_HIP(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int)*8, hipMemcpyDeviceToHost, s));
   GDF_HIP(c, hipStreamSynchronize(s));
-  if (c->timing) {
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, c->ev[7], c->ev[8]);
-    c->pass_ms[5] += ms;             // summed over the passes of one step
-    c->grav_launches++;
-  }
   if (c->h_counters[2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in gravity pass");
   return GDF_OK;
 }

@@ -178,12 +178,18 @@ struct gdf_ctx {
 
   // timing
   bool timing = false;
-  hipEvent_t ev[12] = {};      // [9], [10] bracket the hydro pair kernel
+  hipEvent_t ev[12] = {};      // pass-boundary events
   double pass_ms[7] = {};       // ... [5] gravity walk kernel, [6] hydro pair kernel
-  bool mark_passes = true;
+  bool mark_passes = true;     // record the pass-boundary events (first force pass of a step)
   int grav_launches = 0;       // gravity walk launches in the current step
   long long n_active = 0;      // particles the current force pass computes (block steps: active)
-  int hydro_launches = 0;      // hydro pair-kernel launches in the current step     // record the pass-boundary events (first force pass of a step)
+  int hydro_launches = 0;      // hydro pair-kernel launches in the current step
+  // pair-kernel launches bracketed by event pairs, read back at the end of the step (or when
+  // the pairs run out) rather than with a host wait after every launch
+  static constexpr int kKevPairs = 16;
+  hipEvent_t kev[2*kKevPairs] = {};
+  int kev_kind[kKevPairs] = {};
+  int nkev = 0;
   long long icounts_last[6] = {};
 };
 
@@ -206,6 +212,12 @@ int tree_build(gdf_ctx* c);
 int tree_refit_h(gdf_ctx* c);
 // compacted list of the walk groups holding a particle with status[i] == 1 (ascending, count at
 // list[Ncap]); the optional per-group outputs of every other group are cleared
+// kernel timing (ctx->timing): kind 5 gravity walk, 6 hydro pair kernel.  begin returns the
+// event pair's slot (-1 with timing off); collect waits for the recorded pairs and adds them
+// to pass_ms and the launch counts
+int ktime_begin(gdf_ctx* c, hipStream_t s, int kind);
+void ktime_end(gdf_ctx* c, hipStream_t s, int slot);
+void ktime_collect(gdf_ctx* c);
 int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, int* gnn);
 TreeView tree_view(const gdf_ctx* c);
 

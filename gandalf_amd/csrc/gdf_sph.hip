@@ -810,7 +810,7 @@ int sph_hydro(gdf_ctx* c, int variant) {
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
   const bool blk = c->nlevels > 1;
   const int G = c->ngroups;
-  if (c->timing) hipEventRecord(c->ev[9], s);
+  const int kt = ktime_begin(c, s, 6);
   if (variant == 0) {
     if (blk) {
       if (mixed) k_hydro_mixed<0, true><<<G, kWave, 0, s>>>(h);
@@ -829,14 +829,7 @@ int sph_hydro(gdf_ctx* c, int variant) {
     }
   }
   GDF_HIP(c, hipGetLastError());
-  if (c->timing) {
-    hipEventRecord(c->ev[10], s);
-    hipEventSynchronize(c->ev[10]);
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, c->ev[9], c->ev[10]);
-    c->pass_ms[6] += ms;
-    c->hydro_launches++;
-  }
+  ktime_end(c, s, kt);
   return GDF_OK;
 }
 

@@ -160,6 +160,7 @@ int GradhSphSimulationGpu<ndim>::MainLoop() {
   c->grav_launches = 0;
   c->pass_ms[6] = 0.0;
   c->hydro_launches = 0;
+  c->nkev = 0;                 // kernel timings of earlier passes belong to no step
   c->nsteps++;
   const double dtstep = c->timestep;
   c->t = c->t + c->timestep;
@@ -191,6 +192,7 @@ template <int ndim>
 void GradhSphSimulationGpu<ndim>::FinishStepStats() {
   gdf_ctx* c = ctx;
   if (c->timing) {
+    gdf::ktime_collect(c);
     float ms = 0.f;
     // tree, density, hydro, gravity, integrate
     hipEventElapsedTime(&ms, c->ev[1], c->ev[2]); c->pass_ms[0] = ms;
@@ -213,6 +215,7 @@ int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
   c->grav_launches = 0;
   c->pass_ms[6] = 0.0;
   c->hydro_launches = 0;
+  c->nkev = 0;                 // kernel timings of earlier passes belong to no step
   c->n_int += 1;
   c->nsteps++;
   c->t = c->t + c->timestep;
@@ -293,6 +296,7 @@ int NbodySimulationGpu<ndim>::MainLoop() {
   c->grav_launches = 0;
   c->pass_ms[6] = 0.0;
   c->hydro_launches = 0;
+  c->nkev = 0;                 // kernel timings of earlier passes belong to no step
   c->nsteps++;
   const double dtstep = c->timestep;
   c->t = c->t + c->timestep;
