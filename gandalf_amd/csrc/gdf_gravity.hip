@@ -563,7 +563,7 @@ int gravity_launch(gdf_ctx* c) {
   k_zero8<<<1, 64, 0, s2>>>(c->counters + 16);
   // half the persistent wavefronts of a lone walk: at the full count they fill every CU's LDS
   // and the scatter-gather walk beside them waits for the gravity walk to drain
-  const int grid = std::min(c->ngroups, kWalkSlots/2);
+  const int grid = std::min(c->ngroups, kGravSlots/2);
   const int kt = ktime_begin(c, s2, 5);
   if (c->cfg.precision == GDF_PREC_MIXED) k_gravity<true, false, false><<<grid, kWave, 0, s2>>>(a);
   else k_gravity<false, false, false><<<grid, kWave, 0, s2>>>(a);
@@ -600,7 +600,7 @@ int gravity_walk(gdf_ctx* c) {
   GDF_TRY(grav_group_list(c, a));
   k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
   k_zero8<<<1, 64, 0, s>>>(c->counters);
-  const int grid = std::min(c->ngroups, kWalkSlots);
+  const int grid = std::min(c->ngroups, kGravSlots);
   const int kt = ktime_begin(c, s, 5);
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
   if (c->phys.nbody) {

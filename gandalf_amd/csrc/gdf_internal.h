@@ -14,8 +14,10 @@ namespace gdf {
 // Walk groups hold at most one wavefront of targets.
 constexpr int kWave = 64;
 constexpr int kMaxGroup = 64;
-// Resident walk slots (one wavefront each); walk kernels grab groups from a counter.
+// Persistent wavefronts of the candidate walks and of the gravity walk (4 per SIMD; 8 per SIMD
+// made the candidate walks slower); walk kernels grab groups from a counter.
 constexpr int kWalkSlots = 4096;
+constexpr int kGravSlots = 4096;
 // Per-slot stack capacity of the tree walks (node indices).
 constexpr int kStackCap = 8192;
 
