@@ -319,8 +319,10 @@ int integ_seg_timesteps(gdf_ctx* c, double* out) {
   return GDF_OK;
 }
 
+// per-particle timesteps into D_DT; dtmin == NULL skips reading back their minimum (a block
+// step reads it only when it resynchronises the levels)
 int integ_timestep(gdf_ctx* c, double* dtmin) {
-  if (c->cfg.fixed_dt > 0.0) { *dtmin = c->cfg.fixed_dt; return GDF_OK; }
+  if (c->cfg.fixed_dt > 0.0) { if (dtmin) *dtmin = c->cfg.fixed_dt; return GDF_OK; }
   TsArgs t;
   t.N = c->N; t.hydro = c->phys.hydro_forces; t.energy = c->phys.energy_integration;
   t.nbody = c->phys.nbody;
@@ -331,6 +333,7 @@ int integ_timestep(gdf_ctx* c, double* dtmin) {
   t.u = c->pf[F_U]; t.dudt = c->df[D_DUDT]; t.dt_out = c->df[D_DT]; t.part = c->red;
   const int nb = std::min(nblocks(c->N, BS), RED_BLOCKS);
   k_timestep<<<nb, BS, 0, c->stream>>>(t);
+  if (!dtmin) return GDF_OK;
   k_min_final<<<1, BS, 0, c->stream>>>(nb, c->red, c->red + RED_BLOCKS*ND);
   GDF_HIP(c, hipMemcpyAsync(c->h_red, c->red + RED_BLOCKS*ND, sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
@@ -696,8 +699,10 @@ int block_check_timesteps(gdf_ctx* c, int* activecount) {
 
 int block_timesteps(gdf_ctx* c) {
   double dmin = 0.0;
-  GDF_TRY(integ_timestep(c, &dmin));      // per-particle criteria into D_DT, and their minimum
-  if (c->n_int == c->nresync) {
+  // per-particle criteria into D_DT; their minimum only where the levels resynchronise
+  const bool resync = c->n_int == c->nresync;
+  GDF_TRY(integ_timestep(c, resync ? &dmin : nullptr));
+  if (resync) {
     c->n_int = 0;
     c->timestep = dmin;
     c->level_max = c->nlevels - 1;

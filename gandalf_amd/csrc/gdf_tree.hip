@@ -656,23 +656,29 @@ __global__ void k_groups_fill(int N, const int* __restrict__ nsel, const int* __
 // kernels fill the device: a look-back scan then stalls until its predecessor workgroups find
 // a compute unit, and any workgroup that needs LDS waits until enough LDS is free on one CU
 // (both measured at up to 2-4 ms beside the gravity walk); one LDS-free wavefront starts at
-// once.  Each lane takes a contiguous run of 4-byte words (the flag buffer is padded to a
-// multiple of 4 bytes).
-__global__ __launch_bounds__(64) void k_compact_flags(int n, const unsigned* __restrict__ words,
+// once.  Each lane takes a contiguous run of 16-byte blocks (the flag buffer is zero-padded to
+// a multiple of 16 bytes).
+__global__ __launch_bounds__(64) void k_compact_flags(int n, const uint4* __restrict__ blocks,
                                                       int* __restrict__ out, int cap) {
   const int lane = threadIdx.x;
-  const int nword = (n + 3)/4;
-  const int per = (nword + kWave - 1)/kWave;
-  const int w0 = min(lane*per, nword), w1 = min(w0 + per, nword);
+  const int nblk = (n + 15)/16;
+  const int per = (nblk + kWave - 1)/kWave;
+  const int b0 = min(lane*per, nblk), b1 = min(b0 + per, nblk);
   int cnt = 0;
-  for (int w = w0; w < w1; w++) cnt += __popc(words[w]);   // bytes are 0 or 1
+  for (int b = b0; b < b1; b++) {
+    const uint4 x = blocks[b];
+    cnt += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);   // bytes are 0 or 1
+  }
   const int incl = wave_incl_scan(cnt);
   int o = incl - cnt;
-  for (int w = w0; w < w1; w++) {
-    const unsigned x = words[w];
-    if (!x) continue;
-    for (int k = 0; k < 4; k++)
-      if ((x >> (8*k)) & 1u) out[o++] = 4*w + k;
+  for (int b = b0; b < b1; b++) {
+    const uint4 x = blocks[b];
+    const unsigned wd[4] = {x.x, x.y, x.z, x.w};
+    for (int k = 0; k < 4; k++) {
+      if (!wd[k]) continue;
+      for (int q = 0; q < 4; q++)
+        if ((wd[k] >> (8*q)) & 1u) out[o++] = 16*b + 4*k + q;
+    }
   }
   if (lane == kWave - 1) out[cap] = incl;
 }
@@ -689,9 +695,9 @@ __global__ void k_group_flags(int ngroups, const int* __restrict__ grp_first,
   const bool any = __ballot(on) != 0;
   if (lane == 0) {
     flags[g] = any ? 1 : 0;
-    // the compaction reads whole 4-byte words: the bytes after the last flag read 0
+    // the compaction reads whole 16-byte blocks: the bytes after the last flag read 0
     if (g == ngroups - 1)
-      for (int k = ngroups; k & 3; k++) flags[k] = 0;
+      for (int k = ngroups; k & 15; k++) flags[k] = 0;
     if (!any) {
       if (ncand) ncand[g] = 0;
       if (gnc) { gnc[g] = 0; gnn[g] = 0; }
@@ -756,7 +762,7 @@ int tree_alloc(gdf_ctx* c) {
   GDF_HIP(c, hipMalloc(&c->first, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->last, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->parent, sizeof(int)*nn));
-  GDF_HIP(c, hipMalloc(&c->visit, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->visit, sizeof(int)*(N + 16)));   // also 16-byte-padded flags
   // refit frontiers: phase 1 (one RB slot per chunk), chunk counts, phase 2 (N + count)
   GDF_HIP(c, hipMalloc(&c->front, sizeof(int)*(2*N + 1024 + N/1024 + 16)));
   GDF_HIP(c, hipMalloc(&c->bbmin, sizeof(double)*3*nn));
@@ -868,7 +874,7 @@ int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, i
   constexpr int gpb = BS/kWave;   // groups per block
   k_group_flags<<<(ng + gpb - 1)/gpb, BS, 0, s>>>(ng, c->grp_first, c->grp_count, status, flags,
                                                   ncand, gnc, gnn);
-  k_compact_flags<<<1, kWave, 0, s>>>(ng, reinterpret_cast<const unsigned*>(flags), list,
+  k_compact_flags<<<1, kWave, 0, s>>>(ng, reinterpret_cast<const uint4*>(flags), list,
                                       c->Ncap);
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
