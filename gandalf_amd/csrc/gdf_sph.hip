@@ -48,11 +48,16 @@ struct CandWalk {
 // MODE 0: search set |r_ij|^2 <= (k hs_i)^2 (FP64); 1: scatter-gather set
 // |r_ij|^2 < (k h_i)^2 or < (k h_j)^2 (FP64, the oracle's operation order); 2: the same set
 // decided in FP32 in the group-centred frame, exactly as the mixed-precision gravity walk
-// decides it.  PER: periodic nearest images.
+// decides it; 3: a superset of the MODE 0 set decided in FP32 in the group-centred frame, the
+// radius widened by 1e-6 of the group's half-extent plus k hs_i (sixteen times the FP32
+// rounding of the frame's coordinates) and the square by 1e-5 (mixed h-rho pass: the extra
+// candidates lie beyond k h_i and carry zero weight).  PER: periodic nearest images.
 template <int MODE, bool PER>
 __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
-  __shared__ double4 sc[MODE == 2 ? 1 : kWave];   // MODE 0/1: (x, y, z, (k h_j)^2)
-  __shared__ float4 fc[MODE == 2 ? kWave : 1];    // MODE 2: group-frame (x, y, z, (k h_j)^2)
+  constexpr bool F32 = MODE >= 2;      // group-frame FP32 tests
+  constexpr bool GATHER = MODE == 0 || MODE == 3;
+  __shared__ double4 sc[F32 ? 1 : kWave];   // MODE 0/1: (x, y, z, (k h_j)^2)
+  __shared__ float4 fc[F32 ? kWave : 1];    // MODE 2/3: group-frame (x, y, z, (k h_j)^2)
   __shared__ int lstk[kLdsStack];    // node stack; deeper entries go to the global region
   const int lane = threadIdx.x;
   int* stk = a.stacks + (size_t) blockIdx.x*kStackCap;
@@ -82,6 +87,11 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
     const double4 gc = a.geo_c[G], gh = a.geo_h[G];
     const float rxi = (float) (xi - gc.x), ryi = (float) (yi - gc.y), rzi = (float) (zi - gc.z);
     const float r2if = (float) r2i;
+    float r2gf = 0.f;
+    if (MODE == 3 && has) {
+      const double rg = sqrt(r2i) + 1.0e-6*(gh.x + gh.y + gh.z + sqrt(r2i));
+      r2gf = (float) (rg*rg*(1.0 + 1.0e-5));
+    }
     int* idx = a.idx + (size_t) g*a.capb*kWave;
     unsigned long long* msk = a.mask + (size_t) g*a.capb*kWave;
     const int cap = a.capb*kWave;
@@ -108,7 +118,7 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
         const double gy = fmax(fabs(dy) - nh.y - gh.y, 0.0);
         const double gz = fmax(fabs(dz) - nh.z - gh.z, 0.0);
         const double dbox2 = gx*gx + gy*gy + gz*gz;
-        const double rr = MODE == 0 ? kr*hG : kr*fmax(hG, nc.w);
+        const double rr = GATHER ? kr*hG : kr*fmax(hG, nc.w);
         if (dbox2 <= rr*rr*NODE_SLACK) {
           ef = nl.first; ec = nl.count;
           if (ec <= a.nleaf) emit = true;
@@ -151,8 +161,8 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
       const int nb = (tail - b + nbat - 1)/nbat;
       if (lane < nb) {
         const int j = idx[b + lane*nbat];
-        const double r2j = MODE == 0 ? 0.0 : range2_rn(krsqd, a.hcand[j]);
-        if (MODE == 2) {
+        const double r2j = GATHER ? 0.0 : range2_rn(krsqd, a.hcand[j]);
+        if (F32) {
           double dx = a.x[j] - gc.x, dy = a.y[j] - gc.y, dz = a.z[j] - gc.z;
           nearest_image<double>(pp, dx, dy, dz);
           fc[lane] = make_float4((float) dx, (float) dy, (float) dz, (float) r2j);
@@ -165,7 +175,10 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
       if (has) {
         for (int q = 0; q < nb; q++) {
           bool in;
-          if (MODE == 2) {
+          if (MODE == 3) {
+            const float4 c = fc[q];
+            in = dot3_f(c.x - rxi, c.y - ryi, c.z - rzi) <= r2gf;
+          } else if (MODE == 2) {
             const float4 c = fc[q];
             const float d2 = dot3_f(c.x - rxi, c.y - ryi, c.z - rzi);
             in = (d2 < r2if) | (d2 < c.w);
@@ -698,16 +711,19 @@ int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status,
   if (nopbc) w.p.periodic = 0;
   k_zero_ints<<<1, BS, 0, c->stream>>>(8, c->counters + 4);
   const int grid = std::min(c->ngroups, kWalkSlots);
-  const int m = mode == 0 ? 0 : (c->cfg.precision == GDF_PREC_MIXED ? 2 : 1);
+  const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
+  const int m = mode == 0 ? (mixed ? 3 : 0) : (mixed ? 2 : 1);
   hipStream_t st = c->stream;
   if (w.p.periodic) {
     if (m == 0) k_walk_cand<0, true><<<grid, kWave, 0, st>>>(w);
     else if (m == 1) k_walk_cand<1, true><<<grid, kWave, 0, st>>>(w);
-    else k_walk_cand<2, true><<<grid, kWave, 0, st>>>(w);
+    else if (m == 2) k_walk_cand<2, true><<<grid, kWave, 0, st>>>(w);
+    else k_walk_cand<3, true><<<grid, kWave, 0, st>>>(w);
   } else {
     if (m == 0) k_walk_cand<0, false><<<grid, kWave, 0, st>>>(w);
     else if (m == 1) k_walk_cand<1, false><<<grid, kWave, 0, st>>>(w);
-    else k_walk_cand<2, false><<<grid, kWave, 0, st>>>(w);
+    else if (m == 2) k_walk_cand<2, false><<<grid, kWave, 0, st>>>(w);
+    else k_walk_cand<3, false><<<grid, kWave, 0, st>>>(w);
   }
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
