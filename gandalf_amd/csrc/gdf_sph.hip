@@ -348,8 +348,10 @@ __global__ __launch_bounds__(64) void k_density(DensArgs a) {
       zeta *= invhsqd;
       const int rc = hrho_step(p, m, hmax, iteration, rho, h, h_lower, h_upper);
       if (rc == 1) {
-        hrho_finish(a, p, i, m, rho, invomega, zeta, h_lower);
-        a.status[i] = 0;
+        // converged: the sums and the bracket go to k_hrho_finish (status 2), which keeps
+        // the final h, grad-h terms and EOS out of this kernel's register budget
+        a.rho[i] = rho; a.invomega[i] = invomega; a.zeta[i] = zeta; a.hfactor[i] = h_lower;
+        a.status[i] = 2;
         ngather = inside;
         state = 1;
       } else if (rc == 0) {
@@ -365,6 +367,15 @@ __global__ __launch_bounds__(64) void k_density(DensArgs a) {
   }
   wave_add_ll(&a.icount[0], ngather);
   wave_add_ll(&a.icount[4], evals);
+}
+
+// Final h, grad-h terms and EOS of the particles the h-rho pass has just converged (status 2:
+// rho, invomega and zeta hold the sums at the converged h, hfactor the bracket's lower end)
+__global__ void k_hrho_finish(DensArgs a) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i >= a.N || a.status[i] != 2) return;
+  hrho_finish(a, a.p, i, a.posm[i].w, a.rho[i], a.invomega[i], a.zeta[i], a.hfactor[i]);
+  a.status[i] = 0;
 }
 
 // act (block timesteps): only the active particles iterate; NULL = all
@@ -771,6 +782,7 @@ int sph_density(gdf_ctx* c, int nopbc) {
     d.capb = c->cs_g.capb; d.idx = c->cs_g.idx; d.mask = c->cs_g.mask; d.ncand = c->cs_g.ncand;
     if (c->cfg.precision == GDF_PREC_MIXED) k_density<float><<<c->ngroups, kWave, 0, s>>>(d);
     else k_density<double><<<c->ngroups, kWave, 0, s>>>(d);
+    k_hrho_finish<<<nblocks(N, BS), BS, 0, s>>>(d);
     GDF_HIP(c, hipGetLastError());
     GDF_TRY(read_flags(c));
     const int* f = c->h_counters;
