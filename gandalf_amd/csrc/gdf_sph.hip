@@ -541,8 +541,9 @@ __global__ __launch_bounds__(64) void k_hydro(HydroArgs a) {
 // FP32 variant: records staged in LDS in the group frame (positions relative to the group
 // centre, velocities relative to the group's first particle), pair terms in FP32 with the
 // hardware reciprocal and square root, per-batch register sums, Kahan-compensated totals.
+// (at most 96 VGPRs: five wavefronts per SIMD instead of four, a few registers spilled)
 template <int VAR, bool BLK>
-__global__ __launch_bounds__(64) void k_hydro_mixed(HydroArgs a) {
+__global__ __launch_bounds__(64, 5) void k_hydro_mixed(HydroArgs a) {
   __shared__ float4 fP[kWave], fV[kWave], fB[kWave];
   __shared__ float fZ[kWave];
   const int g = xcd_block(blockIdx.x, gridDim.x);
