@@ -173,7 +173,11 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
       __syncthreads();
       unsigned long long m = 0;
       if (has) {
-        for (int q = 0; q < nb; q++) {
+        // all 64 slots in a fixed-count unrolled loop, so several LDS reads are in flight (a
+        // loop over the nb filled slots waited on one read per test); slots past nb are
+        // masked off after the loop
+        #pragma unroll 16
+        for (int q = 0; q < kWave; q++) {
           bool in;
           if (MODE == 3) {
             const float4 c = fc[q];
@@ -192,6 +196,7 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
           }
           m |= (unsigned long long) in << q;
         }
+        if (nb < kWave) m &= (1ULL << nb) - 1ULL;
       }
       msk[b*kWave + lane] = m;
       __syncthreads();
