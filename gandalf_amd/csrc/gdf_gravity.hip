@@ -266,6 +266,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
         if (has) {
           if (MIXED) {
             float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
+            #pragma unroll 4
             for (int q = 0; q < nb; q++) {
               const float qq[5] = {sf[4][q], sf[5][q], sf[6][q], sf[7][q], sf[8][q]};
               quad_term<float>(rxi - sf[0][q], ryi - sf[1][q], rzi - sf[2][q], sf[3][q], qq, bx,
@@ -309,6 +310,8 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
           if (MIXED) {
             const float hi = (float) mr.x, zi = (float) mr.y, hfi = (float) mr.z;
             float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
+            // unrolled so that several LDS reads are in flight (same summation order)
+            #pragma unroll 8
             for (int q = 0; q < nb; q++) {
               const float dx = sf[0][q] - rxi, dy = sf[1][q] - ryi, dz = sf[2][q] - rzi;
               const float d2 = dot3_f(dx, dy, dz);
