@@ -157,19 +157,31 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
     // 2. masks: batch b holds the candidates at walk positions b, b + nbat, b + 2 nbat, ...,
     // so every batch samples the whole candidate region and each target's neighbours spread
     // evenly over the batches (fewer idle lanes in the per-target mask loops of the pair passes)
+    // The candidate records of batch b + 1 are loaded while batch b is tested, and the
+    // candidate indices of batch b + 2 with them.
+    auto nbat_of = [&](int bb) { return (tail - bb + nbat - 1)/nbat; };
+    auto record = [&](int j) {
+      return make_double4(a.x[j], a.y[j], a.z[j], GATHER ? 0.0 : a.hcand[j]);
+    };
+    double4 cur = make_double4(0.0, 0.0, 0.0, 0.0), nxt = cur;
+    int jn = 0;
+    if (lane < nbat_of(0)) cur = record(idx[lane*nbat]);
+    if (nbat > 1 && lane < nbat_of(1)) jn = idx[1 + lane*nbat];
     for (int b = 0; b < nbat; b++) {
-      const int nb = (tail - b + nbat - 1)/nbat;
+      const int nb = nbat_of(b);
       if (lane < nb) {
-        const int j = idx[b + lane*nbat];
-        const double r2j = GATHER ? 0.0 : range2_rn(krsqd, a.hcand[j]);
+        const double r2j = GATHER ? 0.0 : range2_rn(krsqd, cur.w);
         if (F32) {
-          double dx = a.x[j] - gc.x, dy = a.y[j] - gc.y, dz = a.z[j] - gc.z;
+          double dx = cur.x - gc.x, dy = cur.y - gc.y, dz = cur.z - gc.z;
           nearest_image<double>(pp, dx, dy, dz);
           fc[lane] = make_float4((float) dx, (float) dy, (float) dz, (float) r2j);
         } else {
-          sc[lane] = make_double4(a.x[j], a.y[j], a.z[j], r2j);
+          sc[lane] = make_double4(cur.x, cur.y, cur.z, r2j);
         }
       }
+      int jn2 = 0;
+      if (b + 1 < nbat && lane < nbat_of(b + 1)) nxt = record(jn);
+      if (b + 2 < nbat && lane < nbat_of(b + 2)) jn2 = idx[b + 2 + lane*nbat];
       __syncthreads();
       unsigned long long m = 0;
       if (has) {
@@ -200,6 +212,8 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
       }
       msk[b*kWave + lane] = m;
       __syncthreads();
+      cur = nxt;
+      jn = jn2;
     }
     if (lane == 0) a.ncand[g] = tail;
   }
