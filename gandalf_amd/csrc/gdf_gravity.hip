@@ -535,12 +535,15 @@ GravArgs grav_args(gdf_ctx* c) {
 }
 
 // in a block step, the compacted list of groups with active particles (built on the main stream)
-int grav_group_list(gdf_ctx* c, GravArgs& a) {
-  if (!a.act || !c->glist) return GDF_OK;
+// zero[0, nzero): the flags and queue counters of the walk, cleared with the list (or by the
+// caller when there is no list: returns false)
+bool grav_group_list(gdf_ctx* c, GravArgs& a, int* zero, int nzero, int& rc) {
+  rc = GDF_OK;
+  if (!a.act || !c->glist) return false;
   int* gl = c->glist + 2*((size_t) c->Ncap + 1);
-  GDF_TRY(group_list(c, a.act, gl, nullptr, c->grav_nc, c->grav_nn));
+  rc = group_list(c, a.act, gl, nullptr, c->grav_nc, c->grav_nn, zero, nzero);
   a.glist = gl;
-  return GDF_OK;
+  return true;
 }
 
 }  // namespace
@@ -554,7 +557,9 @@ int gravity_launch(gdf_ctx* c) {
   // than running them in turn, both at 1M active particles and in sparse block steps
   hipStream_t s2 = c->stream2;
   GravArgs a = grav_args(c);
-  GDF_TRY(grav_group_list(c, a));
+  int rc = GDF_OK;
+  const bool listed = grav_group_list(c, a, c->counters + 16, 16, rc);   // flags and queue
+  GDF_TRY(rc);
   GDF_HIP(c, hipEventRecord(c->ev_dens, c->stream));        // h, zeta, tree, posm are ready
   GDF_HIP(c, hipStreamWaitEvent(s2, c->ev_dens, 0));
   k_pack_grav<<<nblocks(N, 256), 256, 0, s2>>>(N, c->pf[F_H], c->df[D_ZETA], c->df[D_HFACTOR],
@@ -562,8 +567,10 @@ int gravity_launch(gdf_ctx* c) {
   a.stacks = c->stacks2; a.counter = c->counters + 24; a.flags = c->counters + 16;
   a.gwx = c->df[D_GWX]; a.gwy = c->df[D_GWY]; a.gwz = c->df[D_GWZ];
   a.gwp = c->df[D_GWP]; a.gwp0 = c->df[D_GWP0];
-  k_zero8<<<1, 64, 0, s2>>>(c->counters + 24);
-  k_zero8<<<1, 64, 0, s2>>>(c->counters + 16);
+  if (!listed) {
+    k_zero8<<<1, 64, 0, s2>>>(c->counters + 24);
+    k_zero8<<<1, 64, 0, s2>>>(c->counters + 16);
+  }
   // half the persistent wavefronts of a lone walk: at the full count they fill every CU's LDS
   // and the scatter-gather walk beside them waits for the gravity walk to drain
   const int grid = std::min(c->ngroups, kGravSlots/2);
@@ -600,9 +607,12 @@ int gravity_walk(gdf_ctx* c) {
   k_pack_grav<<<nblocks(N, 256), 256, 0, s>>>(N, c->pf[F_H], c->df[D_ZETA], c->df[D_HFACTOR],
                                                 c->grec);
   GravArgs a = grav_args(c);
-  GDF_TRY(grav_group_list(c, a));
-  k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
-  k_zero8<<<1, 64, 0, s>>>(c->counters);
+  int rc = GDF_OK;
+  if (!grav_group_list(c, a, c->counters, 12, rc)) {
+    k_zero8<<<1, 64, 0, s>>>(c->counters + 4);
+    k_zero8<<<1, 64, 0, s>>>(c->counters);
+  }
+  GDF_TRY(rc);
   const int grid = std::min(c->ngroups, kGravSlots);
   const int kt = ktime_begin(c, s, 5);
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;

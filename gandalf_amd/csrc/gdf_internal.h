@@ -212,15 +212,17 @@ int ctx_field_width(gdf_ctx* c, const char* name);
 int seg_upload(gdf_ctx* c);
 int tree_build(gdf_ctx* c);
 int tree_refit_h(gdf_ctx* c);
-// compacted list of the walk groups holding a particle with status[i] == 1 (ascending, count at
-// list[Ncap]); the optional per-group outputs of every other group are cleared
 // kernel timing (ctx->timing): kind 5 gravity walk, 6 hydro pair kernel.  begin returns the
 // event pair's slot (-1 with timing off); collect waits for the recorded pairs and adds them
 // to pass_ms and the launch counts
 int ktime_begin(gdf_ctx* c, hipStream_t s, int kind);
 void ktime_end(gdf_ctx* c, hipStream_t s, int slot);
 void ktime_collect(gdf_ctx* c);
-int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, int* gnn);
+// compacted list of the walk groups holding a particle with status[i] == 1 (ascending, count at
+// list[Ncap]); the optional per-group outputs of every other group are cleared, and so are
+// zero[0, nzero) (nzero <= 256: the flags and queue counters of the walk that follows)
+int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, int* gnn,
+               int* zero = nullptr, int nzero = 0);
 TreeView tree_view(const gdf_ctx* c);
 
 // SPH passes (gdf_sph.hip)

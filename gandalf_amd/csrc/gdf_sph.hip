@@ -723,6 +723,7 @@ int grow_cand(gdf_ctx* c, CandSet& cs, int need) {
 }
 
 // glist: the groups holding targets (from group_list), or NULL to walk every group
+// (the walk's queue counters, counters[4, 12), must be zero: the list builder clears them)
 int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status,
                 const int* glist) {
   CandWalk w;
@@ -740,7 +741,7 @@ int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status,
   w.stacks = c->stacks; w.counter = c->counters + 4; w.flags = c->counters;
   w.p = c->phys;
   if (nopbc) w.p.periodic = 0;
-  k_zero_ints<<<1, BS, 0, c->stream>>>(8, c->counters + 4);
+  if (!glist) k_zero_ints<<<1, BS, 0, c->stream>>>(8, c->counters + 4);
   const int grid = std::min(c->ngroups, kWalkSlots);
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
   const int m = mode == 0 ? (mixed ? 3 : 0) : (mixed ? 2 : 1);
@@ -796,8 +797,8 @@ int sph_density(gdf_ctx* c, int nopbc) {
   d.status = c->status; d.flags = c->counters; d.icount = c->icount;
   d.p = c->phys;
   for (int round = 0; round < 64; round++) {
-    k_zero_ints<<<1, BS, 0, s>>>(4, c->counters);
-    GDF_TRY(group_list(c, c->status, c->glist, c->cs_g.ncand, nullptr, nullptr));
+    // also clears the round's flags and the walk's queue counters
+    GDF_TRY(group_list(c, c->status, c->glist, c->cs_g.ncand, nullptr, nullptr, c->counters, 12));
     GDF_TRY(launch_walk(c, 0, nopbc, c->cs_g, c->status, c->glist));
     d.capb = c->cs_g.capb; d.idx = c->cs_g.idx; d.mask = c->cs_g.mask; d.ncand = c->cs_g.ncand;
     if (c->cfg.precision == GDF_PREC_MIXED) k_density<float><<<c->ngroups, kWave, 0, s>>>(d);
@@ -821,10 +822,10 @@ int sph_density(gdf_ctx* c, int nopbc) {
 int sph_build_sg(gdf_ctx* c) {
   GDF_TRY(ensure_cand(c, c->cs_s));
   for (int round = 0; round < 8; round++) {
-    k_zero_ints<<<1, BS, 0, c->stream>>>(4, c->counters);
     const int* act = c->nlevels > 1 ? c->pi[I_ACT] : nullptr;
     int* sl = c->glist + (c->Ncap + 1);
-    if (act) GDF_TRY(group_list(c, act, sl, c->cs_s.ncand, nullptr, nullptr));
+    if (act) GDF_TRY(group_list(c, act, sl, c->cs_s.ncand, nullptr, nullptr, c->counters, 12));
+    else k_zero_ints<<<1, BS, 0, c->stream>>>(4, c->counters);
     GDF_TRY(launch_walk(c, 1, 0, c->cs_s, act, act ? sl : nullptr));
     GDF_TRY(read_flags(c));
     if (c->h_counters[2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in force pass");

@@ -687,9 +687,11 @@ __global__ __launch_bounds__(64) void k_compact_flags(int n, const uint4* __rest
 __global__ void k_group_flags(int ngroups, const int* __restrict__ grp_first,
                               const int* __restrict__ grp_count, const int* __restrict__ status,
                               unsigned char* __restrict__ flags, int* __restrict__ ncand,
-                              int* __restrict__ gnc, int* __restrict__ gnn) {
+                              int* __restrict__ gnc, int* __restrict__ gnn,
+                              int* __restrict__ zero, int nzero) {
   const int g = blockIdx.x*(BS/kWave) + threadIdx.x/kWave;
   const int lane = threadIdx.x % kWave;
+  if (blockIdx.x == 0 && (int) threadIdx.x < nzero) zero[threadIdx.x] = 0;
   if (g >= ngroups) return;
   const bool on = lane < grp_count[g] && status[grp_first[g] + lane] == 1;
   const bool any = __ballot(on) != 0;
@@ -866,14 +868,15 @@ int tree_build(gdf_ctx* c) {
   return integ_pack_posm(c);
 }
 
-int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, int* gnn) {
+int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, int* gnn,
+               int* zero, int nzero) {
   const int ng = c->ngroups;
   hipStream_t s = c->stream;
   // the refit's arrival counters are free between tree passes (phase 1 clears them)
   unsigned char* flags = reinterpret_cast<unsigned char*>(c->visit);
   constexpr int gpb = BS/kWave;   // groups per block
   k_group_flags<<<(ng + gpb - 1)/gpb, BS, 0, s>>>(ng, c->grp_first, c->grp_count, status, flags,
-                                                  ncand, gnc, gnn);
+                                                  ncand, gnc, gnn, zero, nzero);
   k_compact_flags<<<1, kWave, 0, s>>>(ng, reinterpret_cast<const uint4*>(flags), list,
                                       c->Ncap);
   GDF_HIP(c, hipGetLastError());
