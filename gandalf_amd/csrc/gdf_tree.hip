@@ -352,10 +352,6 @@ __device__ __forceinline__ void store_h(int n, double hmax, double theta2, const
   a.geo_h[n] = gh;
 }
 
-__global__ void k_zero_visit(int n, int* __restrict__ visit) {
-  const int i = blockIdx.x*BS + threadIdx.x;
-  if (i < n) visit[i] = 0;
-}
 
 // Bottom-up refit in three phases.  The second child to reach a node merges it (an arrival
 // counter per node decides which), so every node is merged exactly once from finished children
@@ -380,7 +376,7 @@ __global__ __launch_bounds__(RB) void k_refit_chunks(
     const double* __restrict__ z, const double* __restrict__ mass, const double* __restrict__ h,
     const int* __restrict__ parent, const int* __restrict__ left, const int* __restrict__ first,
     const int* __restrict__ last, double theta2, NodeArgs a, int* __restrict__ visit,
-    int* __restrict__ front1, int* __restrict__ cnt1) {
+    int* __restrict__ front1, int* __restrict__ cnt1, int* __restrict__ front2_count) {
 #pragma clang fp contract(off)
   constexpr int ND = HONLY ? 1 : 16;    // doubles carried per node
   __shared__ double sd[ND*RB];
@@ -395,6 +391,7 @@ __global__ __launch_bounds__(RB) void k_refit_chunks(
   const int l = c0 + t;
   arr[t] = 0;
   if (t == 0) nfront = 0;
+  if (blockIdx.x == 0 && t == 0) *front2_count = 0;   // phase 2 counts its frontier here
   NodeD d;
   if (l <= c1) {
     lpar[t] = parent[N - 1 + l];
@@ -598,10 +595,9 @@ void launch_refit(gdf_ctx* c, const NodeArgs& na, double theta2) {
   int* front1 = c->front;                          // nch*RB entries
   int* cnt1 = front1 + (size_t) nch*RB;            // nch counts
   int* front2 = cnt1 + nch;                        // N entries + count
-  k_zero_visit<<<1, 64, 0, s>>>(1, front2 + N);
   k_refit_chunks<HONLY><<<nch, RB, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M],
                                            c->pf[F_H], c->parent, c->left, c->first, c->last,
-                                           theta2, na, c->visit, front1, cnt1);
+                                           theta2, na, c->visit, front1, cnt1, front2 + N);
   if (nch > 1) {
     const int nmid = (nch + RS - 1)/RS;
     k_refit_mid<HONLY><<<nmid, RB, 0, s>>>(N, nch, c->parent, c->left, c->right, c->first,
