@@ -644,12 +644,28 @@ int read_counter(gdf_ctx* c, int* out) {
   return GDF_OK;
 }
 
+__global__ void k_zero_counter(int* __restrict__ p) {
+  if (threadIdx.x < 2) p[threadIdx.x] = 0;
+}
+
 int zero_counter(gdf_ctx* c) {
-  GDF_HIP(c, hipMemsetAsync(c->counters + 12, 0, sizeof(int)*2, c->stream));
+  // a kernel rather than hipMemsetAsync: the runtime's memset path costs more host time per call
+  k_zero_counter<<<1, 64, 0, c->stream>>>(c->counters + 12);
+  GDF_HIP(c, hipGetLastError());
   return GDF_OK;
 }
 
 }  // namespace
+
+__global__ void k_zero_icount(long long* __restrict__ p) {
+  if (threadIdx.x < 8) p[threadIdx.x] = 0;
+}
+
+int zero_icount(gdf_ctx* c) {
+  k_zero_icount<<<1, 64, 0, c->stream>>>(c->icount);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
 
 int block_init(gdf_ctx* c) {
   c->n_int = 0; c->nresync = 0; c->level_max = 0; c->level_step = 0; c->dt_max = 0.0;

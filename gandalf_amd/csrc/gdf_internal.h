@@ -242,6 +242,7 @@ int gravity_launch(gdf_ctx* c);
 int gravity_join(gdf_ctx* c);
 
 // integration (gdf_integrate.hip)
+int zero_icount(gdf_ctx* c);    // clears the eight interaction counters of a step
 int integ_advance(gdf_ctx* c, double dt);
 // block timesteps (gdf_integrate.hip)
 int block_init(gdf_ctx* c);

@@ -839,7 +839,7 @@ int tree_build(gdf_ctx* c) {
     k_leaves<<<1, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M], c->pf[F_H],
                               theta2, na, c->visit, c->parent);
   // 6. walk groups in ascending first-particle order
-  GDF_HIP(c, hipMemsetAsync(c->visit, 0, sizeof(int)*((size_t) N), s));
+  GDF_HIP(c, hipMemsetAsync(c->visit, 0, (size_t) N, s));   // N byte flags
   k_groups_mark<<<nblocks(2*N - 1, BS), BS, 0, s>>>(N, c->cfg.group_max, c->parent, c->first,
                                                     c->last, c->gstart,
                                                     (unsigned char*) c->visit);

@@ -154,7 +154,7 @@ int GradhSphSimulationGpu<ndim>::MainLoop() {
   gdf_ctx* c = ctx;
   if (!c->setup_done) return gdf::fail(c, GDF_ERR_INVALID, "MainLoop before setup");
   if (c->nlevels > 1) return MainLoopBlock();
-  GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
+  GDF_TRY(gdf::zero_icount(c));
   record(c, 0);
   c->pass_ms[5] = 0.0;
   c->grav_launches = 0;
@@ -209,7 +209,7 @@ void GradhSphSimulationGpu<ndim>::FinishStepStats() {
 template <int ndim>
 int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
   gdf_ctx* c = ctx;
-  GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
+  GDF_TRY(gdf::zero_icount(c));
   record(c, 0);
   c->pass_ms[5] = 0.0;
   c->grav_launches = 0;
@@ -290,7 +290,7 @@ template <int ndim>
 int NbodySimulationGpu<ndim>::MainLoop() {
   gdf_ctx* c = ctx;
   if (!c->setup_done) return gdf::fail(c, GDF_ERR_INVALID, "MainLoop before setup");
-  GDF_HIP(c, hipMemsetAsync(c->icount, 0, sizeof(long long)*8, c->stream));
+  GDF_TRY(gdf::zero_icount(c));
   record(c, 0);
   c->pass_ms[5] = 0.0;
   c->grav_launches = 0;
