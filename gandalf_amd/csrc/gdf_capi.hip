@@ -393,6 +393,7 @@ int gdf_set_particles(gdf_ctx* c, int N, const double* r, const double* v, const
   if (!c) return GDF_ERR_INVALID;
   if (N <= 0 || !r || !v || !m || !u) return fail(c, GDF_ERR_INVALID, "gdf_set_particles: bad arguments");
   if ((long long) N > (1LL << 30)) return fail(c, GDF_ERR_INVALID, "too many particles");
+  c->snap_porig.clear();
   for (int i = 0; i < N; i++)
     if (!(m[i] > 0.0)) return fail(c, GDF_ERR_INVALID, "particle masses must be positive");
   if (N > c->Ncap) GDF_TRY(alloc_particles(c, N));

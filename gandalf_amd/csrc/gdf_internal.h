@@ -193,6 +193,11 @@ struct gdf_ctx {
   int kev_kind[kKevPairs] = {};
   int nkev = 0;
   long long icounts_last[6] = {};
+  // Seren binary snapshot header state (Simulation.h Noutsnap, Noutlitesnap, tsnaplast,
+  // tlitesnaplast) and the file's particle ids, carried from a read to the next write
+  int64_t snap_nout = 0, snap_nlite = 0;
+  double snap_tlast = 0.0, snap_tlitelast = 0.0;
+  std::vector<int> snap_porig;
 };
 
 namespace gdf {

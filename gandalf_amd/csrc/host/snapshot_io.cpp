@@ -1,11 +1,14 @@
-// Column-format snapshots of the device-resident state.
+// Snapshots of the device-resident state: the reference's column text format and its Seren
+// unformatted binary format (the default out_file_form, Parameters.cpp:175-176).
 //
 // Simulation::WriteColumnSnapshotFile / ReadColumnSnapshotFile
 // (src/Common/SimulationIO.hpp:444-548, :193-272), 3D, dimensionless units (every outscale is
 // 1): a header of Nhydro, Nstar, ndim and t, then one line per hydro particle with r, v, m, h,
 // rho, u and one per star with r, v, m, h, 0, 0, written with the default formatting of a C++
 // stream (6 significant digits) and "   " between values, as the reference writes them.
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -20,6 +23,33 @@ void put_line(std::ofstream& f, const double* r, const double* v, double m, doub
     << v[2] << "   " << m << "   " << h << "   ";
   if (star) f << 0.0 << "   " << 0.0 << std::endl;
   else f << rho << "   " << u << std::endl;
+}
+
+// Seren unformatted layout (SimulationIO.hpp:2009-2249; BinaryIO.h BinaryWriter writes each
+// value's bytes as they lie in memory): a 20-character space-padded tag, the FLOAT size and
+// three dimension counts, 50 int, 50 long, 50 FLOAT and 50 double header words, the unit and
+// array names as 20-character fields, five ints per array (width, 1, count, type code, unit
+// code), then the arrays; stars are one "sink_v1" block of 12 + 2*ndim FLOATs each.
+constexpr int kSuStr = 20;
+constexpr int kSinkLen = 12 + 2*3;
+const char* const kSuTag = "SERENBINARYDUMPV3";
+
+void su_str(std::ofstream& f, const char* s) {
+  char b[kSuStr];
+  std::memset(b, ' ', kSuStr);
+  std::memcpy(b, s, std::min<size_t>(std::strlen(s), kSuStr));
+  f.write(b, kSuStr);
+}
+
+template <class T> void su_put(std::ofstream& f, T v) { f.write((const char*) &v, sizeof(T)); }
+template <class T> bool su_get(std::ifstream& f, T& v) { return (bool) f.read((char*) &v, sizeof(T)); }
+
+std::string su_name(std::ifstream& f) {
+  char b[kSuStr];
+  if (!f.read(b, kSuStr)) return std::string();
+  std::string s(b, kSuStr);
+  const size_t a = s.find_first_not_of(" \t"), e = s.find_last_not_of(" \t");
+  return a == std::string::npos ? std::string() : s.substr(a, e - a + 1);
 }
 
 }  // namespace
@@ -43,9 +73,76 @@ int gdf_format_column(const char* path, int nhydro, int nstar, double t, const d
   return f.good() ? GDF_OK : GDF_ERR_INVALID;
 }
 
+int gdf_format_su(const char* path, int nhydro, int nstar, const int64_t* counts3,
+                  const double* times4, double h_fac, const int* porig, const double* r,
+                  const double* v, const double* m, const double* h, const double* rho,
+                  const double* u) {
+  if (!path || !counts3 || !times4 || nhydro < 0 || nstar < 0 ||
+      (nhydro + nstar > 0 && (!r || !v || !m || !h)) || (nhydro > 0 && (!rho || !u)))
+    return GDF_ERR_INVALID;
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return GDF_ERR_INVALID;
+  int idata[50] = {};
+  long ilpdata[50] = {};
+  double rdata[50] = {}, ddata[50] = {};
+  // every particle here is gas: the type counts are icm, gas, cdm, dust in idata[3..6]
+  idata[0] = nhydro; idata[1] = nstar; idata[4] = nhydro;
+  struct Arr { const char* id; int w, type, unit; };
+  const Arr hyd[] = {{"porig", 1, 2, 0}, {"r", 3, 4, 1}, {"m", 1, 4, 2}, {"h", 1, 4, 1},
+                     {"v", 3, 4, 4}, {"rho", 1, 4, 6}, {"u", 1, 4, 20}};
+  const int ndata = (nhydro > 0 ? 7 : 0) + (nstar > 0 ? 1 : 0);
+  idata[19] = 0;                       // no unit names: dimensionless
+  idata[20] = ndata;
+  ilpdata[0] = (long) counts3[0]; ilpdata[1] = (long) counts3[1]; ilpdata[10] = (long) counts3[2];
+  rdata[0] = h_fac;
+  ddata[0] = times4[0]; ddata[1] = times4[1]; ddata[2] = times4[2]; ddata[10] = times4[3];
+  su_str(f, kSuTag);
+  su_put(f, (int) sizeof(double));
+  for (int k = 0; k < 3; k++) su_put(f, 3);
+  for (int k = 0; k < 50; k++) su_put(f, idata[k]);
+  for (int k = 0; k < 50; k++) su_put(f, ilpdata[k]);
+  for (int k = 0; k < 50; k++) su_put(f, rdata[k]);
+  for (int k = 0; k < 50; k++) su_put(f, ddata[k]);
+  if (nhydro > 0) for (const Arr& a : hyd) su_str(f, a.id);
+  if (nstar > 0) su_str(f, "sink_v1");
+  if (nhydro > 0)
+    for (const Arr& a : hyd) {
+      const int td[5] = {a.w, 1, nhydro, a.type, a.unit};
+      for (int k = 0; k < 5; k++) su_put(f, td[k]);
+    }
+  if (nstar > 0) {
+    const int td[5] = {1, 1, nstar, 7, 0};
+    for (int k = 0; k < 5; k++) su_put(f, td[k]);
+  }
+  const size_t n = (size_t) nhydro;
+  if (nhydro > 0) {
+    for (size_t i = 0; i < n; i++) su_put(f, porig ? porig[i] : (int) i);
+    for (size_t i = 0; i < 3*n; i++) su_put(f, r[i]);
+    for (size_t i = 0; i < n; i++) su_put(f, m[i]);
+    for (size_t i = 0; i < n; i++) su_put(f, h[i]);
+    for (size_t i = 0; i < 3*n; i++) su_put(f, v[i]);
+    for (size_t i = 0; i < n; i++) su_put(f, rho[i]);
+    for (size_t i = 0; i < n; i++) su_put(f, u[i]);
+  }
+  if (nstar > 0) {
+    const int head[6] = {2, 2, 0, kSinkLen, 0, 0};
+    for (int k = 0; k < 6; k++) su_put(f, head[k]);
+    for (int i = 0; i < nstar; i++) {
+      const size_t j = n + (size_t) i;
+      su_put(f, 1); su_put(f, 1); su_put(f, i + 1); su_put(f, 0);
+      double sd[kSinkLen] = {};
+      for (int k = 0; k < 3; k++) { sd[1 + k] = r[3*j + k]; sd[4 + k] = v[3*j + k]; }
+      sd[7] = m[j]; sd[8] = h[j];          // sd[9], the star's radius, is 0 here
+      for (int k = 0; k < kSinkLen; k++) su_put(f, sd[k]);
+    }
+  }
+  return f.good() ? GDF_OK : GDF_ERR_INVALID;
+}
+
 int gdf_write_snapshot(gdf_ctx* c, const char* path, const char* format) {
   if (!c || !path || !format || c->N <= 0) return GDF_ERR_INVALID;
-  if (std::string(format) != "column")
+  const std::string fmt(format);
+  if (fmt != "column" && fmt != "su")
     return gdf::fail(c, GDF_ERR_INVALID, std::string("unsupported snapshot format ") + format);
   const size_t n = (size_t) c->N;
   std::vector<double> r(3*n), v(3*n), m(n), h(n), rho(n), u(n);
@@ -57,13 +154,105 @@ int gdf_write_snapshot(gdf_ctx* c, const char* path, const char* format) {
   }
   // an N-body context holds stars only (NbodySimulation writes them as the star block)
   const int nh = c->phys.nbody ? 0 : c->N, ns = c->phys.nbody ? c->N : 0;
-  const int rc = gdf_format_column(path, nh, ns, c->t, r.data(), v.data(), m.data(), h.data(),
-                                   rho.data(), u.data());
+  int rc;
+  if (fmt == "column") {
+    rc = gdf_format_column(path, nh, ns, c->t, r.data(), v.data(), m.data(), h.data(),
+                           rho.data(), u.data());
+  } else {
+    // the mean gas mass as SphSimulation.cpp:260-262 sums it, in particle order
+    double mmean = 0.0;
+    for (int i = 0; i < nh; i++) mmean += m[i];
+    if (nh > 0) mmean /= (double) nh;
+    const int64_t counts[3] = {c->snap_nout, c->nsteps, c->snap_nlite};
+    const double times[4] = {c->t, c->snap_tlast, mmean, c->snap_tlitelast};
+    const int* porig = c->snap_porig.size() == (size_t) nh ? c->snap_porig.data() : nullptr;
+    rc = gdf_format_su(path, nh, ns, counts, times, c->phys.h_fac, porig, r.data(), v.data(),
+                       m.data(), h.data(), rho.data(), u.data());
+  }
   return rc == GDF_OK ? rc : gdf::fail(c, rc, std::string("cannot write ") + path);
+}
+
+// ReadSerenUnformSnapshotFile (SimulationIO.hpp:1294-1540) into a device context: the header's
+// step count and time are taken as the reference takes them, and its snapshot counters are kept
+// for the next write; "temp" is skipped, other per-particle arrays are refused as there.
+static int read_su(gdf_ctx* c, const char* path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return gdf::fail(c, GDF_ERR_INVALID, std::string("cannot read ") + path);
+  if (su_name(f) != kSuTag) return gdf::fail(c, GDF_ERR_INVALID, "not a Seren binary snapshot");
+  int w = 0, d[3] = {};
+  if (!su_get(f, w) || w != (int) sizeof(double))
+    return gdf::fail(c, GDF_ERR_INVALID, "snapshot FLOAT size is not 8 bytes");
+  for (int k = 0; k < 3; k++)
+    if (!su_get(f, d[k]) || d[k] != 3) return gdf::fail(c, GDF_ERR_INVALID, "snapshot is not 3D");
+  int idata[50];
+  long ilpdata[50];
+  double rdata[50], ddata[50];
+  bool ok = true;
+  for (int k = 0; k < 50; k++) ok = ok && su_get(f, idata[k]);
+  for (int k = 0; k < 50; k++) ok = ok && su_get(f, ilpdata[k]);
+  for (int k = 0; k < 50; k++) ok = ok && su_get(f, rdata[k]);
+  for (int k = 0; k < 50; k++) ok = ok && su_get(f, ddata[k]);
+  const int nh = idata[0], ns = idata[1], nunit = idata[19], ndata = idata[20];
+  if (!ok || nh < 0 || ns < 0 || nh + ns <= 0 || nunit < 0 || nunit > 50 || ndata < 0 ||
+      ndata > 50)
+    return gdf::fail(c, GDF_ERR_INVALID, "bad Seren snapshot header");
+  if (c->phys.nbody ? nh != 0 : ns != 0)
+    return gdf::fail(c, GDF_ERR_INVALID, c->phys.nbody ? "N-body context: hydro particles in snapshot"
+                                                       : "hydro context: star particles in snapshot");
+  for (int k = 0; k < nunit; k++) su_name(f);
+  std::vector<std::string> ids(ndata);
+  for (int k = 0; k < ndata; k++) ids[k] = su_name(f);
+  std::vector<int> td(5*(size_t) ndata);
+  for (int& x : td) ok = ok && su_get(f, x);
+  if (!ok) return gdf::fail(c, GDF_ERR_INVALID, "truncated Seren snapshot header");
+  const size_t n = (size_t) (nh + ns), nhs = (size_t) nh;
+  std::vector<double> r(3*n), v(3*n), m(n), h(n), rho(n, 0.0), u(n, 0.0);
+  std::vector<int> porig;
+  for (int j = 0; j < ndata && ok; j++) {
+    const std::string& id = ids[j];
+    if (id == "porig") { porig.resize(nhs); for (int& x : porig) ok = ok && su_get(f, x); }
+    else if (id == "r") { for (size_t i = 0; i < 3*nhs; i++) ok = ok && su_get(f, r[i]); }
+    else if (id == "m") { for (size_t i = 0; i < nhs; i++) ok = ok && su_get(f, m[i]); }
+    else if (id == "h") { for (size_t i = 0; i < nhs; i++) ok = ok && su_get(f, h[i]); }
+    else if (id == "v") { for (size_t i = 0; i < 3*nhs; i++) ok = ok && su_get(f, v[i]); }
+    else if (id == "rho") { for (size_t i = 0; i < nhs; i++) ok = ok && su_get(f, rho[i]); }
+    else if (id == "u") { for (size_t i = 0; i < nhs; i++) ok = ok && su_get(f, u[i]); }
+    else if (id == "temp") { f.seekg((std::streamoff) (sizeof(double)*nhs), std::ios_base::cur); }
+    else if (id == "sink_v1") {
+      int head[6];
+      for (int& x : head) ok = ok && su_get(f, x);
+      if (!ok || head[0] != 2 || head[1] != 2 || head[2] != 0 || head[3] != kSinkLen)
+        return gdf::fail(c, GDF_ERR_INVALID, "bad sink_v1 block");
+      for (int i = 0; i < ns && ok; i++) {
+        int flags[4];
+        double sd[kSinkLen];
+        for (int& x : flags) ok = ok && su_get(f, x);
+        for (double& x : sd) ok = ok && su_get(f, x);
+        const size_t k = nhs + (size_t) i;
+        for (int a = 0; a < 3; a++) { r[3*k + a] = sd[1 + a]; v[3*k + a] = sd[4 + a]; }
+        m[k] = sd[7]; h[k] = sd[8];
+      }
+    } else if (td[5*(size_t) j] >= 1) {
+      return gdf::fail(c, GDF_ERR_INVALID, "snapshot array " + id + " cannot be read");
+    }
+  }
+  if (!ok || !f) return gdf::fail(c, GDF_ERR_INVALID, "truncated Seren snapshot");
+  int rc = gdf_set_particles(c, (int) n, r.data(), v.data(), m.data(), h.data(), u.data());
+  if (rc != GDF_OK) return rc;
+  if (!c->phys.nbody) rc = gdf_set_field(c, "rho", rho.data());
+  if (rc != GDF_OK) return rc;
+  c->t = ddata[0];
+  c->nsteps = ilpdata[1];
+  c->snap_nout = ilpdata[0]; c->snap_nlite = ilpdata[10];
+  c->snap_tlast = ddata[1]; c->snap_tlitelast = ddata[10];
+  c->snap_porig = std::move(porig);
+  (void) rdata;
+  return GDF_OK;
 }
 
 int gdf_read_snapshot(gdf_ctx* c, const char* path, const char* format) {
   if (!c || !path || !format) return GDF_ERR_INVALID;
+  if (std::string(format) == "su") return read_su(c, path);
   if (std::string(format) != "column")
     return gdf::fail(c, GDF_ERR_INVALID, std::string("unsupported snapshot format ") + format);
   std::ifstream f(path);

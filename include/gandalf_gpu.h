@@ -124,16 +124,26 @@ int gdf_main_loop(gdf_ctx* ctx, int nsteps);
 /* block-timestep state (Simulation.h:187-215): iout = {n, level_max, level_step, nresync,
  * particles that ended a step in the last MainLoop}; dout = {dt_max} */
 int gdf_block_state(const gdf_ctx* ctx, int64_t* iout5, double* dout1);
-/* snapshots of the device state in the reference's column format (Simulation::
- * WriteColumnSnapshotFile / ReadColumnSnapshotFile, src/Common/SimulationIO.hpp:444-548,
- * :193-272): format "column".  Reading sets the particles (r, v, m, h, u; rho as a field) and
- * the time; an N-body context reads and writes the star block. */
+/* snapshots of the device state (Simulation::WriteSnapshotFile / ReadSnapshotFile,
+ * src/Common/SimulationIO.hpp:60-150): format "column" (WriteColumnSnapshotFile /
+ * ReadColumnSnapshotFile, :444-548, :193-272) or "su" (WriteSerenUnformSnapshotFile /
+ * ReadSerenUnformSnapshotFile, :2009-2249, :1294-1540).  Reading sets the particles (r, v, m,
+ * h, u; rho as a field) and the time, and "su" the step count; an N-body context reads and
+ * writes the star block. */
 int gdf_write_snapshot(gdf_ctx* ctx, const char* path, const char* format);
 int gdf_read_snapshot(gdf_ctx* ctx, const char* path, const char* format);
 /* the column writer on host arrays (no device call): nhydro gas lines then nstar star lines */
 int gdf_format_column(const char* path, int nhydro, int nstar, double t, const double* r,
                       const double* v, const double* m, const double* h, const double* rho,
                       const double* u);
+/* the Seren unformatted writer on host arrays (Simulation::WriteSerenUnformSnapshotFile,
+ * src/Common/SimulationIO.hpp:2009-2249, serial build, FLOAT = double, dimensionless units):
+ * counts3 = {Noutsnap, Nsteps, Noutlitesnap}, times4 = {t, tsnaplast, mmean, tlitesnaplast};
+ * porig may be NULL (ids 0..nhydro-1); stars follow the gas in r, v, m, h */
+int gdf_format_su(const char* path, int nhydro, int nstar, const int64_t* counts3,
+                  const double* times4, double h_fac, const int* porig, const double* r,
+                  const double* v, const double* m, const double* h, const double* rho,
+                  const double* u);
 /* simulation time, current timestep and step count */
 int gdf_time(const gdf_ctx* ctx, double* t, double* timestep, int64_t* nsteps);
 

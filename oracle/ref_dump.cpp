@@ -9,7 +9,8 @@
 //   int32 N, int32 nfields, then nfields arrays of N doubles, in FIELD_NAMES order.
 //
 // usage: ref_dump <paramfile> <nsteps> <outprefix> [dump_every]
-//   writes <outprefix>_s<step>.bin (step 0 = after setup) and <outprefix>_diag.txt
+//   writes <outprefix>_s<step>.bin (step 0 = after setup) and <outprefix>_diag.txt, and the
+//   reference's own column and Seren unformatted snapshots of each dumped step after setup
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -117,6 +118,8 @@ int main(int argc, char** argv)
       dump(sim, prefix + name);
       snprintf(name, sizeof(name), "_s%d.col", s);
       sim->WriteColumnSnapshotFile(prefix + name);
+      snprintf(name, sizeof(name), "_s%d.su", s);
+      sim->WriteSerenUnformSnapshotFile(prefix + name);
     }
   }
   fclose(dfp);

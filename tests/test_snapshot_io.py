@@ -1,7 +1,9 @@
 """Column snapshots (Simulation::WriteColumnSnapshotFile / ReadColumnSnapshotFile,
-SimulationIO.hpp:444-548, :193-272).  The writer, given the state the reference held after 10
-steps of the periodic box, reproduces the reference's own snapshot file byte for byte (CPU);
-a snapshot read into a device context and written back is the same file (GPU)."""
+SimulationIO.hpp:444-548, :193-272) and Seren unformatted ones (WriteSerenUnformSnapshotFile /
+ReadSerenUnformSnapshotFile, :2009-2249, :1294-1540).  Each writer, given the state the
+reference held after 10 steps of the periodic box, reproduces the reference's own snapshot file
+byte for byte (CPU); a snapshot read into a device context and written back is the same file
+(GPU)."""
 import ctypes as C
 import os
 
@@ -53,3 +55,68 @@ def test_column_snapshot_roundtrip_through_device(tmp_path):
     out = tmp_path / "out.col"
     s.write_snapshot(out)
     assert out.read_bytes() == src.read_bytes()
+
+
+# Seren header words: the tag (20 bytes), FLOAT size and 3 dimension ints, then 50 int, 50 long,
+# 50 FLOAT and 50 double words
+_SU_I, _SU_L, _SU_R, _SU_D = 36, 236, 636, 1036
+
+
+def _su_header(b):
+    return (np.frombuffer(b[_SU_I:_SU_L], np.int32), np.frombuffer(b[_SU_L:_SU_R], np.int64),
+            np.frombuffer(b[_SU_R:_SU_D], np.float64), np.frombuffer(b[_SU_D:_SU_D + 400], np.float64))
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libgandalf_gpu.so not built")
+def test_su_writer_matches_reference_file(tmp_path):
+    d = load_golden("box16_m4")
+    n = d["meta"]["nsteps"]
+    ref = bytes(d[f"s{n}_su"])
+    N = d["in_m"].shape[0]
+    idata, ilp, rdata, ddata = _su_header(ref)
+    # what the header holds: counts, step count, time, h_fac and the mean mass summed in order
+    assert idata[0] == N and idata[1] == 0 and idata[4] == N and idata[20] == 7
+    assert ilp[1] == n and ddata[0] == float(d["diag"][n, 1])
+    assert rdata[0] == d["meta"]["params"]["h_fac"]
+    assert ddata[2] == np.cumsum(d["in_m"])[-1] / N
+    off = len(ref) - N * (4 + 8 * 10)          # the particle ids come first: 0..N-1
+    assert np.array_equal(np.frombuffer(ref[off:off + 4 * N], np.int32), np.arange(N))
+    lib = C.CDLL(LIB)
+    f = lib.gdf_format_su
+    f.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7
+    f.restype = C.c_int
+    counts = np.array([ilp[0], ilp[1], ilp[10]], dtype=np.int64)
+    times = np.array([ddata[0], ddata[1], ddata[2], ddata[10]])
+    r = vec(d, f"s{n}")
+    v = vec(d, f"s{n}", ("vx", "vy", "vz"))
+    out = tmp_path / "snap.su"
+    rc = f(str(out).encode(), N, 0, counts.ctypes.data_as(C.c_void_p), _p(times), float(rdata[0]),
+           None, _p(r), _p(v), _p(d["in_m"]), _p(d[f"s{n}_h"]), _p(d[f"s{n}_rho"]),
+           _p(d[f"s{n}_u"]))
+    assert rc == 0
+    assert out.read_bytes() == ref
+
+
+@pytest.mark.gpu
+def test_su_snapshot_roundtrip_through_device(tmp_path):
+    from gandalf_amd.gpu import GpuSim
+    d = load_golden("box16_m4")
+    n = d["meta"]["nsteps"]
+    src = tmp_path / "ref.su"
+    src.write_bytes(bytes(d[f"s{n}_su"]))
+    s = GpuSim(d["meta"]["params"])
+    s.read_snapshot(src, "su")
+    assert s.N == d["in_m"].shape[0]
+    assert np.array_equal(s.get("r"), vec(d, f"s{n}"))
+    assert np.array_equal(s.get("v"), vec(d, f"s{n}", ("vx", "vy", "vz")))
+    assert np.array_equal(s.get("h"), d[f"s{n}_h"]) and np.array_equal(s.get("rho"), d[f"s{n}_rho"])
+    assert np.array_equal(s.get("u"), d[f"s{n}_u"]) and np.array_equal(s.get("m"), d["in_m"])
+    t, _, nsteps = s.time()
+    assert t == float(d["diag"][n, 1]) and nsteps == n
+    out = tmp_path / "out.su"
+    s.write_snapshot(out, "su")
+    assert out.read_bytes() == src.read_bytes()
+    # the column writer of the same device state gives the reference's column file
+    col = tmp_path / "out.col"
+    s.write_snapshot(col)
+    assert col.read_bytes() == bytes(d[f"s{n}_column"])
