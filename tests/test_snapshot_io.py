@@ -120,3 +120,32 @@ def test_su_snapshot_roundtrip_through_device(tmp_path):
     col = tmp_path / "out.col"
     s.write_snapshot(col)
     assert col.read_bytes() == bytes(d[f"s{n}_column"])
+
+
+@pytest.mark.gpu
+def test_su_star_block_roundtrip(tmp_path):
+    """An N-body context writes its stars as the sink_v1 block (SimulationIO.hpp:2226-2246:
+    per star two flags, id i+1, 0 and 12 + 2*ndim FLOATs with r, v, m, h at 1..8) and reads
+    them back to the same state; the file holds no gas arrays."""
+    from gandalf_amd import ic as ICS
+    from gandalf_amd.gpu import GpuSim
+    g = ICS.plummer_sphere(N=512, seed=5)
+    h = 0.01 + 0.01 * np.random.default_rng(1).random(512)
+    s = GpuSim(g["params"])
+    s.set_particles(g["r"], g["v"], g["m"], g["u"], h=h)
+    out = tmp_path / "stars.su"
+    s.write_snapshot(out, "su")
+    b = out.read_bytes()
+    idata, ilp, rdata, ddata = _su_header(b)
+    assert idata[0] == 0 and idata[1] == 512 and idata[20] == 1
+    assert b[1436:1456] == b"sink_v1".ljust(20)
+    rec = np.frombuffer(b[1476 + 24:], np.dtype([("f", "<i4", 4), ("s", "<f8", 18)]))
+    assert rec.shape[0] == 512 and np.array_equal(rec["f"][:, 2], np.arange(1, 513))
+    assert np.array_equal(rec["s"][:, 1:4], g["r"]) and np.array_equal(rec["s"][:, 7], g["m"])
+    t = GpuSim(g["params"])
+    t.read_snapshot(out, "su")
+    for name, want in (("r", g["r"]), ("v", g["v"]), ("m", g["m"]), ("h", h)):
+        assert np.array_equal(t.get(name), want), name
+    again = tmp_path / "again.su"
+    t.write_snapshot(again, "su")
+    assert again.read_bytes() == b
