@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <string>
 #include <vector>
 
@@ -139,10 +140,76 @@ int gdf_format_su(const char* path, int nhydro, int nstar, const int64_t* counts
   return f.good() ? GDF_OK : GDF_ERR_INVALID;
 }
 
+// Seren formatted layout (WriteSerenFormSnapshotFile, SimulationIO.hpp:993-1222, with its
+// formatted_output stream, formatted_output.h:27-80): the same header words and arrays as the
+// unformatted file, one value per line, integers right-aligned in 2 (the first four lines), 10
+// and, in the star records, 8 columns, reals in scientific notation with 10 digits in 18.
+int gdf_format_sf(const char* path, int nhydro, int nstar, const int64_t* counts3,
+                  const double* times4, double h_fac, const int* porig, const double* r,
+                  const double* v, const double* m, const double* h, const double* rho,
+                  const double* u) {
+  if (!path || !counts3 || !times4 || nhydro < 0 || nstar < 0 ||
+      (nhydro + nstar > 0 && (!r || !v || !m || !h)) || (nhydro > 0 && (!rho || !u)))
+    return GDF_ERR_INVALID;
+  std::ofstream f(path);
+  if (!f) return GDF_ERR_INVALID;
+  f.setf(std::ios::scientific, std::ios::floatfield);
+  auto I = [&f](long long x, int w) { f << std::setw(w) << x; };
+  auto D = [&f](double x) { f << std::setw(18) << std::setprecision(10) << x; };
+  const int ndata = (nhydro > 0 ? 7 : 0) + (nstar > 0 ? 1 : 0);
+  long long idata[50] = {}, ilpdata[50] = {};
+  double rdata[50] = {}, ddata[50] = {};
+  idata[0] = nhydro; idata[1] = nstar; idata[4] = nhydro; idata[20] = ndata;
+  ilpdata[0] = counts3[0]; ilpdata[1] = counts3[1]; ilpdata[10] = counts3[2];
+  rdata[0] = h_fac;
+  ddata[0] = times4[0]; ddata[1] = times4[1]; ddata[2] = times4[2]; ddata[10] = times4[3];
+  f << "SERENASCIIDUMPV2" << std::endl;
+  for (int x : {4, 3, 3, 3}) { I(x, 2); f << std::endl; }
+  for (long long x : idata) { I(x, 10); f << std::endl; }
+  for (long long x : ilpdata) { I(x, 10); f << std::endl; }
+  for (double x : rdata) { D(x); f << std::endl; }
+  for (double x : ddata) { D(x); f << std::endl; }
+  struct Arr { const char* id; int w, type, unit; };
+  const Arr hyd[] = {{"porig", 1, 2, 0}, {"r", 3, 4, 1}, {"m", 1, 4, 2}, {"h", 1, 4, 1},
+                     {"v", 3, 4, 4}, {"rho", 1, 4, 6}, {"u", 1, 4, 20}};
+  if (nhydro > 0) for (const Arr& a : hyd) f << a.id << std::endl;
+  if (nstar > 0) f << "sink_v1" << std::endl;
+  if (nhydro > 0)
+    for (const Arr& a : hyd) {
+      for (int x : {a.w, 1, nhydro, a.type, a.unit}) I(x, 10);
+      f << std::endl;
+    }
+  if (nstar > 0) { for (int x : {1, 1, nstar, 7, 0}) I(x, 10); f << std::endl; }
+  const size_t n = (size_t) nhydro;
+  auto vecs = [&](const double* a) {
+    for (size_t i = 0; i < n; i++) { D(a[3*i]); D(a[3*i + 1]); D(a[3*i + 2]); f << std::endl; }
+  };
+  auto scal = [&](const double* a) { for (size_t i = 0; i < n; i++) { D(a[i]); f << std::endl; } };
+  if (nhydro > 0) {
+    for (size_t i = 0; i < n; i++) { I(porig ? porig[i] : (long long) i, 10); f << std::endl; }
+    vecs(r); scal(m); scal(h); vecs(v); scal(rho); scal(u);
+  }
+  if (nstar > 0) {
+    for (int x : {2, 2, 0, kSinkLen, 0, 0}) I(x, 10);
+    f << std::endl;
+    for (int i = 0; i < nstar; i++) {
+      const size_t j = n + (size_t) i;
+      f << "TT" << std::endl;
+      I(i + 1, 8); I(0, 8); f << std::endl;
+      double sd[kSinkLen] = {};
+      for (int k = 0; k < 3; k++) { sd[1 + k] = r[3*j + k]; sd[4 + k] = v[3*j + k]; }
+      sd[7] = m[j]; sd[8] = h[j];
+      for (double x : sd) D(x);
+      f << std::endl;
+    }
+  }
+  return f.good() ? GDF_OK : GDF_ERR_INVALID;
+}
+
 int gdf_write_snapshot(gdf_ctx* c, const char* path, const char* format) {
   if (!c || !path || !format || c->N <= 0) return GDF_ERR_INVALID;
   const std::string fmt(format);
-  if (fmt != "column" && fmt != "su")
+  if (fmt != "column" && fmt != "su" && fmt != "sf")
     return gdf::fail(c, GDF_ERR_INVALID, std::string("unsupported snapshot format ") + format);
   const size_t n = (size_t) c->N;
   std::vector<double> r(3*n), v(3*n), m(n), h(n), rho(n), u(n);
@@ -166,8 +233,9 @@ int gdf_write_snapshot(gdf_ctx* c, const char* path, const char* format) {
     const int64_t counts[3] = {c->snap_nout, c->nsteps, c->snap_nlite};
     const double times[4] = {c->t, c->snap_tlast, mmean, c->snap_tlitelast};
     const int* porig = c->snap_porig.size() == (size_t) nh ? c->snap_porig.data() : nullptr;
-    rc = gdf_format_su(path, nh, ns, counts, times, c->phys.h_fac, porig, r.data(), v.data(),
-                       m.data(), h.data(), rho.data(), u.data());
+    rc = (fmt == "su" ? gdf_format_su : gdf_format_sf)(path, nh, ns, counts, times,
+                                                        c->phys.h_fac, porig, r.data(), v.data(),
+                                                        m.data(), h.data(), rho.data(), u.data());
   }
   return rc == GDF_OK ? rc : gdf::fail(c, rc, std::string("cannot write ") + path);
 }
@@ -250,9 +318,87 @@ static int read_su(gdf_ctx* c, const char* path) {
   return GDF_OK;
 }
 
+// ReadSerenFormSnapshotFile (SimulationIO.hpp:601-925): the formatted twin of read_su.
+static int read_sf(gdf_ctx* c, const char* path) {
+  std::ifstream f(path);
+  if (!f) return gdf::fail(c, GDF_ERR_INVALID, std::string("cannot read ") + path);
+  std::string tag;
+  f >> tag;
+  if (tag != "SERENASCIIDUMPV2" && tag != "SERENASCIIDUMPV3")
+    return gdf::fail(c, GDF_ERR_INVALID, "not a Seren formatted snapshot");
+  int w = 0, d[3] = {};
+  f >> w >> d[0] >> d[1] >> d[2];
+  if (!f || d[0] != 3 || d[1] != 3 || d[2] != 3)
+    return gdf::fail(c, GDF_ERR_INVALID, "snapshot is not 3D");
+  long long idata[50], ilpdata[50];
+  double rdata[50], ddata[50];
+  for (long long& x : idata) f >> x;
+  for (long long& x : ilpdata) f >> x;
+  for (double& x : rdata) f >> x;
+  for (double& x : ddata) f >> x;
+  const long long nh = idata[0], ns = idata[1], nunit = idata[19], ndata = idata[20];
+  if (!f || nh < 0 || ns < 0 || nh + ns <= 0 || nh + ns > (1LL << 30) || nunit < 0 ||
+      nunit > 50 || ndata < 0 || ndata > 50)
+    return gdf::fail(c, GDF_ERR_INVALID, "bad Seren snapshot header");
+  if (c->phys.nbody ? nh != 0 : ns != 0)
+    return gdf::fail(c, GDF_ERR_INVALID, c->phys.nbody ? "N-body context: hydro particles in snapshot"
+                                                       : "hydro context: star particles in snapshot");
+  std::string tmp;
+  for (long long k = 0; k < nunit; k++) f >> tmp;
+  std::vector<std::string> ids((size_t) ndata);
+  for (std::string& x : ids) f >> x;
+  std::vector<long long> td(5*(size_t) ndata);
+  for (long long& x : td) f >> x;
+  if (!f) return gdf::fail(c, GDF_ERR_INVALID, "truncated Seren snapshot header");
+  const size_t n = (size_t) (nh + ns), nhs = (size_t) nh;
+  std::vector<double> r(3*n), v(3*n), m(n), h(n), rho(n, 0.0), u(n, 0.0);
+  std::vector<int> porig;
+  for (size_t j = 0; j < ids.size() && f; j++) {
+    const std::string& id = ids[j];
+    if (id == "porig") { porig.resize(nhs); for (int& x : porig) f >> x; }
+    else if (id == "r") { for (size_t i = 0; i < 3*nhs; i++) f >> r[i]; }
+    else if (id == "m") { for (size_t i = 0; i < nhs; i++) f >> m[i]; }
+    else if (id == "h") { for (size_t i = 0; i < nhs; i++) f >> h[i]; }
+    else if (id == "v") { for (size_t i = 0; i < 3*nhs; i++) f >> v[i]; }
+    else if (id == "rho") { for (size_t i = 0; i < nhs; i++) f >> rho[i]; }
+    else if (id == "u") { for (size_t i = 0; i < nhs; i++) f >> u[i]; }
+    else if (id == "temp") { double x; for (size_t i = 0; i < nhs; i++) f >> x; }
+    else if (id == "sink_v1") {
+      long long head[6];
+      for (long long& x : head) f >> x;
+      if (!f || head[0] != 2 || head[1] != 2 || head[2] != 0 || head[3] != kSinkLen)
+        return gdf::fail(c, GDF_ERR_INVALID, "bad sink_v1 block");
+      for (long long i = 0; i < ns && f; i++) {
+        long long id1, zero;
+        double sd[kSinkLen];
+        f >> tmp >> id1 >> zero;
+        for (double& x : sd) f >> x;
+        const size_t k = nhs + (size_t) i;
+        for (int a = 0; a < 3; a++) { r[3*k + a] = sd[1 + a]; v[3*k + a] = sd[4 + a]; }
+        m[k] = sd[7]; h[k] = sd[8];
+      }
+    } else if (td[5*j] >= 1) {
+      return gdf::fail(c, GDF_ERR_INVALID, "snapshot array " + id + " cannot be read");
+    }
+  }
+  if (!f) return gdf::fail(c, GDF_ERR_INVALID, "truncated Seren snapshot");
+  int rc = gdf_set_particles(c, (int) n, r.data(), v.data(), m.data(), h.data(), u.data());
+  if (rc != GDF_OK) return rc;
+  if (!c->phys.nbody) rc = gdf_set_field(c, "rho", rho.data());
+  if (rc != GDF_OK) return rc;
+  c->t = ddata[0];
+  c->nsteps = ilpdata[1];
+  c->snap_nout = ilpdata[0]; c->snap_nlite = ilpdata[10];
+  c->snap_tlast = ddata[1]; c->snap_tlitelast = ddata[10];
+  c->snap_porig = std::move(porig);
+  (void) w; (void) rdata;
+  return GDF_OK;
+}
+
 int gdf_read_snapshot(gdf_ctx* c, const char* path, const char* format) {
   if (!c || !path || !format) return GDF_ERR_INVALID;
   if (std::string(format) == "su") return read_su(c, path);
+  if (std::string(format) == "sf") return read_sf(c, path);
   if (std::string(format) != "column")
     return gdf::fail(c, GDF_ERR_INVALID, std::string("unsupported snapshot format ") + format);
   std::ifstream f(path);

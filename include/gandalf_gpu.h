@@ -126,8 +126,9 @@ int gdf_main_loop(gdf_ctx* ctx, int nsteps);
 int gdf_block_state(const gdf_ctx* ctx, int64_t* iout5, double* dout1);
 /* snapshots of the device state (Simulation::WriteSnapshotFile / ReadSnapshotFile,
  * src/Common/SimulationIO.hpp:60-150): format "column" (WriteColumnSnapshotFile /
- * ReadColumnSnapshotFile, :444-548, :193-272) or "su" (WriteSerenUnformSnapshotFile /
- * ReadSerenUnformSnapshotFile, :2009-2249, :1294-1540).  Reading sets the particles (r, v, m,
+ * ReadColumnSnapshotFile, :444-548, :193-272), "su" (WriteSerenUnformSnapshotFile /
+ * ReadSerenUnformSnapshotFile, :2009-2249, :1294-1540) or "sf" (WriteSerenFormSnapshotFile /
+ * ReadSerenFormSnapshotFile, :993-1222, :601-925).  Reading sets the particles (r, v, m,
  * h, u; rho as a field) and the time, and "su" the step count; an N-body context reads and
  * writes the star block. */
 int gdf_write_snapshot(gdf_ctx* ctx, const char* path, const char* format);
@@ -141,6 +142,12 @@ int gdf_format_column(const char* path, int nhydro, int nstar, double t, const d
  * counts3 = {Noutsnap, Nsteps, Noutlitesnap}, times4 = {t, tsnaplast, mmean, tlitesnaplast};
  * porig may be NULL (ids 0..nhydro-1); stars follow the gas in r, v, m, h */
 int gdf_format_su(const char* path, int nhydro, int nstar, const int64_t* counts3,
+                  const double* times4, double h_fac, const int* porig, const double* r,
+                  const double* v, const double* m, const double* h, const double* rho,
+                  const double* u);
+/* the Seren formatted writer on host arrays (Simulation::WriteSerenFormSnapshotFile,
+ * src/Common/SimulationIO.hpp:993-1222), arguments as gdf_format_su */
+int gdf_format_sf(const char* path, int nhydro, int nstar, const int64_t* counts3,
                   const double* times4, double h_fac, const int* porig, const double* r,
                   const double* v, const double* m, const double* h, const double* rho,
                   const double* u);

@@ -120,6 +120,8 @@ int main(int argc, char** argv)
       sim->WriteColumnSnapshotFile(prefix + name);
       snprintf(name, sizeof(name), "_s%d.su", s);
       sim->WriteSerenUnformSnapshotFile(prefix + name);
+      snprintf(name, sizeof(name), "_s%d.sf", s);
+      sim->WriteSerenFormSnapshotFile(prefix + name);
     }
   }
   fclose(dfp);

@@ -90,6 +90,8 @@ def run_case(name, gen, over, nsteps, outdir):
             # and its Seren unformatted snapshot (SimulationIO.hpp:2009)
             with open(os.path.join(td, f"out_s{nsteps}.su"), "rb") as f:
                 arrs[f"s{nsteps}_su"] = np.frombuffer(f.read(), dtype=np.uint8)
+            with open(os.path.join(td, f"out_s{nsteps}.sf"), "rb") as f:
+                arrs[f"s{nsteps}_sf"] = np.frombuffer(f.read(), dtype=np.uint8)
         diag = np.loadtxt(os.path.join(td, "out_diag.txt"))
         arrs["diag"] = np.atleast_2d(diag)
     meta = {"params": prm, "nsteps": nsteps, "build": "serial, COMPILER_MODE=STANDARD, "
