@@ -299,6 +299,9 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
       cfg->fixed_dt < 0.0)
     return GDF_ERR_INVALID;
   if (cfg->Nlevels < 1 || cfg->Nlevels > 20 || cfg->level_diff_max < 1) return GDF_ERR_INVALID;
+  if (cfg->hrho_iteration_max < 0 || cfg->hrho_iteration_max > 1000 ||
+      cfg->cand_batches_init < 0 || cfg->cand_batches_init > 4096 || !(cfg->h_converge >= 0.0))
+    return GDF_ERR_INVALID;
   if (cfg->Nlevels > 1 && (cfg->nbody || cfg->fixed_dt > 0.0)) return GDF_ERR_INVALID;
   if (cfg->periodic)
     for (int k = 0; k < 3; k++)
@@ -336,6 +339,7 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
   if (p.nbody) { p.hydro_forces = 0; p.self_gravity = 1; p.energy_integration = 0; }
   p.eps2 = cfg->nbody_eps*cfg->nbody_eps;
   p.nbody_mult = cfg->nbody_mult;
+  p.hrho_itmax = cfg->hrho_iteration_max > 0 ? cfg->hrho_iteration_max : 30;
   c->nlevels = cfg->Nlevels; c->level_diff_max = cfg->level_diff_max;
   int rc = upload_kernel_tables(c);
   if (rc == GDF_OK) {

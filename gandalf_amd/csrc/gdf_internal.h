@@ -71,6 +71,7 @@ struct PhysParams {
   int nbody;                 // pure N-body: stars only, no gas pass
   double eps2;               // Plummer softening length squared of the N-body tree walk
   double nbody_mult;
+  int hrho_itmax;            // fixed-point iterations before bisection (reference: 30)
 };
 
 // Tree node arrays in the unified index (internal 0..N-2, leaf l -> N-1+l).

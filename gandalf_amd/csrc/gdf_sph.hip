@@ -231,8 +231,8 @@ struct DensArgs {
 };
 
 // One step of the grad-h iteration after the sums at h (GradhSph.cpp:219-256): the fixed point
-// h <- h_fac (m/rho)^(1/3), bisection on [h_lower, h_upper] after 30 iterations, failure after
-// 150.  Returns 2 to iterate again, 1 done (converged, or the bracket closed), 0 retry with a
+// h <- h_fac (m/rho)^(1/3), bisection on [h_lower, h_upper] after hrho_itmax (30) iterations,
+// failure after five times as many.  Returns 2 to iterate again, 1 done (converged, or the bracket closed), 0 retry with a
 // larger search radius, -1 no convergence.
 __device__ __forceinline__ int hrho_step(const PhysParams& p, double m, double hmax, int iteration,
                                          double rho, double& h, double& h_lower,
@@ -240,9 +240,9 @@ __device__ __forceinline__ int hrho_step(const PhysParams& p, double m, double h
   const double invh = 1.0/h;
   const double hrho = rho > 0.0 ? p.h_fac*pow(m/rho, 1.0/3) : 0.0;
   if (rho > 0.0 && h > h_lower && fabs(h - hrho)*invh < p.h_converge) return 1;
-  if (iteration < 30) h = hrho;
-  else if (iteration == 30) h = 0.5*(h_lower + h_upper);
-  else if (iteration < 150) {
+  if (iteration < p.hrho_itmax) h = hrho;
+  else if (iteration == p.hrho_itmax) h = 0.5*(h_lower + h_upper);
+  else if (iteration < 5*p.hrho_itmax) {
     if (rho < 1.0e-20 || h > hrho) h_upper = h; else h_lower = h;
     h = 0.5*(h_lower + h_upper);
   } else return -1;
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(64) void k_density(DensArgs a) {
         ngather = inside;
         state = 1;
       } else if (rc == 0) {
-        a.hsearch[i] = hmax*1.1;
+        a.hsearch[i] = hmax*1.05;       // GradhSphTree.cpp:154: hmax = 1.05 hmax per retry
         atomicAdd(&a.flags[0], 1);
         state = 0;
       } else if (rc < 0) {
@@ -402,7 +402,7 @@ __global__ void k_init_density(int N, const double* __restrict__ h, double* __re
                                int* __restrict__ status, const int* __restrict__ act) {
   const int i = blockIdx.x*BS + threadIdx.x;
   if (i >= N) return;
-  hs[i] = 1.1*h[i];
+  hs[i] = 1.05*h[i];      // the search radius and bisection bracket, as GradhSphTree.cpp:154
   status[i] = act ? act[i] : 1;
 }
 
@@ -715,7 +715,9 @@ int alloc_cand(gdf_ctx* c, CandSet& cs, int capb) {
 
 int ensure_cand(gdf_ctx* c, CandSet& cs) {
   if (cs.idx && c->ngroups <= cs.gcap) return GDF_OK;
-  return alloc_cand(c, cs, cs.capb > 0 ? cs.capb : kInitBatches);
+  return alloc_cand(c, cs, cs.capb > 0 ? cs.capb
+                            : (c->cfg.cand_batches_init > 0 ? c->cfg.cand_batches_init
+                                                            : kInitBatches));
 }
 
 int grow_cand(gdf_ctx* c, CandSet& cs, int need) {

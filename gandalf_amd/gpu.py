@@ -41,6 +41,7 @@ class Config(C.Structure):
         ("nbody", C.c_int), ("nbody_direct", C.c_int), ("nbody_eps", C.c_double),
         ("nbody_mult", C.c_double), ("fixed_dt", C.c_double),
         ("Nlevels", C.c_int), ("level_diff_max", C.c_int),
+        ("hrho_iteration_max", C.c_int), ("cand_batches_init", C.c_int),
     ]
 
 
@@ -148,6 +149,8 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
     c.fixed_dt = float(prm.get("fixed_dt", 0.0))
     c.Nlevels = int(prm.get("Nlevels", 1))
     c.level_diff_max = int(prm.get("level_diff_max", 1))
+    c.hrho_iteration_max = int(prm.get("hrho_iteration_max", 0))
+    c.cand_batches_init = int(prm.get("cand_batches_init", 0))
     return c
 
 

@@ -173,14 +173,17 @@ def plummer_sphere(N=131072, seed=4, a=1.0, mass=1.0, rmax_frac=0.999):
     return dict(r=r, v=v, m=m, u=np.zeros(N), params=params)
 
 
-def write_column_file(path, ic):
-    """Reference column snapshot (src/Common/SimulationIO.hpp:193-270): header N, Nstar,
-    ndim, t; then x y z vx vy vz m h rho u per particle, at full precision."""
+def write_column_file(path, ic, stars=False, h=None):
+    """Reference column snapshot (src/Common/SimulationIO.hpp:193-270): header Nhydro, Nstar,
+    ndim, t; then x y z vx vy vz m h rho u per particle, at full precision.  With ``stars``
+    every particle is written as a star (Nhydro = 0) and ``h`` gives the stars' smoothing
+    lengths, which the reference's N-body sums and timesteps use."""
     r, v, m, u = ic["r"], ic["v"], ic["m"], ic["u"]
     N = r.shape[0]
-    cols = np.column_stack([r, v, m, np.zeros(N), np.zeros(N), u])
+    hh = np.zeros(N) if h is None else np.asarray(h, dtype=np.float64)
+    cols = np.column_stack([r, v, m, hh, np.zeros(N), u])
     with open(path, "w") as f:
-        f.write(f"{N}\n0\n3\n0.0\n")
+        f.write(f"0\n{N}\n3\n0.0\n" if stars else f"{N}\n0\n3\n0.0\n")
         np.savetxt(f, cols, fmt="%.17g")
 
 

@@ -60,6 +60,12 @@ typedef struct gdf_config {
    * "level_diff_max" (max. level difference to neighbours) */
   int Nlevels;
   int level_diff_max;
+  /* failure-path controls, 0 = the reference's behaviour: the fixed-point iterations of the
+   * h-rho iteration before it turns to bisection (GradhSph.cpp:156, iteration_max = 30; it
+   * fails after five times as many), and the candidate batches of 64 reserved per walk group
+   * before the first overflow regrows them (16) */
+  int hrho_iteration_max;
+  int cand_batches_init;
 } gdf_config;
 
 typedef struct gdf_ctx gdf_ctx;

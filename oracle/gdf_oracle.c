@@ -1169,6 +1169,12 @@ static inline void quad_force(const double* com, const double* q, double mc,
   *gpot += mc*invdrmag + 0.5*qscalar*invdr5;
 }
 
+/* the cell term above on its own, for its known-answer tests (a point target outside a cell) */
+void orc_cell_quad_force(const double* com, const double* q, double mc, const double* rp,
+                         double* agrav, double* gpot) {
+  quad_force(com, q, mc, rp, agrav, gpot);
+}
+
 #define GRAV_NEIB_SLACK (1.0 + 1.0e-4)
 
 /* act (optional, input order): groups without an active particle are skipped (counts 0) and

@@ -121,6 +121,11 @@ void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
                       const double* hfactor, double* atree, double* gpot,
                       int32_t* ncell, int32_t* nnear);
 
+/* one quadrupole cell term (NeighbourSearch.h:423-475): a cell of mass mc, centre of mass com
+ * and traceless moments q[5] acting on a point rp; adds to agrav[3] and *gpot */
+void orc_cell_quad_force(const double* com, const double* q, double mc, const double* rp,
+                         double* agrav, double* gpot);
+
 /* ---- integrator (SphLeapfrogKDK.cpp:76-272) and timesteps (SphIntegration.cpp:81-134) ---- */
 void orc_advance(int N, double dt, int energy_integration, const double* r0,
                  const double* v0, const double* a0, const double* u0, const double* dudt0,

@@ -83,6 +83,7 @@ def lib():
         L.orc_tree_build.restype = i
         L.orc_tree_refit_h.argtypes = [pp, C.POINTER(Tree), vp]
         L.orc_tree_gravity.argtypes = [pp, C.POINTER(Tree)] + [vp] * 9
+        L.orc_cell_quad_force.argtypes = [vp, vp, d, vp, vp, vp]
         L.orc_timesteps.argtypes = [pp, i] + [vp] * 7
         L.orc_timesteps.restype = d
         L.orc_diagnostics.argtypes = [pp, i] + [vp] * 6
@@ -213,6 +214,14 @@ def nbody_direct_smoothed(p: Params, r, v, m, h):
     lib().orc_nbody_direct_smoothed(C.byref(p), N, _p(r), _p(v), _p(m), _p(h), _p(a), _p(ad),
                                     _p(gp))
     return a, ad, gp
+
+
+def cell_quad_force(com, q, mc, rp):
+    """The oracle's quadrupole cell term at one point: (acceleration, potential)."""
+    a = np.zeros(3)
+    pot = np.zeros(1)
+    lib().orc_cell_quad_force(_p(_d(com)), _p(_d(q)), float(mc), _p(_d(rp)), _p(a), _p(pot))
+    return a, float(pot[0])
 
 
 class TreeTwin:

@@ -9,7 +9,8 @@
 //   int32 N, int32 nfields, then nfields arrays of N doubles, in FIELD_NAMES order.
 //
 // usage: ref_dump <paramfile> <nsteps> <outprefix> [dump_every]
-//   writes <outprefix>_s<step>.bin (step 0 = after setup) and <outprefix>_diag.txt, and the
+//   writes <outprefix>_s<step>.bin (step 0 = after setup; gas), <outprefix>_stars_s<step>.bin
+//   (stars: r v a adot m h gpot dt) and <outprefix>_diag.txt, and the
 //   reference's own column and Seren unformatted snapshots of each dumped step after setup
 #include <cstdio>
 #include <cstdlib>
@@ -19,6 +20,8 @@
 #include "Simulation.h"
 #include "Exception.h"
 #include "Particle.h"
+#include "Nbody.h"
+#include "NbodyParticle.h"
 
 static const char* FIELD_NAMES[] = {
   "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "atx", "aty", "atz",
@@ -68,6 +71,44 @@ static void dump(Simulation<3>* sim, const std::string& fname)
   fclose(fp);
 }
 
+// pure N-body runs (sim = nbody): the stars' state in the same flat format
+static const char* STAR_FIELDS[] = {
+  "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "adotx", "adoty", "adotz", "m", "h",
+  "gpot", "dt"
+};
+static const int NSTARFIELDS = sizeof(STAR_FIELDS)/sizeof(STAR_FIELDS[0]);
+
+static void dump_stars(Simulation<3>* sim, const std::string& fname)
+{
+  Nbody<3>* nbody = sim->nbody;
+  const int N = nbody->Nstar;
+  std::vector<double> buf((size_t) N*NSTARFIELDS);
+  for (int i=0; i<N; i++) {
+    const NbodyParticle<3>& p = nbody->stardata[i];
+    int k = 0;
+    for (int d=0; d<3; d++) buf[(size_t)(k++)*N + i] = p.r[d];
+    for (int d=0; d<3; d++) buf[(size_t)(k++)*N + i] = p.v[d];
+    for (int d=0; d<3; d++) buf[(size_t)(k++)*N + i] = p.a[d];
+    for (int d=0; d<3; d++) buf[(size_t)(k++)*N + i] = p.adot[d];
+    buf[(size_t)(k++)*N + i] = p.m;
+    buf[(size_t)(k++)*N + i] = p.h;
+    buf[(size_t)(k++)*N + i] = p.gpot;
+    buf[(size_t)(k++)*N + i] = p.dt;
+  }
+  FILE* fp = fopen(fname.c_str(), "wb");
+  if (!fp) { perror(fname.c_str()); exit(2); }
+  int hdr[2] = {N, NSTARFIELDS};
+  fwrite(hdr, sizeof(int), 2, fp);
+  fwrite(&buf[0], sizeof(double), buf.size(), fp);
+  fclose(fp);
+}
+
+static void dump_all(Simulation<3>* sim, const std::string& prefix, const std::string& tag)
+{
+  if (sim->hydro->Nhydro > 0) dump(sim, prefix + tag + ".bin");
+  if (sim->nbody->Nstar > 0) dump_stars(sim, prefix + "_stars" + tag + ".bin");
+}
+
 static void diag_line(Simulation<3>* sim, FILE* fp)
 {
   sim->CalculateDiagnostics();
@@ -108,14 +149,14 @@ int main(int argc, char** argv)
   fprintf(dfp, "# Nsteps t timestep Etot ketot utot gpetot momx momy momz angx angy angz mtot"
           " n level_max level_step nresync dt_max nnew\n");
   diag_line(sim, dfp);
-  dump(sim, prefix + "_s0.bin");
+  dump_all(sim, prefix, "_s0");
   for (int s=1; s<=nsteps; s++) {
     sim->MainLoop();
     diag_line(sim, dfp);
     if (every > 0 && (s % every == 0 || s == nsteps)) {
       char name[64];
-      snprintf(name, sizeof(name), "_s%d.bin", s);
-      dump(sim, prefix + name);
+      snprintf(name, sizeof(name), "_s%d", s);
+      dump_all(sim, prefix, name);
       snprintf(name, sizeof(name), "_s%d.col", s);
       sim->WriteColumnSnapshotFile(prefix + name);
       snprintf(name, sizeof(name), "_s%d.su", s);

@@ -59,8 +59,40 @@ CASES = {
     "cloud4k_quintic_blk": (lambda: ic.rotating_m2_cloud(N=4096, seed=16, rho_bary=1.0,
                                                          nlevels=8),
                             {"neib_search": "bruteforce"}, 12),
+    # h_converge = 0: the convergence test never passes, so every particle leaves the
+    # fixed-point iteration after 30 iterations and finishes on the bisection branch, when the
+    # bracket closes at FP64 resolution (GradhSph.cpp:219-255)
+    "box8_m4_bisect": (lambda: ic.jittered_lattice_box(n_side=8, seed=21),
+                       {"h_converge": 0.0}, 0),
+    # 100-step runs for the reference's own energy error (diagnostics rows only): the hydro
+    # box, and the self-gravitating cold sphere through the reference KD-tree with adiabatic
+    # gas (an isothermal gas does not conserve the total energy)
+    "box16_m4_e100": (lambda: ic.jittered_lattice_box(n_side=16, seed=19), {}, 100),
+    "sphere4k_m4_kd_e100": (lambda: ic.cold_sphere(N=4096, seed=20),
+                            {"gas_eos": "energy_eqn"}, 100),
+    # pure N-body: unsoftened and kernel-smoothed star-star direct sums, 5 KDK steps
+    "plummer512_direct": (lambda: _plummer_stars(512, 17, 0), {}, 5),
+    "plummer512_smoothed": (lambda: _plummer_stars(512, 18, 1), {}, 5),
 }
 BLOCK_KEEP = ["level", "nstep", "nlast", "tlast", "levelneib", "dt"]
+STAR_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "adotx", "adoty", "adotz",
+               "m", "h", "gpot", "dt"]
+
+
+def _plummer_stars(N, seed, softening):
+    """A Plummer cluster run as the reference's pure N-body simulation (sim = nbody) with the
+    leapfrog KDK integrator and direct sums: unsoftened (nbody_softening = 0,
+    Nbody.cpp:233-293) or kernel-smoothed over the mean of the two stars' h
+    (nbody_softening = 1, NbodyLeapfrogKDK.cpp:78-150).  The stars' h are seeded in
+    [0.05, 0.35] so that a good share of pairs falls inside the M4 kernel."""
+    g = ic.plummer_sphere(N=N, seed=seed)
+    g["h"] = 0.05 + 0.3 * np.random.default_rng(seed + 1000).random(N)
+    g["params"] = {k: v for k, v in g["params"].items()
+                   if k not in ("nbody_eps", "fixed_dt", "hydro_forces", "self_gravity")}
+    g["params"].update({"nbody": "lfkdk", "nbody_softening": softening, "nbody_mult": 0.1,
+                        "kernel": "m4", "tabulated_kernel": 0})
+    g["stars"] = True
+    return g
 
 
 def run_case(name, gen, over, nsteps, outdir):
@@ -70,13 +102,30 @@ def run_case(name, gen, over, nsteps, outdir):
     with tempfile.TemporaryDirectory() as td:
         col = os.path.join(td, "in.col")
         dat = os.path.join(td, "in.dat")
-        ic.write_column_file(col, g)
+        stars = g.get("stars", False)
+        ic.write_column_file(col, g, stars=stars, h=g.get("h"))
         ic.write_param_file(dat, prm, col, "FIX")
         steps = max(nsteps, 0)
         subprocess.run([REF, dat, str(steps), os.path.join(td, "out"), str(max(steps, 1))],
                        cwd=td, check=True, stdout=subprocess.DEVNULL)
-        s0 = _load(os.path.join(td, "out_s0.bin"))
+        if stars:
+            arrs = {"in_r": g["r"], "in_v": g["v"], "in_m": g["m"], "in_h": g["h"]}
+            for s in sorted({0, nsteps}):
+                b = open(os.path.join(td, f"out_stars_s{s}.bin"), "rb").read()
+                n, nf = np.frombuffer(b[:8], np.int32)
+                a = np.frombuffer(b[8:], np.float64).reshape(nf, n)
+                for k in range(nf):
+                    if STAR_FIELDS[k] not in ("m", "h"):
+                        arrs[f"s{s}_{STAR_FIELDS[k]}"] = a[k].copy()
+            arrs["diag"] = np.atleast_2d(np.loadtxt(os.path.join(td, "out_diag.txt")))
+            _save(name, g, prm, nsteps, arrs, outdir)
+            return
         arrs = {"in_r": g["r"], "in_v": g["v"], "in_m": g["m"], "in_u": g["u"]}
+        if name.endswith("_e100"):
+            arrs["diag"] = np.atleast_2d(np.loadtxt(os.path.join(td, "out_diag.txt")))
+            _save(name, g, prm, nsteps, arrs, outdir)
+            return
+        s0 = _load(os.path.join(td, "out_s0.bin"))
         keep = KEEP + (BLOCK_KEEP if prm.get("Nlevels", 1) > 1 else [])
         for k in keep:
             arrs[f"s0_{k}"] = s0[k]
@@ -94,6 +143,10 @@ def run_case(name, gen, over, nsteps, outdir):
                 arrs[f"s{nsteps}_sf"] = np.frombuffer(f.read(), dtype=np.uint8)
         diag = np.loadtxt(os.path.join(td, "out_diag.txt"))
         arrs["diag"] = np.atleast_2d(diag)
+    _save(name, g, prm, nsteps, arrs, outdir)
+
+
+def _save(name, g, prm, nsteps, arrs, outdir):
     meta = {"params": prm, "nsteps": nsteps, "build": "serial, COMPILER_MODE=STANDARD, "
             "PRECISION=DOUBLE, DEBUG_LEVEL=1 (oracle/Makefile.ref)",
             "diag_columns": "Nsteps t timestep Etot ketot utot gpetot momx momy momz angx "

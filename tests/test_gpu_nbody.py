@@ -113,3 +113,37 @@ def test_plummer_energy_conservation():
     assert n == 200 and dt == pytest.approx(1e-3)
     print(f"energy {e0:.10f} -> {e1:.10f}, relative drift {abs(e1 - e0) / abs(e0):.3e}")
     assert abs(e1 - e0) / abs(e0) < 1e-3
+
+
+@pytest.mark.parametrize("case,smoothed", [("plummer512_direct", 0),
+                                           ("plummer512_smoothed", 1)])
+def test_nbody_direct_and_trajectory_match_reference(case, smoothed):
+    """The reference's pure N-body run (NbodySimulation with NbodyLeapfrogKDK and direct sums,
+    NbodySimulation.cpp:183-375): the device's setup forces and its five KDK steps against the
+    reference's fixture.  a and gpot to 1e-12; the smoothed jerk within the float rounding of
+    its single-precision (2/(h_i+h_j))^3 factor (see test_nbody_direct_matches_oracle); the
+    timesteps and time of every step to 1e-12 and the final r, v to 1e-11."""
+    from conftest import load_golden, vec
+    from gandalf_amd.gpu import GpuSim
+    d = load_golden(case)
+    prm = d["meta"]["params"]
+    s = GpuSim(prm)
+    s.set_particles(d["in_r"], d["in_v"], d["in_m"], np.zeros(len(d["in_m"])), h=d["in_h"])
+    s.setup()
+    assert relmax(s.get("a"), vec(d, "s0", ("ax", "ay", "az"))) < 1e-12
+    assert relmax(s.get("gpot"), d["s0_gpot"]) < 1e-12
+    adot = s.get("adot")
+    assert relmax(adot, vec(d, "s0", ("adotx", "adoty", "adotz"))) < (1e-12 if not smoothed
+                                                                      else 1e-5)
+    diag = d["diag"]
+    assert s.time()[1] == pytest.approx(diag[0, 2], rel=1e-12)
+    n = d["meta"]["nsteps"]
+    for k in range(1, n + 1):
+        s.step()
+        t, dt, steps = s.time()
+        assert steps == k
+        assert t == pytest.approx(diag[k, 1], rel=1e-12), k
+        assert dt == pytest.approx(diag[k, 2], rel=1e-12), k
+    assert relmax(s.get("r"), vec(d, f"s{n}")) < 1e-11
+    assert relmax(s.get("v"), vec(d, f"s{n}", ("vx", "vy", "vz"))) < 1e-11
+    assert relmax(s.get("a"), vec(d, f"s{n}", ("ax", "ay", "az"))) < 1e-10

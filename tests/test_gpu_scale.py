@@ -116,5 +116,5 @@ def test_mixed_precision_large():
           f"median {np.median(e_setup):.3e}")
     assert np.max(np.abs(h32 / h64 - 1)) < 1e-5
     assert np.max(e) < 1e-4
-    assert np.median(e_setup) < 1e-5
-    assert np.max(e_setup) < 2e-3
+    assert np.median(e_setup) < 1e-6
+    assert np.max(e_setup) < 1e-4

@@ -166,3 +166,99 @@ def test_nbody_direct_third_law_and_newtonian_limit():
     assert np.allclose(a2, a, rtol=1e-12, atol=0)
     assert np.allclose(gp2, gp, rtol=1e-12, atol=0)
     assert np.allclose(ad2, ad, rtol=1e-9, atol=1e-12)
+
+
+def test_direct_plummer_two_body_closed_form():
+    """orc_direct_plummer on two and three bodies: the Plummer-softened pair acceleration
+    m dr / (r^2 + eps^2)^(3/2) and potential -m / sqrt(r^2 + eps^2), superposed over the
+    sources, with the target itself excluded."""
+    r = np.array([[0.1, -0.2, 0.3], [0.4, 0.2, 0.35], [-0.5, 0.05, 0.0]])
+    m = np.array([2.0, 3.0, 0.7])
+    for eps in (0.0, 0.01, 0.3):
+        a, pot = O.direct_plummer(r[:2], m[:2], eps, [0, 1])
+        dr = r[1] - r[0]
+        s2 = dr @ dr + eps * eps
+        assert np.allclose(a[0], m[1] * dr / s2 ** 1.5, rtol=1e-14, atol=0)
+        assert np.allclose(a[1], -m[0] * dr / s2 ** 1.5, rtol=1e-14, atol=0)
+        assert pot[0] == pytest.approx(-m[1] / np.sqrt(s2), rel=1e-14)
+        assert pot[1] == pytest.approx(-m[0] / np.sqrt(s2), rel=1e-14)
+        a3, pot3 = O.direct_plummer(r, m, eps, [2])
+        want_a = sum(m[j] * (r[j] - r[2]) / ((r[j] - r[2]) @ (r[j] - r[2]) + eps * eps) ** 1.5
+                     for j in (0, 1))
+        want_p = sum(-m[j] / np.sqrt((r[j] - r[2]) @ (r[j] - r[2]) + eps * eps) for j in (0, 1))
+        assert np.allclose(a3[0], want_a, rtol=1e-14, atol=0)
+        assert pot3[0] == pytest.approx(want_p, rel=1e-14)
+
+
+def test_direct_plummer_unsoftened_limit_is_nbody_direct():
+    """eps -> 0: the Plummer sum becomes the unsoftened star-star sum, whose oracle is pinned
+    against the reference (Nbody::CalculateDirectGravForces fixture); the softened sum departs
+    from it as O(eps^2 / r^2)."""
+    rng = np.random.default_rng(5)
+    N = 200
+    r = rng.normal(size=(N, 3))
+    m = 0.1 + rng.random(N)
+    a0, _, gp0 = O.nbody_direct(r, np.zeros((N, 3)), m)
+    t = np.arange(N)
+    a, pot = O.direct_plummer(r, m, 0.0, t)
+    assert np.allclose(a, a0, rtol=1e-12, atol=1e-14 * np.abs(a0).max())
+    assert np.allclose(-pot, gp0, rtol=1e-12, atol=0)
+    d2min = min(((r[i] - r[j]) ** 2).sum() for i in range(N) for j in range(i))
+    prev = None
+    for eps in (1e-2, 1e-3, 1e-4):
+        a, pot = O.direct_plummer(r, m, eps, t)
+        err = np.abs(-pot - gp0).max() / np.abs(gp0).max()
+        assert err < 2.0 * eps * eps / d2min
+        if prev is not None:
+            assert err < prev * 0.02       # quadratic in eps
+        prev = err
+
+
+def _cluster_exact(rs, ms, rp):
+    dr = rs - rp
+    d = np.sqrt((dr ** 2).sum(1))
+    return (ms[:, None] * dr / d[:, None] ** 3).sum(0), (ms / d).sum()
+
+
+@pytest.mark.parametrize("kind,order", [("generic", 3), ("symmetric_pair", 4)])
+def test_cell_quadrupole_far_field_order(kind, order):
+    """The quadrupole cell term (NeighbourSearch.h:423-475) against the exact sum over the
+    cell's members: the monopole alone errs as (d/R)^2; adding the quadrupole leaves the
+    octupole, (d/R)^3, or (d/R)^4 for a symmetric pair whose octupole vanishes.  A dropped
+    term, a wrong sign or a transposed moment leaves a (d/R)^2 error and fails the slope."""
+    rng = np.random.default_rng(6)
+    if kind == "generic":
+        rs = rng.normal(scale=0.1, size=(12, 3)) + np.array([0.3, -0.2, 0.1])
+        ms = 0.5 + rng.random(12)
+    else:
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        rs = np.array([0.3, -0.2, 0.1]) + np.outer([0.1, -0.1], axis)
+        ms = np.array([1.5, 1.5])
+    # the moments as the tree twin builds them for the node holding every member (the root),
+    # pinned against brute force in test_tree_twin_moments_brute_force
+    tw = O.TreeTwin(params("m4"), rs, ms, np.full(len(ms), 1e-3)).arrays()
+    com, q, mc = tw["com"][0], tw["quad"][0], tw["mass"][0]
+    assert mc == pytest.approx(ms.sum(), rel=1e-14)
+    size = np.sqrt((((rs - com) ** 2).sum(1)).max())
+    dirs = rng.normal(size=(16, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    Rs = size * np.array([8.0, 16.0, 32.0, 64.0])
+    eq, ep, em = [], [], []
+    for R in Rs:
+        wq, wp, wm = 0.0, 0.0, 0.0
+        for u in dirs:
+            rp = com + R * u
+            ax, px = _cluster_exact(rs, ms, rp)
+            aq, pq = O.cell_quad_force(com, q, mc, rp)
+            am, _ = O.cell_quad_force(com, np.zeros(5), mc, rp)
+            # the cell term accumulates the acceleration the cell exerts on the point
+            wq = max(wq, np.linalg.norm(aq - ax) / np.linalg.norm(ax))
+            wp = max(wp, abs(pq - px) / px)
+            wm = max(wm, np.linalg.norm(am - ax) / np.linalg.norm(ax))
+        eq.append(wq); ep.append(wp); em.append(wm)
+    slope = lambda e: np.polyfit(np.log(Rs), np.log(e), 1)[0]
+    assert slope(em) == pytest.approx(-2.0, abs=0.25)
+    assert slope(eq) < -(order - 0.2)
+    assert slope(ep) < -(order - 0.2)
+    assert eq[0] < 0.2 * em[0]

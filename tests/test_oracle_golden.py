@@ -124,3 +124,63 @@ def test_block_timesteps_match_reference(case):
     assert relmax(s.get("v"), vec(d, f"s{nsteps}", ("vx", "vy", "vz"))) < 1e-9
     assert relmax(s.get("u"), d[f"s{nsteps}_u"]) < 1e-11
     assert relmax(s.get("h"), d[f"s{nsteps}_h"]) < 1e-11
+
+
+@pytest.mark.parametrize("case,smoothed", [("plummer512_direct", 0),
+                                           ("plummer512_smoothed", 1)])
+def test_nbody_direct_sums_match_reference(case, smoothed):
+    """Star-star direct sums of the reference's pure N-body run after setup: unsoftened
+    (Nbody::CalculateDirectGravForces, Nbody.cpp:233-293) and kernel-smoothed over
+    2/(h_i + h_j) (NbodyLeapfrogKDK::CalculateDirectSmoothedGravForces, :78-150), a, jerk and
+    potential to 1e-12 of their maxima."""
+    d = load_golden(case)
+    r, v, m, h = d["in_r"], d["in_v"], d["in_m"], d["in_h"]
+    assert np.array_equal(vec(d, "s0"), r)
+    if smoothed:
+        p = O.params_from_dict(d["meta"]["params"])
+        a, ad, gp = O.nbody_direct_smoothed(p, r, v, m, h)
+        # every star has several partners inside the kernel, so the smoothed branch is exercised
+        dr = r[:, None, :] - r[None, :, :]
+        s = np.sqrt((dr ** 2).sum(-1)) * 2.0 / (h[:, None] + h[None, :])
+        assert (s < 2.0).sum() - len(m) > 5 * len(m)
+    else:
+        a, ad, gp = O.nbody_direct(r, v, m)
+    assert relmax(a, vec(d, "s0", ("ax", "ay", "az"))) < 1e-12
+    assert relmax(ad, vec(d, "s0", ("adotx", "adoty", "adotz"))) < 1e-12
+    assert relmax(gp, d["s0_gpot"]) < 1e-12
+
+
+DIAG_CASES = [("box16_m4", 0), ("box16_m4", 10), ("box16_m4_tab", 10), ("sphere4k_m4_bf", 0),
+              ("sphere4k_m4_bf", 3), ("cloud4k_quintic_bf", 3), ("blob16_m4_blk", 24),
+              ("cloud4k_quintic_blk", 12)]
+
+
+def diag_scales(r, v, m):
+    """Magnitudes against which the momentum and angular momentum (sums that cancel to near
+    zero) are compared."""
+    pm = (m[:, None] * np.abs(v)).sum()
+    lm = (m * np.linalg.norm(r, axis=1) * np.linalg.norm(v, axis=1)).sum()
+    return pm, lm
+
+
+def check_diag_row(out, row, r, v, m, tol=1e-12):
+    """out = {Etot, ketot, utot, gpetot, mom[3], angmom[3], mtot} against a reference .diag row
+    (columns Nsteps t timestep Etot ketot utot gpetot momx momy momz angx angy angz mtot)."""
+    pm, lm = diag_scales(r, v, m)
+    for k, name in enumerate(("Etot", "ketot", "utot", "gpetot")):
+        assert out[k] == pytest.approx(row[3 + k], rel=tol, abs=tol * abs(row[3])), name
+    assert np.allclose(out[4:7], row[7:10], rtol=0, atol=tol * pm), "mom"
+    assert np.allclose(out[7:10], row[10:13], rtol=0, atol=tol * lm), "angmom"
+    assert out[10] == pytest.approx(row[13], rel=1e-14), "mtot"
+
+
+@pytest.mark.parametrize("case,step", DIAG_CASES)
+def test_diagnostics_match_reference_rows(case, step):
+    """Simulation::CalculateDiagnostics (SimAnalysis.hpp:52-185) restated: every column of
+    the reference's diagnostics (energy terms, momentum, angular momentum, mass) from the
+    reference's own particle state at that step."""
+    d = load_golden(case)
+    p = O.params_from_dict(d["meta"]["params"])
+    r, v = vec(d, f"s{step}"), vec(d, f"s{step}", ("vx", "vy", "vz"))
+    out = O.diagnostics(p, r, v, d["in_m"], d[f"s{step}_u"], d[f"s{step}_gpot"])
+    check_diag_row(out, d["diag"][step], r, v, d["in_m"])
