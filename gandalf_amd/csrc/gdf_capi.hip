@@ -388,7 +388,11 @@ void gdf_destroy(gdf_ctx* c) {
   delete c;
 }
 
-const char* gdf_last_error(const gdf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* gdf_last_error(const gdf_ctx* c) {
+  if (c) return c->err.c_str();
+  const char* e = noalloc_last_error();     // the context-free entry point's last failure
+  return e ? e : "null context";
+}
 
 int gdf_num_particles(const gdf_ctx* c) { return c ? c->N : 0; }
 

@@ -9,6 +9,10 @@
 // so positions and velocities are reproduced bit for bit from identical inputs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <mutex>
+#include <string>
+
 #include "gdf_internal.h"
 #include "gdf_device.h"
 
@@ -168,8 +172,10 @@ __global__ void k_max_h(int N, const double* __restrict__ h, double* __restrict_
 
 constexpr int ND = 11;  // mtot ke u gpe mom[3] ang[3] (+pad)
 
+// stride: elements between one particle's r (and v) component and the next particle's (1 for
+// the context's per-component arrays, 3 for (N,3) row-major arrays)
 struct DiagArgs {
-  int N;
+  int N, stride;
   const double* r[3]; const double* v[3]; const double* m; const double* u; const double* gpot;
   double* part;
 };
@@ -180,8 +186,9 @@ __global__ void k_diag(DiagArgs d) {
   for (int k = 0; k < ND; k++) acc[k] = 0.0;
   for (int i = blockIdx.x*BS + threadIdx.x; i < d.N; i += gridDim.x*BS) {
     const double m = d.m[i];
-    const double rx = d.r[0][i], ry = d.r[1][i], rz = d.r[2][i];
-    const double vx = d.v[0][i], vy = d.v[1][i], vz = d.v[2][i];
+    const size_t o = (size_t) i*d.stride;
+    const double rx = d.r[0][o], ry = d.r[1][o], rz = d.r[2][o];
+    const double vx = d.v[0][o], vy = d.v[1][o], vz = d.v[2][o];
     acc[0] += m;
     acc[1] += m*(vx*vx + vy*vy + vz*vz);
     acc[2] += m*d.u[i];
@@ -199,6 +206,29 @@ __global__ void k_diag(DiagArgs d) {
     __syncthreads();
   }
   if (threadIdx.x < ND) d.part[blockIdx.x*ND + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// The block partials of k_diag summed in block order, then the diagnostics as
+// integ_diagnostics forms them on the host: out = {Etot, ketot, utot, gpetot, mom[3],
+// angmom[3], mtot}.  One block of 64 threads.
+__global__ void k_diag_final(int nb, const double* __restrict__ part, int hydro, int grav,
+                             double* __restrict__ out) {
+  __shared__ double acc[ND];
+  const int k = threadIdx.x;
+  if (k < ND) {
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[b*ND + k];
+    acc[k] = s;
+  }
+  __syncthreads();
+  if (k != 0) return;
+  const double ketot = 0.5*acc[1], gpetot = 0.5*acc[3];
+  double Etot = ketot;
+  if (hydro) Etot += acc[2];
+  if (grav) Etot += gpetot;
+  out[0] = Etot; out[1] = ketot; out[2] = acc[2]; out[3] = gpetot;
+  for (int j = 0; j < 3; j++) { out[4 + j] = acc[4 + j]; out[7 + j] = acc[7 + j]; }
+  out[10] = acc[0];
 }
 
 __global__ void k_zero_d(int n, double* __restrict__ p) {
@@ -359,7 +389,7 @@ int integ_max_h(gdf_ctx* c, double* hmax) {
 
 int integ_diagnostics(gdf_ctx* c, double* out) {
   DiagArgs d;
-  d.N = c->N;
+  d.N = c->N; d.stride = 1;
   for (int k = 0; k < 3; k++) { d.r[k] = c->pf[F_X + k]; d.v[k] = c->pf[F_VX + k]; }
   d.m = c->pf[F_M]; d.u = c->pf[F_U]; d.gpot = c->df[D_GPOT]; d.part = c->red;
   const int nb = std::min(nblocks(c->N, BS), RED_BLOCKS);
@@ -779,3 +809,93 @@ int block_end_step(gdf_ctx* c) {
 }
 
 }  // namespace gdf
+
+// ---- diagnostics with no device allocation ----------------------------------------------------
+// The block partials of k_diag live in a file-scope device array instead of a context's
+// reduction buffer: RED_BLOCKS x ND doubles (88 KiB), held by every GPU the library runs on
+// until the process ends.  Host code never passes the array's name to a kernel (on the host it
+// names a stand-in, not device memory): the address comes from hipGetSymbolAddress, looked up
+// once per device with that device current, and kept per device.  Every call writes the
+// partials it reads (blocks 0..nb-1), so nothing needs clearing between calls; calls that
+// overlap on one GPU share the array and race.
+namespace gdf {
+namespace {
+
+__device__ __align__(256) double scratch[RED_BLOCKS*ND];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+thread_local std::string g_noalloc_err;
+
+int noalloc_fail(int code, const char* msg) {
+  g_noalloc_err = msg;
+  return code;
+}
+
+// device address of this device's scratch array (looked up on the first call on the device)
+int scratch_on_current_device(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return noalloc_fail(GDF_ERR_HIP, "gdf_calculate_diagnostics_noalloc: no current device");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return noalloc_fail(GDF_ERR_HIP, "gdf_calculate_diagnostics_noalloc: "
+                                       "hipGetSymbolAddress of the scratch array failed");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return GDF_OK;
+}
+
+// a pointer a kernel on this device may read or write: device, pinned host or managed memory
+bool device_accessible(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void) hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeHost ||
+         a.type == hipMemoryTypeManaged || a.type == hipMemoryTypeUnified;
+}
+
+}  // namespace
+
+const char* noalloc_last_error() {
+  return g_noalloc_err.empty() ? nullptr : g_noalloc_err.c_str();
+}
+
+}  // namespace gdf
+
+extern "C" int gdf_calculate_diagnostics_noalloc(int N, const double* r, const double* v,
+                                                 const double* m, const double* u,
+                                                 const double* gpot, int hydro_forces,
+                                                 int self_gravity, double* out, void* stream) {
+  using namespace gdf;
+  if (N < 1 || (long long) N > kNoallocMaxN)
+    return noalloc_fail(GDF_ERR_INVALID, "gdf_calculate_diagnostics_noalloc: N must be in "
+                                         "[1, 2^30]");
+  const void* ptrs[] = {r, v, m, u, gpot, out};
+  for (const void* p : ptrs)
+    if (!p || !device_accessible(p))
+      return noalloc_fail(GDF_ERR_INVALID, "gdf_calculate_diagnostics_noalloc: every array "
+                                           "must be device-accessible memory");
+  double* part = nullptr;
+  GDF_TRY(scratch_on_current_device(&part));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DiagArgs d;
+  d.N = N; d.stride = 3;
+  for (int k = 0; k < 3; k++) { d.r[k] = r + k; d.v[k] = v + k; }
+  d.m = m; d.u = u; d.gpot = gpot; d.part = part;
+  const int nb = std::min(nblocks(N, BS), RED_BLOCKS);
+  k_diag<<<nb, BS, 0, s>>>(d);
+  k_diag_final<<<1, 64, 0, s>>>(nb, part, hydro_forces != 0, self_gravity != 0, out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return noalloc_fail(GDF_ERR_HIP, "gdf_calculate_diagnostics_noalloc: kernel launch failed");
+  g_noalloc_err.clear();
+  return GDF_OK;
+}

@@ -274,4 +274,9 @@ int nbody_direct(gdf_ctx* c, int smoothed);
 
 inline int nblocks(long long n, int bs) { return (int) ((n + bs - 1) / bs); }
 
+// gdf_calculate_diagnostics_noalloc: largest particle count, and the message of its last
+// failure on this thread (NULL after a successful call)
+constexpr long long kNoallocMaxN = 1LL << 30;
+const char* noalloc_last_error();
+
 }  // namespace gdf

@@ -93,6 +93,7 @@ PROTOTYPES = {
     "gdf_pass_times": (_i, [_vp, _vp]),
     "gdf_set_timing": (_i, [_vp, _i]),
     "gdf_nbody_direct_forces": (_i, [_vp, _i]),
+    "gdf_calculate_diagnostics_noalloc": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp]),
 }
 
 

@@ -216,6 +216,20 @@ int gdf_pool_get_field(gdf_pool* pool, int id, const char* name, double* out);
 int gdf_pool_release(gdf_pool* pool, int id);
 int64_t gdf_pool_resident_particles(const gdf_pool* pool);
 
+/* Simulation::CalculateDiagnostics as gdf_calculate_diagnostics, on arrays the caller owns in
+ * device memory, allocating nothing: no counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  r, v: (N,3) row-major; m, u, gpot: N doubles; out: 11 doubles of device memory
+ * receiving {Etot, ketot, utot, gpetot, mom[3], angmom[3], mtot}; utot enters Etot when
+ * hydro_forces, gpetot when self_gravity.  All arrays must be device-accessible (device,
+ * pinned host or managed memory), 1 <= N <= 2^30.  Asynchronous on stream (a hipStream_t, or
+ * NULL for the null stream).  The block partials live in one fixed 88 KiB device array per GPU,
+ * shared by every call on that GPU: calls that overlap on one GPU race.  On failure the
+ * message is gdf_last_error(NULL). */
+int gdf_calculate_diagnostics_noalloc(int N, const double* r, const double* v, const double* m,
+                                      const double* u, const double* gpot, int hydro_forces,
+                                      int self_gravity, double* out, void* stream);
+
 /* event timing of the named passes of the last step, milliseconds:
  * {tree, density, hydro, gravity, integrate, gravity walk kernel alone (summed over its
  * launches), number of gravity walk launches in the step (block timesteps repeat the force pass
