@@ -157,11 +157,21 @@ def cpu_baseline(nsample=131072, steps=4):
         done += s.block_state()[5]
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": done / dt, "unit": "particle-steps/s", "cores": cores,
-            "kind": "port",
-            "sample": f"rotating m=2 cloud at N={nsample}, block timesteps ({NLEVELS} levels), "
-                      f"{steps} base steps after setup ({done} particle-steps), "
-                      f"oracle/gdf_oracle.c OpenMP"}
+    out = {"value": done / dt, "unit": "particle-steps/s", "cores": cores,
+           "kind": "port",
+           "sample": f"rotating m=2 cloud at N={nsample}, block timesteps ({NLEVELS} levels), "
+                     f"{steps} base steps after setup ({done} particle-steps), "
+                     f"oracle/gdf_oracle.c OpenMP"}
+    # the oracle's speed relative to the compiled reference on one thread, measured in the
+    # build container by oracle/calibrate_cpu.py (the reference cannot run on the GPU host)
+    cal = os.path.join(ROOT, "profiles", "cpu_calibration.json")
+    if os.path.exists(cal):
+        with open(cal) as f:
+            c = json.load(f)
+        out["oracle_over_reference_time"] = {
+            k: v["oracle_over_reference"] for k, v in c["cases"].items()}
+        out["calibration"] = "profiles/cpu_calibration.json (1 thread, step loop only)"
+    return out
 
 
 def ensemble(nreq=256):

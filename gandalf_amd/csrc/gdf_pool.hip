@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <new>
 #include <string>
@@ -51,6 +52,9 @@ struct gdf_pool {
   std::map<int, gdf::PoolResult> fin;       // finished
   int next_id = 0;
   std::string err;
+  // set when a failure left the resident requests' state unusable (an allocation that could
+  // not be undone, or a step that stopped half way): every later admit and step is refused
+  bool failed = false;
 };
 
 namespace gdf {
@@ -66,33 +70,75 @@ int pool_ctx_fail(gdf_pool* P, gdf_ctx* c, int rc) {
   return rc;
 }
 
-// grow the batch's particle arrays to hold `need` particles, keeping the resident state
-int batch_reserve(gdf_ctx* c, int need, int64_t maxp) {
+// Fault injection for the failure-path tests: with GDF_FAULT_POOL_GROW=1 in the environment,
+// growing a batch reports an allocation failure after the grown arrays were allocated (the
+// state a failed hipMalloc part way through leaves behind).
+bool grow_fault_injected() {
+  const char* e = std::getenv("GDF_FAULT_POOL_GROW");
+  return e && e[0] == '1';
+}
+
+// Grow the batch's particle arrays to hold `need` particles, keeping the resident state.  If
+// the grown arrays cannot be allocated, the arrays are made again at the previous capacity and
+// the resident state is copied back: the call fails with the batch as it was (*lost = false).
+// Only if that fails too is the resident state gone (*lost = true).
+int batch_reserve(gdf_ctx* c, int need, int64_t maxp, bool* lost) {
+  *lost = false;
   if (need <= c->Ncap) return GDF_OK;
   int ncap = std::max(need, c->Ncap + c->Ncap/2);
   if ((int64_t) ncap > maxp) ncap = (int) std::max<int64_t>(need, maxp);
-  const int N = c->N;
+  const int N = c->N, old_cap = c->Ncap;
   double* keep[F_NPERSIST];
   for (int f = 0; f < F_NPERSIST; f++) { keep[f] = c->pf[f]; c->pf[f] = nullptr; }
   int* keep_io = c->iorig;
   c->iorig = nullptr;
-  GDF_HIP(c, hipStreamSynchronize(c->stream));
-  const int rc = ctx_alloc_particles(c, ncap);
-  if (rc == GDF_OK)        // derived fields a class never computes (density of N-body) read 0
-    for (int f = 0; f < D_NDERIVED; f++)
-      GDF_HIP(c, hipMemsetAsync(c->df[f], 0, sizeof(double)*ncap, c->stream));
-  if (rc == GDF_OK && N > 0) {
-    for (int f = 0; f < F_NPERSIST; f++)
-      GDF_HIP(c, hipMemcpyAsync(c->pf[f], keep[f], sizeof(double)*N, hipMemcpyDeviceToDevice,
-                                c->stream));
-    GDF_HIP(c, hipMemcpyAsync(c->iorig, keep_io, sizeof(int)*N, hipMemcpyDeviceToDevice,
-                              c->stream));
-    GDF_HIP(c, hipStreamSynchronize(c->stream));
+  auto release_kept = [&]() {
+    for (int f = 0; f < F_NPERSIST; f++) if (keep[f]) (void) hipFree(keep[f]);
+    if (keep_io) (void) hipFree(keep_io);
+  };
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    release_kept();
+    *lost = true;
+    return fail(c, GDF_ERR_HIP, "batch grow: stream synchronisation failed");
   }
-  for (int f = 0; f < F_NPERSIST; f++) if (keep[f]) hipFree(keep[f]);
-  if (keep_io) hipFree(keep_io);
+  int rc = ctx_alloc_particles(c, ncap);
+  if (rc == GDF_OK && grow_fault_injected())
+    rc = fail(c, GDF_ERR_NOMEM, "batch grow: injected allocation failure");
+  std::string why;
+  if (rc != GDF_OK) {
+    why = c->err;
+    if (old_cap < 1 || ctx_alloc_particles(c, old_cap) != GDF_OK) {
+      release_kept();
+      *lost = true;
+      return fail(c, rc, why + "; the resident requests could not be restored");
+    }
+    ncap = old_cap;
+  }
+  hipError_t e = hipSuccess;
+  // derived fields a class never computes (density of N-body) read 0
+  for (int f = 0; f < D_NDERIVED && e == hipSuccess; f++)
+    e = hipMemsetAsync(c->df[f], 0, sizeof(double)*ncap, c->stream);
+  if (N > 0) {
+    for (int f = 0; f < F_NPERSIST && e == hipSuccess; f++)
+      e = hipMemcpyAsync(c->pf[f], keep[f], sizeof(double)*N, hipMemcpyDeviceToDevice,
+                         c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->iorig, keep_io, sizeof(int)*N, hipMemcpyDeviceToDevice, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  release_kept();
+  if (e != hipSuccess) {
+    *lost = true;
+    return check_hip(c, e, "batch grow: copying the resident state");
+  }
   c->N = N;
-  return rc;
+  // the new integer arrays need the resident segments' ids again
+  if (N > 0) {
+    const int rs = seg_upload(c);
+    if (rs != GDF_OK) { *lost = true; return rs; }
+  }
+  if (rc != GDF_OK) return fail(c, rc, why + " (the resident requests are unchanged)");
+  return GDF_OK;
 }
 
 // keep the listed segments (in order), closing the gaps left by the others
@@ -187,13 +233,17 @@ void gdf_pool_destroy(gdf_pool* P) {
   delete P;
 }
 
-const char* gdf_pool_last_error(const gdf_pool* P) { return P ? P->err.c_str() : "null pool"; }
+const char* gdf_pool_last_error(const gdf_pool* P) {
+  if (!P) return "null pool";
+  return P->failed && P->err.empty() ? "pool failed" : P->err.c_str();
+}
 
 int64_t gdf_pool_resident_particles(const gdf_pool* P) { return P ? P->batch->N : 0; }
 
 int gdf_pool_admit(gdf_pool* P, int N, const double* r, const double* v, const double* m,
                    const double* u, int nsteps, int* id) {
   if (!P || !id || N < 2 || nsteps < 1 || !r || !v || !m || !u) return GDF_ERR_INVALID;
+  if (P->failed) return GDF_ERR_INVALID;
   gdf_ctx* c = P->batch;
   if ((int64_t) c->N + N > P->max_particles || (int) P->res.size() >= kMaxSeg)
     return pool_fail(P, GDF_ERR_CAPACITY, "pool full: the request waits for resident ones to finish");
@@ -214,7 +264,12 @@ int gdf_pool_admit(gdf_pool* P, int N, const double* r, const double* v, const d
   if (rc != GDF_OK) return pool_ctx_fail(P, sc, rc);
   // join the batch as the last segment
   const int N0 = c->N;
-  rc = batch_reserve(c, N0 + N, P->max_particles);
+  bool lost = false;
+  rc = batch_reserve(c, N0 + N, P->max_particles, &lost);
+  if (lost) {
+    P->failed = true;
+    return pool_ctx_fail(P, c, rc);
+  }
   if (rc == GDF_OK) {
     for (int f = 0; f < F_NPERSIST && rc == GDF_OK; f++)
       if (hipMemcpyAsync(c->pf[f] + N0, sc->pf[f], sizeof(double)*N, hipMemcpyDeviceToDevice,
@@ -241,6 +296,7 @@ int gdf_pool_admit(gdf_pool* P, int N, const double* r, const double* v, const d
 
 int gdf_pool_step(gdf_pool* P, int64_t* out2) {
   if (!P) return GDF_ERR_INVALID;
+  if (P->failed) return GDF_ERR_INVALID;
   gdf_ctx* c = P->batch;
   if (out2) { out2[0] = 0; out2[1] = 0; }
   const int ns = (int) P->res.size();
@@ -264,7 +320,11 @@ int gdf_pool_step(gdf_pool* P, int64_t* out2) {
   if (rc == GDF_OK) rc = upload_seg_dt(c, kick);
   if (rc == GDF_OK) rc = integ_seg_end_step(c);
   if (rc == GDF_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = GDF_ERR_HIP;
-  if (rc != GDF_OK) return pool_ctx_fail(P, c, rc);
+  if (rc != GDF_OK) {
+    // the requests stopped part way through their step: their state is not a step boundary
+    P->failed = true;
+    return pool_ctx_fail(P, c, rc);
+  }
   int64_t psteps = 0;
   std::vector<int> keep;
   for (int sg = 0; sg < ns; sg++) {

@@ -111,9 +111,22 @@ int GradhSphSimulationGpu<ndim>::ComputeForces() {
   return gdf::fail(ctx, GDF_ERR_INVALID, "Error: No forces included in simulation");
 }
 
+// The setup starts from the time and step count the particles came with: zero after
+// gdf_set_particles, the file's after a snapshot read, which the reference keeps through its
+// setup (SimulationIO.hpp:1363-1382); the file's particle ids survive only a restart
+// (SphSimulation.cpp:215-217)
+static void initial_state(gdf_ctx* c, double* t0, int64_t* nsteps0) {
+  *t0 = c->t;
+  *nsteps0 = c->nsteps;
+  if (!c->cfg.restart) c->snap_porig.clear();
+}
+
 template <int ndim>
 int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
   gdf_ctx* c = ctx;
+  double t0;
+  int64_t nsteps0;
+  initial_state(c, &t0, &nsteps0);
   // initial smoothing lengths (Sph::InitialSmoothingLengthGuess), then the three h-rho passes
   // of the reference setup: before periodic ghosts exist, after them, and in the force loop
   std::vector<double> r(3*(size_t) c->N);
@@ -121,7 +134,7 @@ int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
   const double hg = initial_h_guess(r, c->phys.kern.range, c->phys.h_fac);
   std::vector<double> h((size_t) c->N, hg);
   GDF_TRY(gdf_set_field(c, "h", h.data()));
-  c->t = 0.0;
+  c->t = t0;
   // every particle active, level 0, tlast = t (SphSimulation.cpp:225-330)
   if (c->nlevels > 1) GDF_TRY(gdf::block_init(c));
   GDF_TRY(sphneib.BuildTree(true, 0, 1, 1, 0.0));
@@ -131,8 +144,8 @@ int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
   GDF_TRY(sphneib.BuildTree(true, 0, 1, 1, 0.0));
   GDF_TRY(sphneib.UpdateAllSphProperties(false));
   GDF_TRY(ComputeForces());
-  c->t = 0.0;
-  c->tlast = 0.0;
+  c->t = t0;
+  c->tlast = t0;
   if (c->nlevels > 1) {
     // the first ComputeBlockTimesteps resynchronises (n = nresync = 0); EndTimestep with the
     // particles' initial dt = 0 sets r0, v0, a0, u0, dudt0 and each particle's first step
@@ -144,7 +157,7 @@ int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
     GDF_TRY(hydroint.EndTimestep(0, c->t, 0.0));
   }
   GDF_HIP(c, hipStreamSynchronize(c->stream));
-  c->nsteps = 0;
+  c->nsteps = nsteps0;
   c->setup_done = true;
   return GDF_OK;
 }
@@ -275,13 +288,16 @@ int NbodySimulationGpu<ndim>::ComputeForces() {
 template <int ndim>
 int NbodySimulationGpu<ndim>::PostInitialConditionsSetup() {
   gdf_ctx* c = ctx;
-  c->t = 0.0;
-  c->tlast = 0.0;
+  double t0;
+  int64_t nsteps0;
+  initial_state(c, &t0, &nsteps0);
+  c->t = t0;
+  c->tlast = t0;
   GDF_TRY(ComputeForces());
   GDF_TRY(gdf::integ_timestep(c, &c->timestep));
   GDF_TRY(nbodyint.EndTimestep(0, c->t, 0.0));
   GDF_HIP(c, hipStreamSynchronize(c->stream));
-  c->nsteps = 0;
+  c->nsteps = nsteps0;
   c->setup_done = true;
   return GDF_OK;
 }

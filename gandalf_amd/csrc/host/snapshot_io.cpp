@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <iomanip>
 #include <string>
 #include <vector>
@@ -240,6 +241,24 @@ int gdf_write_snapshot(gdf_ctx* c, const char* path, const char* format) {
   return rc == GDF_OK ? rc : gdf::fail(c, rc, std::string("cannot write ") + path);
 }
 
+// What a Seren read keeps (SimulationIO.hpp:1363-1394): the time and step count always; the
+// snapshot counters only on a restart (otherwise the constructor's zeros); the file's particle
+// ids until the setup, which resets them unless restarting (SphSimulation.cpp:215-217)
+extern "C++" template <typename I>
+static void keep_header(gdf_ctx* c, const I* ilpdata, const double* ddata,
+                        std::vector<int>&& porig) {
+  c->t = ddata[0];
+  c->nsteps = ilpdata[1];
+  if (c->cfg.restart) {
+    c->snap_nout = ilpdata[0]; c->snap_nlite = ilpdata[10];
+    c->snap_tlast = ddata[1]; c->snap_tlitelast = ddata[10];
+  } else {
+    c->snap_nout = 0; c->snap_nlite = 0;
+    c->snap_tlast = 0.0; c->snap_tlitelast = 0.0;
+  }
+  c->snap_porig = std::move(porig);
+}
+
 // ReadSerenUnformSnapshotFile (SimulationIO.hpp:1294-1540) into a device context: the header's
 // step count and time are taken as the reference takes them, and its snapshot counters are kept
 // for the next write; "temp" is skipped, other per-particle arrays are refused as there.
@@ -261,8 +280,8 @@ static int read_su(gdf_ctx* c, const char* path) {
   for (int k = 0; k < 50; k++) ok = ok && su_get(f, rdata[k]);
   for (int k = 0; k < 50; k++) ok = ok && su_get(f, ddata[k]);
   const int nh = idata[0], ns = idata[1], nunit = idata[19], ndata = idata[20];
-  if (!ok || nh < 0 || ns < 0 || nh + ns <= 0 || nunit < 0 || nunit > 50 || ndata < 0 ||
-      ndata > 50)
+  if (!ok || nh < 0 || ns < 0 || (int64_t) nh + ns <= 0 || (int64_t) nh + ns > (1LL << 30) ||
+      nunit < 0 || nunit > 50 || ndata < 0 || ndata > 50)
     return gdf::fail(c, GDF_ERR_INVALID, "bad Seren snapshot header");
   if (c->phys.nbody ? nh != 0 : ns != 0)
     return gdf::fail(c, GDF_ERR_INVALID, c->phys.nbody ? "N-body context: hydro particles in snapshot"
@@ -309,11 +328,7 @@ static int read_su(gdf_ctx* c, const char* path) {
   if (rc != GDF_OK) return rc;
   if (!c->phys.nbody) rc = gdf_set_field(c, "rho", rho.data());
   if (rc != GDF_OK) return rc;
-  c->t = ddata[0];
-  c->nsteps = ilpdata[1];
-  c->snap_nout = ilpdata[0]; c->snap_nlite = ilpdata[10];
-  c->snap_tlast = ddata[1]; c->snap_tlitelast = ddata[10];
-  c->snap_porig = std::move(porig);
+  keep_header(c, ilpdata, ddata, std::move(porig));
   (void) rdata;
   return GDF_OK;
 }
@@ -337,7 +352,7 @@ static int read_sf(gdf_ctx* c, const char* path) {
   for (double& x : rdata) f >> x;
   for (double& x : ddata) f >> x;
   const long long nh = idata[0], ns = idata[1], nunit = idata[19], ndata = idata[20];
-  if (!f || nh < 0 || ns < 0 || nh + ns <= 0 || nh + ns > (1LL << 30) || nunit < 0 ||
+  if (!f || nh < 0 || ns < 0 || (int64_t) nh + ns <= 0 || (int64_t) nh + ns > (1LL << 30) || nunit < 0 ||
       nunit > 50 || ndata < 0 || ndata > 50)
     return gdf::fail(c, GDF_ERR_INVALID, "bad Seren snapshot header");
   if (c->phys.nbody ? nh != 0 : ns != 0)
@@ -386,27 +401,36 @@ static int read_sf(gdf_ctx* c, const char* path) {
   if (rc != GDF_OK) return rc;
   if (!c->phys.nbody) rc = gdf_set_field(c, "rho", rho.data());
   if (rc != GDF_OK) return rc;
-  c->t = ddata[0];
-  c->nsteps = ilpdata[1];
-  c->snap_nout = ilpdata[0]; c->snap_nlite = ilpdata[10];
-  c->snap_tlast = ddata[1]; c->snap_tlitelast = ddata[10];
-  c->snap_porig = std::move(porig);
+  keep_header(c, ilpdata, ddata, std::move(porig));
   (void) w; (void) rdata;
   return GDF_OK;
 }
 
+static int read_column(gdf_ctx* c, const char* path);
+
 int gdf_read_snapshot(gdf_ctx* c, const char* path, const char* format) {
   if (!c || !path || !format) return GDF_ERR_INVALID;
-  if (std::string(format) == "su") return read_su(c, path);
-  if (std::string(format) == "sf") return read_sf(c, path);
-  if (std::string(format) != "column")
-    return gdf::fail(c, GDF_ERR_INVALID, std::string("unsupported snapshot format ") + format);
+  try {
+    if (std::string(format) == "su") return read_su(c, path);
+    if (std::string(format) == "sf") return read_sf(c, path);
+    if (std::string(format) == "column") return read_column(c, path);
+  } catch (const std::bad_alloc&) {
+    return gdf::fail(c, GDF_ERR_NOMEM, "snapshot read: host allocation failed");
+  }
+  return gdf::fail(c, GDF_ERR_INVALID, std::string("unsupported snapshot format ") + format);
+}
+
+}  // extern "C"
+
+// ReadColumnSnapshotFile (SimulationIO.hpp:193-270)
+static int read_column(gdf_ctx* c, const char* path) {
   std::ifstream f(path);
   if (!f) return gdf::fail(c, GDF_ERR_INVALID, std::string("cannot read ") + path);
   int nh = 0, ns = 0, ndim = 0;
   double t = 0.0;
   f >> nh >> ns >> ndim >> t;
-  if (!f || ndim != 3 || nh < 0 || ns < 0 || nh + ns <= 0)
+  if (!f || ndim != 3 || nh < 0 || ns < 0 || (int64_t) nh + ns <= 0 ||
+      (int64_t) nh + ns > (1LL << 30))
     return gdf::fail(c, GDF_ERR_INVALID, "bad column snapshot header");
   if (c->phys.nbody ? nh != 0 : ns != 0)
     return gdf::fail(c, GDF_ERR_INVALID, c->phys.nbody ? "N-body context: hydro particles in snapshot"
@@ -421,8 +445,8 @@ int gdf_read_snapshot(gdf_ctx* c, const char* path, const char* format) {
   int rc = gdf_set_particles(c, (int) n, r.data(), v.data(), m.data(), h.data(), u.data());
   if (rc != GDF_OK) return rc;
   if (!c->phys.nbody) rc = gdf_set_field(c, "rho", rho.data());
-  c->t = t;
+  c->t = t;                         // ReadColumnSnapshotFile: t and tsnaplast from the header
+  if (!c->cfg.restart) { c->snap_nout = 0; c->snap_nlite = 0; c->snap_tlitelast = 0.0; }
+  c->snap_tlast = t;
   return rc;
 }
-
-}  // extern "C"

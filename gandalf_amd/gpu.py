@@ -42,6 +42,7 @@ class Config(C.Structure):
         ("nbody_mult", C.c_double), ("fixed_dt", C.c_double),
         ("Nlevels", C.c_int), ("level_diff_max", C.c_int),
         ("hrho_iteration_max", C.c_int), ("cand_batches_init", C.c_int),
+        ("restart", C.c_int),
     ]
 
 
@@ -152,6 +153,7 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
     c.level_diff_max = int(prm.get("level_diff_max", 1))
     c.hrho_iteration_max = int(prm.get("hrho_iteration_max", 0))
     c.cand_batches_init = int(prm.get("cand_batches_init", 0))
+    c.restart = int(prm.get("restart", 0))
     return c
 
 

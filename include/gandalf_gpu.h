@@ -66,6 +66,11 @@ typedef struct gdf_config {
    * before the first overflow regrows them (16) */
   int hrho_iteration_max;
   int cand_batches_init;
+  /* restart (the reference's "gandalf -r", SimulationBase::restart): a snapshot read keeps its
+   * snapshot counters (Noutsnap, Noutlitesnap, tsnaplast, tlitesnaplast) and the setup keeps
+   * the file's particle ids; otherwise both are reset as for "ic = file"
+   * (SimulationIO.hpp:1385-1394, SphSimulation.cpp:215-217) */
+  int restart;
 } gdf_config;
 
 typedef struct gdf_ctx gdf_ctx;

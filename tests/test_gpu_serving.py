@@ -106,3 +106,34 @@ def test_serving_loop_small_stream():
     assert all(q.finished > 0 and len(q.step_latency) == 3 for q in reqs)
     assert out["particle_steps"] == sum(3 * q.n for q in reqs)
     assert out["step_latency_ms"]["p99"] >= out["step_latency_ms"]["p50"] > 0
+
+
+def test_failed_growth_leaves_resident_requests_untouched(monkeypatch):
+    """Admitting a request grows the batch's arrays; if that allocation fails part way
+    (forced from the host: GDF_FAULT_POOL_GROW makes the grow report an allocation failure
+    after allocating), the admission fails with GDF_ERR_NOMEM, the resident request keeps its
+    state, and the pool steps on to the same result as a pool that never saw the failure."""
+    from gandalf_amd import serving as S
+    from gandalf_amd.gpu import GdfError, GpuPool
+    a, b = ics("sphere", (1500, 2000), 80)
+    params = S.class_params("sphere")
+
+    def run(fail_admission):
+        pool = GpuPool(params, "mixed", max_particles=1 << 16)
+        ia = pool.admit(a["r"], a["v"], a["m"], a["u"], 3)
+        pool.step()
+        if fail_admission:
+            monkeypatch.setenv("GDF_FAULT_POOL_GROW", "1")
+            with pytest.raises(GdfError, match="status -2: .*injected allocation failure.*"
+                                               "resident requests are unchanged"):
+                pool.admit(b["r"], b["v"], b["m"], b["u"], 3)
+            monkeypatch.delenv("GDF_FAULT_POOL_GROW")
+            assert pool.resident_particles == 1500
+        pool.step()
+        pool.step()
+        assert pool.status(ia)[:2] == (2, 3)
+        return {f: pool.get(ia, f) for f in FIELDS}
+
+    clean, failed = run(False), run(True)
+    for f in FIELDS:
+        assert np.array_equal(clean[f], failed[f]), f

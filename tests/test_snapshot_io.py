@@ -217,3 +217,94 @@ def test_sf_snapshot_roundtrip_through_device(tmp_path):
     b = tmp_path / "stars2.sf"
     t.write_snapshot(b, "sf")
     assert b.read_bytes() == a.read_bytes()
+
+
+def _format_su(path, d, n, counts, times, porig):
+    lib = C.CDLL(LIB)
+    f = lib.gdf_format_su
+    f.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7
+    f.restype = C.c_int
+    N = d["in_m"].shape[0]
+    cnt = np.array(counts, dtype=np.int64)
+    tim = np.array(times, dtype=np.float64)
+    po = None if porig is None else np.ascontiguousarray(porig, dtype=np.int32)
+    rc = f(str(path).encode(), N, 0, cnt.ctypes.data_as(C.c_void_p), _p(tim),
+           float(d["meta"]["params"]["h_fac"]),
+           None if po is None else po.ctypes.data_as(C.c_void_p),
+           _p(vec(d, f"s{n}")), _p(vec(d, f"s{n}", ("vx", "vy", "vz"))), _p(d["in_m"]),
+           _p(d[f"s{n}_h"]), _p(d[f"s{n}_rho"]), _p(d[f"s{n}_u"]))
+    assert rc == 0
+
+
+@pytest.mark.gpu
+def test_su_read_then_setup_keeps_time_and_steps(tmp_path):
+    """A run continued from a snapshot (ic = file) keeps the file's time and step count through
+    the setup (SimulationIO.hpp:1363-1382; PostInitialConditionsSetup does not reset them):
+    after the setup t and Nsteps are the file's, and one step later t + dt and Nsteps + 1."""
+    from gandalf_amd.gpu import GpuSim
+    d = load_golden("box16_m4")
+    n = d["meta"]["nsteps"]
+    src = tmp_path / "ref.su"
+    src.write_bytes(bytes(d[f"s{n}_su"]))
+    _, ilp, _, ddata = _su_header(src.read_bytes())
+    s = GpuSim(d["meta"]["params"])
+    s.read_snapshot(src, "su")
+    s.setup()
+    t, dt, steps = s.time()
+    assert t == ddata[0] and steps == ilp[1] == n
+    s.step()
+    t1, _, steps1 = s.time()
+    assert t1 == t + dt and steps1 == n + 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("restart", [0, 1])
+def test_su_restart_keeps_counters_and_ids(tmp_path, restart):
+    """With restart (gandalf -r) a read keeps the snapshot counters (Noutsnap, Noutlitesnap,
+    tsnaplast, tlitesnaplast) and the setup keeps the file's particle ids; without it the
+    counters start from zero (SimulationIO.hpp:1385-1394) and the setup renumbers the
+    particles 0..N-1 (SphSimulation.cpp:215-217).  Written back, each shows in the header and
+    the porig block."""
+    from gandalf_amd.gpu import GpuSim
+    d = load_golden("box16_m4")
+    n = d["meta"]["nsteps"]
+    N = d["in_m"].shape[0]
+    t = float(d["diag"][n, 1])
+    porig = np.arange(N)[::-1].copy()
+    src = tmp_path / "in.su"
+    _format_su(src, d, n, (3, n, 2), (t, 0.5 * t, 1.0 / N, 0.25 * t), porig)
+    s = GpuSim(dict(d["meta"]["params"], restart=restart))
+    s.read_snapshot(src, "su")
+    s.setup()
+    out = tmp_path / "out.su"
+    s.write_snapshot(out, "su")
+    b = out.read_bytes()
+    _, ilp, _, ddata = _su_header(b)
+    # header (1436 bytes), 7 array names of 20 bytes, 7 x 5 type words, then porig first
+    assert b[1436:1441] == b"porig"
+    ids = np.frombuffer(b[1436 + 7 * 20 + 7 * 5 * 4:][:4 * N], np.int32)
+    if restart:
+        assert (ilp[0], ilp[1], ilp[10]) == (3, n, 2)
+        assert ddata[1] == 0.5 * t and ddata[10] == 0.25 * t
+        assert np.array_equal(ids, porig)
+    else:
+        assert (ilp[0], ilp[1], ilp[10]) == (0, n, 0)
+        assert ddata[1] == 0.0 and ddata[10] == 0.0
+        assert np.array_equal(ids, np.arange(N))
+
+
+@pytest.mark.gpu
+def test_su_header_counts_are_bounded(tmp_path):
+    """A header claiming more than 2^30 particles (or an int overflow of Nhydro + Nstar) is
+    refused before anything is allocated."""
+    from gandalf_amd.gpu import GdfError, GpuSim
+    d = load_golden("box16_m4")
+    n = d["meta"]["nsteps"]
+    b = bytearray(bytes(d[f"s{n}_su"]))
+    for nh, ns in ((1 << 30, 1), (2 ** 31 - 1, 2 ** 31 - 1)):
+        b[_SU_I:_SU_I + 8] = np.array([nh, ns], np.int32).tobytes()
+        bad = tmp_path / "bad.su"
+        bad.write_bytes(bytes(b))
+        s = GpuSim(d["meta"]["params"])
+        with pytest.raises(GdfError, match="bad Seren snapshot header"):
+            s.read_snapshot(bad, "su")
