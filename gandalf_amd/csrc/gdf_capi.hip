@@ -53,9 +53,9 @@ __global__ void k_iota(int N, int* p) {
 
 // integer per-particle state exported as doubles
 int int_field(const char* name) {
-  static const char* names[] = {"level", "levelneib", "nstep", "nlast"};
-  static const int ids[] = {I_LEVEL, I_LEVELNEIB, I_NSTEP, I_NLAST};
-  for (int k = 0; k < 4; k++) if (!strcmp(name, names[k])) return ids[k];
+  static const char* names[] = {"level", "levelneib", "nstep", "nlast", "active"};
+  static const int ids[] = {I_LEVEL, I_LEVELNEIB, I_NSTEP, I_NLAST, I_ACT};
+  for (int k = 0; k < 5; k++) if (!strcmp(name, names[k])) return ids[k];
   return -1;
 }
 
@@ -63,6 +63,12 @@ __global__ void k_scatter_out_int(int N, const int* __restrict__ src, const int*
                                   double* __restrict__ out) {
   const int i = blockIdx.x*BS + threadIdx.x;
   if (i < N) out[io[i]] = (double) src[i];
+}
+
+__global__ void k_gather_in_int(int N, int* __restrict__ dst, const int* __restrict__ io,
+                                const double* __restrict__ in) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i < N) dst[i] = (int) in[io[i]];
 }
 
 // resolve a field name to up to three device arrays
@@ -459,6 +465,15 @@ int gdf_set_field(gdf_ctx* c, const char* name, const double* in) {
   if (c->N <= 0) return fail(c, GDF_ERR_INVALID, "no particles");
   double* p[3] = {nullptr, nullptr, nullptr};
   int w = 0;
+  const int fi = int_field(name);
+  if (fi >= 0) {
+    GDF_HIP(c, hipMemcpyAsync(c->tmpd, in, sizeof(double)*(size_t) c->N, hipMemcpyHostToDevice,
+                              c->stream));
+    k_gather_in_int<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, c->pi[fi], c->iorig, c->tmpd);
+    GDF_HIP(c, hipGetLastError());
+    GDF_HIP(c, hipStreamSynchronize(c->stream));
+    return GDF_OK;
+  }
   if (!field_ptrs(c, name, p, &w)) return fail(c, GDF_ERR_INVALID, std::string("unknown field ") + name);
   GDF_HIP(c, hipMemcpyAsync(c->tmpd, in, sizeof(double)*w*(size_t) c->N, hipMemcpyHostToDevice, c->stream));
   k_gather_in<<<nblocks(c->N, BS), BS, 0, c->stream>>>(c->N, w, p[0], p[1], p[2], c->iorig, c->tmpd);
@@ -483,7 +498,7 @@ int gdf_update_all_sph_properties(gdf_ctx* c, int nopbc) {
 int gdf_update_all_sph_hydro_forces(gdf_ctx* c) {
   if (!c || c->N <= 0) return GDF_ERR_INVALID;
   if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "hydro forces before the density pass");
-  GDF_TRY(integ_zero_forces(c));
+  GDF_TRY(integ_zero_accelerations(c));
   GDF_TRY(sph_build_sg(c));
   return sph_hydro(c, 0);
 }
@@ -491,7 +506,7 @@ int gdf_update_all_sph_hydro_forces(gdf_ctx* c) {
 int gdf_update_all_sph_forces(gdf_ctx* c) {
   if (!c || c->N <= 0) return GDF_ERR_INVALID;
   if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "forces before the density pass");
-  GDF_TRY(integ_zero_forces(c));
+  GDF_TRY(integ_zero_accelerations(c));
   if (c->phys.hydro_forces) {
     GDF_TRY(gravity_launch(c));        // beside the hydro pass, on the second stream
     GDF_TRY(sph_build_sg(c));

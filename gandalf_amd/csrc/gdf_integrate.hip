@@ -236,6 +236,11 @@ __global__ void k_zero_d(int n, double* __restrict__ p) {
   if (i < n) p[i] = 0.0;
 }
 
+__global__ void k_fill_i(int n, int* __restrict__ p, int v) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 __global__ void k_add_atree(int N, double* __restrict__ ax, double* __restrict__ ay,
                             double* __restrict__ az, const double* __restrict__ tx,
                             const double* __restrict__ ty, const double* __restrict__ tz,
@@ -407,6 +412,19 @@ int integ_diagnostics(gdf_ctx* c, double* out) {
   for (int k = 0; k < 3; k++) { out[4 + k] = acc[4 + k]; out[7 + k] = acc[7 + k]; }
   out[10] = acc[0];
   return GDF_OK;
+}
+
+void fill_int(gdf_ctx* c, int* p, int n, int v) {
+  if (n > 0) k_fill_i<<<nblocks(n, BS), BS, 0, c->stream>>>(n, p, v);
+}
+
+// Sph::ZeroAccelerations (Sph.cpp:126-140): every particle with global timesteps, the active
+// ones (and their neighbour levels) with block timesteps, whose thermal state is refreshed for
+// every particle as SphSimulation::MainLoop does (SphSimulation.cpp:667-676, nradstep = 1)
+int integ_zero_accelerations(gdf_ctx* c) {
+  if (c->nlevels == 1) return integ_zero_forces(c);
+  GDF_TRY(block_zero_active(c));
+  return block_eos_all(c);
 }
 
 int integ_zero_forces(gdf_ctx* c) {

@@ -264,6 +264,11 @@ int integ_timestep(gdf_ctx* c, double* dtmin);
 int integ_diagnostics(gdf_ctx* c, double* out);
 int integ_max_h(gdf_ctx* c, double* hmax);
 int integ_zero_forces(gdf_ctx* c);
+// Sph::ZeroAccelerations for the context's timestep mode (every particle, or the active ones
+// of a block step plus the thermal state of all)
+int integ_zero_accelerations(gdf_ctx* c);
+// p[0, n) = v on the context's stream
+void fill_int(gdf_ctx* c, int* p, int n, int v);
 int integ_add_atree(gdf_ctx* c);
 int integ_pack_posm(gdf_ctx* c);
 // serving batches: per-segment timestep minima into out[nseg]; drift/kick with seg_dt

@@ -40,14 +40,7 @@ int GradhSphTreeGpu<ndim>::UpdateAllSphProperties(bool nopbc) {
   return gdf::sph_density(ctx, nopbc ? 1 : 0);
 }
 
-// Sph::ZeroAccelerations (Sph.cpp:126-140): every particle with global timesteps, the active
-// ones (and their neighbour levels) with block timesteps; the thermal state of every particle
-// is refreshed as SphSimulation::MainLoop does (SphSimulation.cpp:667-676, nradstep = 1)
-static int zero_accelerations(gdf_ctx* c) {
-  if (c->nlevels == 1) return gdf::integ_zero_forces(c);
-  GDF_TRY(gdf::block_zero_active(c));
-  return gdf::block_eos_all(c);
-}
+static int zero_accelerations(gdf_ctx* c) { return gdf::integ_zero_accelerations(c); }
 
 template <int ndim>
 int GradhSphTreeGpu<ndim>::UpdateAllSphHydroForces() {

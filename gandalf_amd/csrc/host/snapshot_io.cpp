@@ -450,3 +450,20 @@ static int read_column(gdf_ctx* c, const char* path) {
   c->snap_tlast = t;
   return rc;
 }
+
+extern "C" {
+
+int gdf_snapshot_counters(gdf_ctx* c, const int64_t* set2, const double* set2d, int64_t* get2,
+                          double* get2d) {
+  if (!c) return GDF_ERR_INVALID;
+  if (get2) { get2[0] = c->snap_nout; get2[1] = c->snap_nlite; }
+  if (get2d) { get2d[0] = c->snap_tlast; get2d[1] = c->snap_tlitelast; }
+  if (set2) {
+    if (set2[0] < 0 || set2[1] < 0) return gdf::fail(c, GDF_ERR_INVALID, "negative snapshot count");
+    c->snap_nout = set2[0]; c->snap_nlite = set2[1];
+  }
+  if (set2d) { c->snap_tlast = set2d[0]; c->snap_tlitelast = set2d[1]; }
+  return GDF_OK;
+}
+
+}  // extern "C"

@@ -95,6 +95,14 @@ PROTOTYPES = {
     "gdf_set_timing": (_i, [_vp, _i]),
     "gdf_nbody_direct_forces": (_i, [_vp, _i]),
     "gdf_calculate_diagnostics_noalloc": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp]),
+    "gdf_advance_particles_block": (_i, [_vp, _i, _d, _d]),
+    "gdf_set_active_particles": (_i, [_vp, _i]),
+    "gdf_correction_terms": (_i, [_vp, _i, _d, _d]),
+    "gdf_check_timesteps": (_i, [_vp, _i, _i, _i, _d, C.POINTER(_i)]),
+    "gdf_compute_block_timesteps": (_i, [_vp]),
+    "gdf_end_timestep_block": (_i, [_vp, _i, _d, _d]),
+    "gdf_get_gather_neighbour_list": (_i, [_vp, _vp, _d, _i, _vp]),
+    "gdf_snapshot_counters": (_i, [_vp, _vp, _vp, _vp, _vp]),
 }
 
 
@@ -258,6 +266,18 @@ class GpuSim:
         self._chk(lib().gdf_read_snapshot(self._h, str(path).encode(), fmt.encode()))
         self.N = lib().gdf_num_particles(self._h)
 
+    def snapshot_counters(self, nout=None, nlite=None, tlast=None, tlitelast=None):
+        """(Noutsnap, Noutlitesnap, tsnaplast, tlitesnaplast) the snapshot writers use; values
+        given replace them (all four together, or none)."""
+        g = np.zeros(2, dtype=np.int64)
+        gd = np.zeros(2)
+        st = sd = None
+        if nout is not None:
+            st = np.array([nout, nlite], dtype=np.int64)
+            sd = np.array([tlast, tlitelast], dtype=np.float64)
+        self._chk(lib().gdf_snapshot_counters(self._h, _p(st), _p(sd), _p(g), _p(gd)))
+        return int(g[0]), int(g[1]), float(gd[0]), float(gd[1])
+
     def block_state(self):
         """(n, level_max, level_step, nresync, dt_max, particles that ended a step last)"""
         iv = np.zeros(5, dtype=np.int64)
@@ -311,6 +331,43 @@ class GpuSim:
         """Star-star direct sums (Nbody::CalculateDirectGravForces, or the kernel-smoothed
         NbodyLeapfrogKDK variant): fills a, adot and gpot."""
         self._chk(lib().gdf_nbody_direct_forces(self._h, int(bool(smoothed))))
+
+    # ---- per-method entry points (TimeIntegration / NeighbourSearch of the reference) ----
+    def advance_particles_block(self, n, t, timestep):
+        self._chk(lib().gdf_advance_particles_block(self._h, int(n), float(t), float(timestep)))
+
+    def set_active_particles(self, n):
+        self._chk(lib().gdf_set_active_particles(self._h, int(n)))
+
+    def correction_terms(self, n, t, timestep):
+        self._chk(lib().gdf_correction_terms(self._h, int(n), float(t), float(timestep)))
+
+    def check_timesteps(self, level_diff_max, level_step, n, timestep):
+        out = C.c_int(0)
+        self._chk(lib().gdf_check_timesteps(self._h, int(level_diff_max), int(level_step),
+                                            int(n), float(timestep), C.byref(out)))
+        return out.value
+
+    def compute_block_timesteps(self):
+        self._chk(lib().gdf_compute_block_timesteps(self._h))
+
+    def end_timestep_block(self, n, t, timestep):
+        self._chk(lib().gdf_end_timestep_block(self._h, int(n), float(t), float(timestep)))
+
+    def check_boundaries(self):
+        self._chk(lib().gdf_check_boundaries(self._h))
+
+    def gather_neighbour_list(self, rp, rsearch, nmax=None):
+        """Import-order ids within rsearch of rp, or None when more than nmax (the reference's
+        -1 overflow code)."""
+        nmax = self.N if nmax is None else int(nmax)
+        out = np.zeros(max(nmax, 1), dtype=np.int32)
+        n = lib().gdf_get_gather_neighbour_list(self._h, _p(_d(rp)), float(rsearch), nmax,
+                                                 _p(out))
+        if n == -1:
+            return None
+        self._chk(n)
+        return out[:n].copy()
 
     def gravity_group_counts(self):
         a = self.tree_arrays()

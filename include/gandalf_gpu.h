@@ -92,8 +92,9 @@ int gdf_set_particles(gdf_ctx* ctx, int N, const double* r, const double* v, con
  * SphSnapshot::CopyDataFromSimulation (src/Common/SphSnapshot.cpp:340-372).  Names:
  * r v a atree r0 v0 a0 m h rho u u0 dudt dudt0 div_v gpot invomega zeta sound pressure hfactor
  * dt (timestep criterion) adot; block-timestep state: tlast dtstep (the particle's step
- * length, Particle.h dt) level levelneib nstep nlast (integers returned as doubles) */
+ * length, Particle.h dt) level levelneib nstep nlast active (integers returned as doubles) */
 int gdf_get_field(gdf_ctx* ctx, const char* name, double* out);
+/* write a per-particle field in import order (the names above; integers passed as doubles) */
 int gdf_set_field(gdf_ctx* ctx, const char* name, const double* in);
 int gdf_num_particles(const gdf_ctx* ctx);
 
@@ -126,6 +127,40 @@ int gdf_compute_global_timestep(gdf_ctx* ctx, double* dt_min);
  * out = {Etot, ketot, utot, gpetot, mom[3], angmom[3], mtot} */
 int gdf_calculate_diagnostics(gdf_ctx* ctx, double* out);
 
+/* The per-method form of TimeIntegration<3> and of Simulation::ComputeBlockTimesteps, for a
+ * caller that runs SphSimulation::MainLoop's sequence itself, block timesteps included
+ * (SphSimulation.cpp:574-880).  n, t and timestep are the caller's integer time, time and base
+ * timestep, as in the reference's signatures; each call sets the context's block clock from
+ * them.  With one timestep level every particle is active at every integer time. */
+/* SphLeapfrogKDK::AdvanceParticles(n, t, timestep) (SphLeapfrogKDK.cpp:76-127): each particle
+ * drifts from the start of its own step (t - tlast) and is active iff its step ends at n */
+int gdf_advance_particles_block(gdf_ctx* ctx, int n, double t, double timestep);
+/* SphLeapfrogKDK::SetActiveParticles (SphLeapfrogKDK.cpp:192-211): active iff the particle's
+ * end-of-step flag is set or n - nlast == nstep */
+int gdf_set_active_particles(gdf_ctx* ctx, int n);
+/* SphLeapfrogKDK::CorrectionTerms (SphLeapfrogKDK.cpp:138-183): particles whose step ends at n
+ * take v += dt (a - a0)/2 (and the internal-energy correction with its first-order fallback) */
+int gdf_correction_terms(gdf_ctx* ctx, int n, double t, double timestep);
+/* SphLeapfrogKDK::CheckTimesteps (SphLeapfrogKDK.cpp:284-329): particles whose neighbours'
+ * level exceeds theirs by more than level_diff_max end their step early where it is
+ * synchronised; *activecount = their number (0 with one level) */
+int gdf_check_timesteps(gdf_ctx* ctx, int level_diff_max, int level_step, int n,
+                        double timestep, int* activecount);
+/* Simulation::ComputeBlockTimesteps (src/Common/Simulation.cpp:1764-2197): new levels,
+ * level_max / level_step / nresync / dt_max and the base timestep (read them back with
+ * gdf_block_state and gdf_time: the integer time may be rescaled) */
+int gdf_compute_block_timesteps(gdf_ctx* ctx);
+/* SphLeapfrogKDK::EndTimestep(n, t, timestep) (SphLeapfrogKDK.cpp:219-272): particles whose
+ * step ended close it (kick, r0/v0/a0, nlast = n, tlast = t) */
+int gdf_end_timestep_block(gdf_ctx* ctx, int n, double t, double timestep);
+/* NeighbourSearch::GetGatherNeighbourList (src/Headers/NeighbourSearch.h:83) ->
+ * HydroTree::GetGatherNeighbourList (src/Tree/HydroTree.cpp:451-471, Tree.cpp:208-280): the
+ * import-order ids, ascending, of the particles with |r - rp|^2 < rsearch^2, periodic images
+ * included (the reference finds those through its ghost tree, as copies of these particles).
+ * Returns their number, or -1 when it exceeds nmax (the reference's overflow code). */
+int gdf_get_gather_neighbour_list(gdf_ctx* ctx, const double* rp, double rsearch, int nmax,
+                                  int* list);
+
 /* ---- whole-step drivers: SphSimulation<3> (src/Hydrodynamics/SphSimulation.cpp) ---- */
 /* PostInitialConditionsSetup (:204-565) for initial conditions without smoothing lengths */
 int gdf_setup_simulation(gdf_ctx* ctx);
@@ -144,6 +179,13 @@ int gdf_block_state(const gdf_ctx* ctx, int64_t* iout5, double* dout1);
  * writes the star block. */
 int gdf_write_snapshot(gdf_ctx* ctx, const char* path, const char* format);
 int gdf_read_snapshot(gdf_ctx* ctx, const char* path, const char* format);
+/* the snapshot counters the Seren writers put in their headers and a restart read restores
+ * (Simulation::Noutsnap, Noutlitesnap, tsnaplast, tlitesnaplast; SimulationIO.hpp:1385-1394,
+ * 2145-2160): get2 = {Noutsnap, Noutlitesnap}, get2d = {tsnaplast, tlitesnaplast} receive the
+ * context's values, then set2 / set2d (if not NULL) replace them.  The caller's output loop
+ * (Simulation::Output) owns them between writes. */
+int gdf_snapshot_counters(gdf_ctx* ctx, const int64_t* set2, const double* set2d,
+                          int64_t* get2, double* get2d);
 /* the column writer on host arrays (no device call): nhydro gas lines then nstar star lines */
 int gdf_format_column(const char* path, int nhydro, int nstar, double t, const double* r,
                       const double* v, const double* m, const double* h, const double* rho,
