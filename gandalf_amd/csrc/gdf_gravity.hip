@@ -137,10 +137,15 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
   __shared__ int pbuf[PCAP];
   __shared__ int cbuf[CCAP];
   __shared__ int sidx[kWave];
-  // staged batch: particles (x, y, z, m, r2, h, zeta, hfactor) or cells (com x3, m, q x5)
-  __shared__ double stage[(MIXED ? 5 : 9)*kWave];   // the FP32 path stages 9 floats per lane
+  // staged batch: particles (x, y, z, m, r2, h, zeta, hfactor) or cells (com x3, m, q x5).
+  // The FP32 path stages per lane two float4 records and one float, each read back with one
+  // broadcast LDS load: particles (dx, dy, dz, m), (h, zeta, hfactor, -), r2; cells
+  // (dx, dy, dz, m), (q0, q1, q2, q3), q4
+  __shared__ __align__(16) double stage[(MIXED ? 5 : 9)*kWave];
   double (*sd)[kWave] = reinterpret_cast<double (*)[kWave]>(stage);
-  float (*sf)[kWave] = reinterpret_cast<float (*)[kWave]>(stage);
+  float4* s4a = reinterpret_cast<float4*>(stage);
+  float4* s4b = s4a + kWave;
+  float* s1 = reinterpret_cast<float*>(s4b + kWave);
   __shared__ int lstk[kLdsStack];    // node stack; deeper entries go to the global region
   const int lane = threadIdx.x;
   int* stk = a.stacks + (size_t) blockIdx.x*kStackCap;
@@ -254,9 +259,10 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
             cx = gc.x + dx; cy = gc.y + dy; cz = gc.z + dz;
           }
           if (MIXED) {
-            sf[0][lane] = (float) (cx - gc.x); sf[1][lane] = (float) (cy - gc.y);
-            sf[2][lane] = (float) (cz - gc.z);
-            for (int k = 3; k < 9; k++) sf[k][lane] = (float) m[k];
+            s4a[lane] = make_float4((float) (cx - gc.x), (float) (cy - gc.y),
+                                    (float) (cz - gc.z), (float) m[3]);
+            s4b[lane] = make_float4((float) m[4], (float) m[5], (float) m[6], (float) m[7]);
+            s1[lane] = (float) m[8];
           } else {
             sd[0][lane] = cx; sd[1][lane] = cy; sd[2][lane] = cz;
             for (int k = 3; k < 9; k++) sd[k][lane] = m[k];
@@ -268,9 +274,10 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
             float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
             #pragma unroll 4
             for (int q = 0; q < nb; q++) {
-              const float qq[5] = {sf[4][q], sf[5][q], sf[6][q], sf[7][q], sf[8][q]};
-              quad_term<float>(rxi - sf[0][q], ryi - sf[1][q], rzi - sf[2][q], sf[3][q], qq, bx,
-                               by, bz, bp, PLUMMER ? eps2f : 1.0e-20f);
+              const float4 c = s4a[q], qa = s4b[q];
+              const float qq[5] = {qa.x, qa.y, qa.z, qa.w, s1[q]};
+              quad_term<float>(rxi - c.x, ryi - c.y, rzi - c.z, c.w, qq, bx, by, bz, bp,
+                               PLUMMER ? eps2f : 1.0e-20f);
             }
             fx.add(bx); fy.add(by); fz.add(bz); fp.add(bp);
           } else {
@@ -294,11 +301,9 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
           if (MIXED) {
             double dx = pj.x - gc.x, dy = pj.y - gc.y, dz = pj.z - gc.z;
             nearest_image<double>(a.p, dx, dy, dz);
-            sf[0][lane] = (float) dx; sf[1][lane] = (float) dy; sf[2][lane] = (float) dz;
-            sf[3][lane] = (float) pj.w; sf[4][lane] = (float) range2_rn(krsqd, rj.x);
-            if (SOFT) {
-              sf[5][lane] = (float) rj.x; sf[6][lane] = (float) rj.y; sf[7][lane] = (float) rj.z;
-            }
+            s4a[lane] = make_float4((float) dx, (float) dy, (float) dz, (float) pj.w);
+            s1[lane] = (float) range2_rn(krsqd, rj.x);
+            if (SOFT) s4b[lane] = make_float4((float) rj.x, (float) rj.y, (float) rj.z, 0.f);
           } else {
             sd[0][lane] = pj.x; sd[1][lane] = pj.y; sd[2][lane] = pj.z; sd[3][lane] = pj.w;
             sd[4][lane] = range2_rn(krsqd, rj.x);
@@ -313,20 +318,24 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
             // unrolled so that several LDS reads are in flight (same summation order)
             #pragma unroll 8
             for (int q = 0; q < nb; q++) {
-              const float dx = sf[0][q] - rxi, dy = sf[1][q] - ryi, dz = sf[2][q] - rzi;
+              // every record is loaded whatever the pair's branch: no LDS read under a mask
+              const float4 s = s4a[q];
+              const float r2j = s1[q];
+              const float dx = s.x - rxi, dy = s.y - ryi, dz = s.z - rzi;
               const float d2 = dot3_f(dx, dy, dz);
-              const bool soft = (d2 < r2if) | (d2 < sf[4][q]);
+              const bool soft = (d2 < r2if) | (d2 < r2j);
               if (PLUMMER) {
                 const float inv = rsq_f(d2 + eps2f);
-                const float mj = sidx[q] == i ? 0.f : sf[3][q];
+                const float mj = sidx[q] == i ? 0.f : s.w;
                 const float minv3 = mj*inv*inv*inv;
                 bx += minv3*dx; by += minv3*dy; bz += minv3*dz; bp += mj*inv;
               } else if (SOFT) {
-                pair_fp32(K, d2, soft, dx, dy, dz, sf[3][q], hi, sf[5][q], zi, sf[6][q], hfi,
-                          sf[7][q], bx, by, bz, bp);
+                const float4 hq = s4b[q];
+                pair_fp32(K, d2, soft, dx, dy, dz, s.w, hi, hq.x, zi, hq.y, hfi, hq.z, bx, by,
+                          bz, bp);
               } else {
                 const float inv = rsq_f(d2 + 1.0e-20f);
-                const float mj = soft ? 0.f : sf[3][q];
+                const float mj = soft ? 0.f : s.w;
                 const float minv3 = mj*inv*inv*inv;
                 bx += minv3*dx; by += minv3*dy; bz += minv3*dz; bp += mj*inv;
               }
