@@ -290,6 +290,7 @@ int gdf_config_default(gdf_config* cfg) {
   cfg->nbody = 0; cfg->nbody_direct = 0; cfg->nbody_eps = 0.0;
   cfg->nbody_mult = 0.1; cfg->fixed_dt = 0.0;
   cfg->Nlevels = 1; cfg->level_diff_max = 1;
+  cfg->nbody_softening = 1; cfg->conservative_sph_star_gravity = 1;
   return GDF_OK;
 }
 
@@ -309,6 +310,9 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
       cfg->cand_batches_init < 0 || cfg->cand_batches_init > 4096 || !(cfg->h_converge >= 0.0))
     return GDF_ERR_INVALID;
   if (cfg->Nlevels > 1 && (cfg->nbody || cfg->fixed_dt > 0.0)) return GDF_ERR_INVALID;
+  if ((cfg->nbody_softening != 0 && cfg->nbody_softening != 1) ||
+      (cfg->conservative_sph_star_gravity != 0 && cfg->conservative_sph_star_gravity != 1))
+    return GDF_ERR_INVALID;
   if (cfg->periodic)
     for (int k = 0; k < 3; k++)
       if (!(cfg->boxmax[k] > cfg->boxmin[k]) || cfg->boxmax[k] - cfg->boxmin[k] > 1e20)
@@ -375,6 +379,7 @@ void gdf_destroy(gdf_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   free_particles(c);
+  stars_free(c);
   if (c->counters) hipFree(c->counters);
   if (c->icount) hipFree(c->icount);
   if (c->red) hipFree(c->red);
@@ -500,7 +505,8 @@ int gdf_update_all_sph_hydro_forces(gdf_ctx* c) {
   if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "hydro forces before the density pass");
   GDF_TRY(integ_zero_accelerations(c));
   GDF_TRY(sph_build_sg(c));
-  return sph_hydro(c, 0);
+  GDF_TRY(sph_hydro(c, 0));
+  return stars_on_gas(c);
 }
 
 int gdf_update_all_sph_forces(gdf_ctx* c) {
@@ -511,10 +517,12 @@ int gdf_update_all_sph_forces(gdf_ctx* c) {
     GDF_TRY(gravity_launch(c));        // beside the hydro pass, on the second stream
     GDF_TRY(sph_build_sg(c));
     GDF_TRY(sph_hydro(c, 1));
-    return gravity_join(c);
+    GDF_TRY(gravity_join(c));
+    return stars_on_gas(c);
   }
   GDF_TRY(gravity_walk(c));
-  return integ_add_atree(c);
+  GDF_TRY(integ_add_atree(c));
+  return stars_on_gas(c);
 }
 
 int gdf_maximum_smoothing_length(gdf_ctx* c, double* hmax) {

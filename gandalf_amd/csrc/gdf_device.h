@@ -271,6 +271,16 @@ __device__ __forceinline__ void kdens(const KernelDesc& k, T s2, T& w0, T& wom, 
   else wz = k.type == 0 ? m4_wzeta<T>(s) : q_wzeta<T>(s);
 }
 
+// wzeta at s^2 alone (the stars' zeta terms, GradhSph.cpp:289-303)
+__device__ __forceinline__ double kwzeta_s2(const KernelDesc& k, double s2) {
+  if (k.tabulated) {
+    if (s2 >= k.rangesqd) return 0.0;
+    return k.tab[KWZETAS2*k.res + (int) (s2*k.resinvsqd)];
+  }
+  const double s = sqrt(s2);
+  return k.type == 0 ? m4_wzeta<double>(s) : q_wzeta<double>(s);
+}
+
 // ---- periodic nearest image (GhostNeighbours.hpp:119-138) ----
 template <typename T>
 __device__ __forceinline__ void nearest_image(const PhysParams& p, T& dx, T& dy, T& dz) {

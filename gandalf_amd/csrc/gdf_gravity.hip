@@ -401,6 +401,7 @@ struct NbDirectArgs {
   const double* m; const double* h;
   double* ax; double* ay; double* az; double* adx; double* ady; double* adz; double* gpot;
   KernelDesc kern;
+  int add;          // add to a, adot and gpot (after the stars' gas terms) instead of writing
 };
 
 // Star-star direct sums, one thread per star, source stars staged 256 at a time in LDS.
@@ -470,6 +471,12 @@ __global__ __launch_bounds__(256) void k_nbody_direct(NbDirectArgs a) {
     __syncthreads();
   }
   if (!has) return;
+  if (a.add) {
+    a.ax[i] += ax; a.ay[i] += ay; a.az[i] += az;
+    a.adx[i] += jx; a.ady[i] += jy; a.adz[i] += jz;
+    a.gpot[i] += pot;
+    return;
+  }
   a.ax[i] = ax; a.ay[i] = ay; a.az[i] = az;
   a.adx[i] = jx; a.ady[i] = jy; a.adz[i] = jz;
   a.gpot[i] = pot;
@@ -516,6 +523,27 @@ int nbody_direct(gdf_ctx* c, int smoothed) {
   a.adx = c->df[D_ADOTX]; a.ady = c->df[D_ADOTY]; a.adz = c->df[D_ADOTZ];
   a.gpot = c->df[D_GPOT];
   a.kern = c->phys.kern;
+  a.add = 0;
+  const int nb = (N + 255)/256;
+  if (smoothed) k_nbody_direct<true><<<nb, 256, 0, c->stream>>>(a);
+  else k_nbody_direct<false><<<nb, 256, 0, c->stream>>>(a);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+int nbody_direct_stars(gdf_ctx* c, int smoothed) {
+  const int N = c->Nstar;
+  if (N <= 0) return GDF_OK;
+  NbDirectArgs a;
+  a.N = N;
+  a.x = c->sv[S_X]; a.y = c->sv[S_Y]; a.z = c->sv[S_Z];
+  a.vx = c->sv[S_VX]; a.vy = c->sv[S_VY]; a.vz = c->sv[S_VZ];
+  a.m = c->sv[S_M]; a.h = c->sv[S_H];
+  a.ax = c->sv[S_AX]; a.ay = c->sv[S_AY]; a.az = c->sv[S_AZ];
+  a.adx = c->sv[S_ADX]; a.ady = c->sv[S_ADY]; a.adz = c->sv[S_ADZ];
+  a.gpot = c->sv[S_GPOT];
+  a.kern = c->phys.kern;
+  a.add = 1;
   const int nb = (N + 255)/256;
   if (smoothed) k_nbody_direct<true><<<nb, 256, 0, c->stream>>>(a);
   else k_nbody_direct<false><<<nb, 256, 0, c->stream>>>(a);

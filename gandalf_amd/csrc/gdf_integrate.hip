@@ -374,6 +374,7 @@ int integ_timestep(gdf_ctx* c, double* dtmin) {
                             hipMemcpyDeviceToHost, c->stream));
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   *dtmin = c->h_red[0];
+  GDF_TRY(stars_timestep(c, dtmin));     // Simulation::ComputeGlobalTimestep's star minimum
   if (!(*dtmin > 0.0) || *dtmin > 1.0e20)
     return fail(c, GDF_ERR_TIMESTEP, c->phys.nbody ? "Error : N-body timestep invalid"
                                                    : "Error : Hydro timestep too large");
@@ -404,10 +405,11 @@ int integ_diagnostics(gdf_ctx* c, double* out) {
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   double acc[ND] = {};
   for (int b = 0; b < nb; b++) for (int k = 0; k < ND; k++) acc[k] += c->h_red[b*ND + k];
+  GDF_TRY(stars_diagnostics(c, acc));    // the star loop after the gas (SimAnalysis.hpp:116-143)
   const double ketot = 0.5*acc[1], gpetot = 0.5*acc[3];
   double Etot = ketot;
   if (c->phys.hydro_forces) Etot += acc[2];
-  if (c->phys.self_gravity) Etot += gpetot;
+  if (c->phys.self_gravity || c->Nstar > 0) Etot += gpetot;
   out[0] = Etot; out[1] = ketot; out[2] = acc[2]; out[3] = gpetot;
   for (int k = 0; k < 3; k++) { out[4 + k] = acc[4 + k]; out[7 + k] = acc[7 + k]; }
   out[10] = acc[0];

@@ -47,6 +47,16 @@ enum DField {
   D_NDERIVED
 };
 
+// Stars of a gas + star run (NbodyParticle, src/Headers/NbodyParticle.h), in import order, one
+// device array per field: position, velocity, acceleration, jerk, their values at the start of
+// the step, mass, smoothing length, potential.
+enum SField {
+  S_X, S_Y, S_Z, S_VX, S_VY, S_VZ, S_AX, S_AY, S_AZ, S_ADX, S_ADY, S_ADZ,
+  S_X0, S_Y0, S_Z0, S_VX0, S_VY0, S_VZ0, S_AX0, S_AY0, S_AZ0, S_M, S_H, S_GPOT, S_N
+};
+// the star-from-gas sums take chunks of this many gas particles per workgroup
+constexpr int kStarChunk = 2048;
+
 // Smoothing-kernel description passed by value to device code.
 struct KernelDesc {
   int type;            // 0 M4, 1 quintic
@@ -199,6 +209,14 @@ struct gdf_ctx {
   int64_t snap_nout = 0, snap_nlite = 0;
   double snap_tlast = 0.0, snap_tlitelast = 0.0;
   std::vector<int> snap_porig;
+
+  // stars beside the gas (gdf_set_stars): star-gas gravity both ways, star-star sums, the
+  // stars' leapfrog KDK (SphSimulation::MainLoop's N-body half)
+  int Nstar = 0, Nstar_cap = 0;
+  double star_tlast = 0.0;         // the stars' last end-step time (NbodyParticle tlast)
+  double* sv[gdf::S_N] = {};
+  double* star_part = nullptr;     // per-chunk partial sums of the star-from-gas pass
+  size_t star_part_cap = 0;
 };
 
 namespace gdf {
@@ -276,6 +294,18 @@ int integ_seg_timesteps(gdf_ctx* c, double* out);
 int integ_seg_advance(gdf_ctx* c);
 int integ_seg_end_step(gdf_ctx* c);
 int nbody_direct(gdf_ctx* c, int smoothed);
+// stars beside the gas (gdf_stars.hip)
+int nbody_direct_stars(gdf_ctx* c, int smoothed);   // star-star sums added to the stars
+void stars_free(gdf_ctx* c);
+// HydroTree::UpdateAllStarGasForces when with_gas, added to the stars' a and gpot (add) or
+// written over a, gpot with adot cleared (!add: the MainLoop's zeroing of the stars)
+int stars_gas_on_stars(gdf_ctx* c, int with_gas, int add);
+int stars_on_gas(gdf_ctx* c);         // GradhSph::ComputeStarGravForces for the active gas
+int stars_advance(gdf_ctx* c, double t);
+int stars_correction(gdf_ctx* c, double t);
+int stars_end_step(gdf_ctx* c, double t);
+int stars_timestep(gdf_ctx* c, double* dtmin);
+int stars_diagnostics(gdf_ctx* c, double* acc11);   // adds the stars' sums to acc
 
 inline int nblocks(long long n, int bs) { return (int) ((n + bs - 1) / bs); }
 

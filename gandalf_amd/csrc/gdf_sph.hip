@@ -227,6 +227,9 @@ struct DensArgs {
   double* h; double* hsearch; double* rho; double* invomega; double* zeta; double* hfactor;
   double* u; double* sound; double* pressure;
   int* status; int* flags; long long* icount;
+  // stars of a gas + star run for their zeta terms (nstar = 0: none, or the terms are off)
+  int nstar, star_soft;
+  const double* sx; const double* sy; const double* sz; const double* sm; const double* sh;
   PhysParams p;
 };
 
@@ -260,6 +263,20 @@ __device__ void hrho_finish(const DensArgs& a, const PhysParams& p, int i, doubl
   const double deriv = -(1.0/3)*h/rho;
   double iom = 1.0 - deriv*invomega;
   iom = 1.0/iom;
+  // the stars' conservative-gravity zeta terms (Hubber et al. 2013) at the final h, over the
+  // pair's mean smoothing length or the particle's own (GradhSph.cpp:282-303)
+  if (a.nstar > 0) {
+#pragma clang fp contract(off)
+    const double4 me = a.posm[i];
+    for (int j = 0; j < a.nstar; j++) {
+      double invhsqd;
+      if (a.star_soft) { const double t = 2.0/(h + a.sh[j]); invhsqd = t*t; }
+      else invhsqd = 4.0*invh*invh;
+      const double dx = a.sx[j] - me.x, dy = a.sy[j] - me.y, dz = a.sz[j] - me.z;
+      const double ssqd = (dx*dx + dy*dy + dz*dz)*invhsqd;
+      zeta += a.sm[j]*invhsqd*kwzeta_s2(p.kern, ssqd);
+    }
+  }
   const double zt = deriv*zeta*iom;
   a.h[i] = h; a.rho[i] = rho; a.invomega[i] = iom; a.zeta[i] = zt; a.hfactor[i] = hfac4;
   double u = a.u[i], snd, press;
@@ -797,6 +814,9 @@ int sph_density(gdf_ctx* c, int nopbc) {
   d.invomega = c->df[D_INVOMEGA]; d.zeta = c->df[D_ZETA]; d.hfactor = c->df[D_HFACTOR];
   d.u = c->pf[F_U]; d.sound = c->df[D_SOUND]; d.pressure = c->df[D_PRESSURE];
   d.status = c->status; d.flags = c->counters; d.icount = c->icount;
+  d.nstar = c->cfg.conservative_sph_star_gravity ? c->Nstar : 0;
+  d.star_soft = c->cfg.nbody_softening;
+  d.sx = c->sv[S_X]; d.sy = c->sv[S_Y]; d.sz = c->sv[S_Z]; d.sm = c->sv[S_M]; d.sh = c->sv[S_H];
   d.p = c->phys;
   for (int round = 0; round < 64; round++) {
     // also clears the round's flags and the walk's queue counters

@@ -46,7 +46,9 @@ template <int ndim>
 int GradhSphTreeGpu<ndim>::UpdateAllSphHydroForces() {
   GDF_TRY(zero_accelerations(ctx));
   GDF_TRY(gdf::sph_build_sg(ctx));
-  return gdf::sph_hydro(ctx, 0);
+  GDF_TRY(gdf::sph_hydro(ctx, 0));
+  // the stars' pull on the active gas (GradhSphTree.cpp:385-391)
+  return gdf::stars_on_gas(ctx);
 }
 
 template <int ndim>
@@ -59,11 +61,13 @@ int GradhSphTreeGpu<ndim>::UpdateAllSphForces() {
     GDF_TRY(gdf::sph_build_sg(ctx));
     GDF_TRY(gdf::sph_hydro(ctx, 1));
     record(ctx, 3);
-    return gdf::gravity_join(ctx);
+    GDF_TRY(gdf::gravity_join(ctx));
+    return gdf::stars_on_gas(ctx);      // GradhSphTree.cpp:600-607
   }
   record(ctx, 3);
   GDF_TRY(gdf::gravity_walk(ctx));
-  return gdf::integ_add_atree(ctx);
+  GDF_TRY(gdf::integ_add_atree(ctx));
+  return gdf::stars_on_gas(ctx);
 }
 
 template <int ndim>
@@ -73,7 +77,9 @@ int GradhSphTreeGpu<ndim>::GetMaximumSmoothingLength(double* hmax) {
 
 template <int ndim>
 int SphLeapfrogKDKGpu<ndim>::AdvanceParticles(int, double t, double) {
-  return gdf::integ_advance(ctx, t - ctx->tlast);
+  GDF_TRY(gdf::integ_advance(ctx, t - ctx->tlast));
+  // NbodyLeapfrogKDK::AdvanceParticles beside the gas's (SphSimulation.cpp:594-595)
+  return gdf::stars_advance(ctx, t);
 }
 
 template <int ndim>
@@ -84,6 +90,8 @@ int SphLeapfrogKDKGpu<ndim>::CheckBoundaries() {
 template <int ndim>
 int SphLeapfrogKDKGpu<ndim>::EndTimestep(int, double t, double timestep) {
   GDF_TRY(gdf::integ_end_step(ctx, timestep));
+  // the stars' end-step terms after the gas's (SphSimulation.cpp:874-881)
+  GDF_TRY(gdf::stars_end_step(ctx, t));
   ctx->tlast = t;
   return GDF_OK;
 }
@@ -91,6 +99,16 @@ int SphLeapfrogKDKGpu<ndim>::EndTimestep(int, double t, double timestep) {
 template <int ndim>
 int GradhSphSimulationGpu<ndim>::ComputeGlobalTimestep(double* dt) {
   return gdf::integ_timestep(ctx, dt);
+}
+
+// The N-body iteration of the step (SphSimulation.cpp:775-807, Npec = 1): the stars' a, adot
+// and gpot cleared, the gas's pull (self-gravitating runs), the star-star sums, and the
+// velocity correction for the step that ends at t
+static int star_forces(gdf_ctx* c, bool correct) {
+  if (c->Nstar <= 0) return GDF_OK;
+  GDF_TRY(gdf::stars_gas_on_stars(c, c->phys.self_gravity && c->N > 0, 0));
+  GDF_TRY(gdf::nbody_direct_stars(c, c->cfg.nbody_softening));
+  return correct ? gdf::stars_correction(c, c->t) : GDF_OK;
 }
 
 template <int ndim>
@@ -139,6 +157,7 @@ int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
   GDF_TRY(ComputeForces());
   c->t = t0;
   c->tlast = t0;
+  GDF_TRY(star_forces(c, false));      // the initial N-body forces (SphSimulation.cpp:490-502)
   if (c->nlevels > 1) {
     // the first ComputeBlockTimesteps resynchronises (n = nresync = 0); EndTimestep with the
     // particles' initial dt = 0 sets r0, v0, a0, u0, dudt0 and each particle's first step
@@ -178,6 +197,7 @@ int GradhSphSimulationGpu<ndim>::MainLoop() {
   GDF_TRY(sphneib.UpdateAllSphProperties(false));
   record(c, 5);
   GDF_TRY(ComputeForces());
+  GDF_TRY(star_forces(c, true));
   record(c, 4);
   double dtnew = 0.0;
   GDF_TRY(ComputeGlobalTimestep(&dtnew));
@@ -349,8 +369,18 @@ template class GradhSphSimulationGpu<3>;
 
 extern "C" {
 
+// the whole-step drivers integrate stars with one global timestep only (the per-method entry
+// points serve a caller's own block-timestep loop)
+static int stars_need_global_steps(gdf_ctx* c) {
+  if (c->Nstar > 0 && c->nlevels > 1)
+    return gdf::fail(c, GDF_ERR_INVALID, "stars beside the gas need one timestep level "
+                                         "(Nlevels = 1) in gdf_setup_simulation / gdf_main_loop");
+  return GDF_OK;
+}
+
 int gdf_setup_simulation(gdf_ctx* c) {
   if (!c || c->N <= 0) return GDF_ERR_INVALID;
+  GDF_TRY(stars_need_global_steps(c));
   if (c->phys.nbody) {
     gandalf_gpu::NbodySimulationGpu<3> sim(c);
     return sim.PostInitialConditionsSetup();
@@ -361,6 +391,7 @@ int gdf_setup_simulation(gdf_ctx* c) {
 
 int gdf_main_loop(gdf_ctx* c, int nsteps) {
   if (!c || c->N <= 0 || nsteps < 0) return GDF_ERR_INVALID;
+  GDF_TRY(stars_need_global_steps(c));
   if (c->phys.nbody) {
     gandalf_gpu::NbodySimulationGpu<3> sim(c);
     for (int s = 0; s < nsteps; s++) GDF_TRY(sim.MainLoop());

@@ -43,6 +43,7 @@ class Config(C.Structure):
         ("Nlevels", C.c_int), ("level_diff_max", C.c_int),
         ("hrho_iteration_max", C.c_int), ("cand_batches_init", C.c_int),
         ("restart", C.c_int),
+        ("nbody_softening", C.c_int), ("conservative_sph_star_gravity", C.c_int),
     ]
 
 
@@ -103,6 +104,14 @@ PROTOTYPES = {
     "gdf_end_timestep_block": (_i, [_vp, _i, _d, _d]),
     "gdf_get_gather_neighbour_list": (_i, [_vp, _vp, _d, _i, _vp]),
     "gdf_snapshot_counters": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "gdf_set_stars": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "gdf_num_stars": (_i, [_vp]),
+    "gdf_get_star_field": (_i, [_vp, C.c_char_p, _vp]),
+    "gdf_update_all_star_gas_forces": (_i, [_vp]),
+    "gdf_nbody_star_forces": (_i, [_vp, _i]),
+    "gdf_nbody_advance_particles": (_i, [_vp, _d]),
+    "gdf_nbody_correction_terms": (_i, [_vp, _d]),
+    "gdf_nbody_end_timestep": (_i, [_vp, _d]),
 }
 
 
@@ -153,8 +162,10 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
     # the reference's direct sums selected by nbody_softening
     c.nbody = 1 if prm.get("sim", "gradhsph") == "nbody" else 0
     c.nbody_eps = float(prm.get("nbody_eps", 0.0))
+    c.nbody_softening = int(prm.get("nbody_softening", 1))
+    c.conservative_sph_star_gravity = int(prm.get("conservative_sph_star_gravity", 1))
     if c.nbody and c.nbody_eps <= 0.0:
-        c.nbody_direct = 2 if int(prm.get("nbody_softening", 0)) else 1
+        c.nbody_direct = 2 if c.nbody_softening else 1
     c.nbody_mult = float(prm.get("nbody_mult", 0.1))
     c.fixed_dt = float(prm.get("fixed_dt", 0.0))
     c.Nlevels = int(prm.get("Nlevels", 1))
@@ -331,6 +342,39 @@ class GpuSim:
         """Star-star direct sums (Nbody::CalculateDirectGravForces, or the kernel-smoothed
         NbodyLeapfrogKDK variant): fills a, adot and gpot."""
         self._chk(lib().gdf_nbody_direct_forces(self._h, int(bool(smoothed))))
+
+    # ---- stars beside the gas (Nbody<3>::stardata of an SphSimulation) ----
+    STAR_FIELDS3 = ("r", "v", "a", "adot", "r0", "v0", "a0")
+
+    def set_stars(self, r, v, m, h):
+        r, v, m, h = _d(r).reshape(-1, 3), _d(v).reshape(-1, 3), _d(m), _d(h)
+        self._chk(lib().gdf_set_stars(self._h, r.shape[0], _p(r), _p(v), _p(m), _p(h)))
+
+    @property
+    def Nstar(self):
+        return lib().gdf_num_stars(self._h)
+
+    def star(self, name):
+        n = self.Nstar
+        w = 3 if name in self.STAR_FIELDS3 else 1
+        out = np.empty(n * w)
+        self._chk(lib().gdf_get_star_field(self._h, name.encode(), _p(out)))
+        return out.reshape(n, 3) if w == 3 else out
+
+    def update_all_star_gas_forces(self):
+        self._chk(lib().gdf_update_all_star_gas_forces(self._h))
+
+    def nbody_star_forces(self, zero_first=False):
+        self._chk(lib().gdf_nbody_star_forces(self._h, int(bool(zero_first))))
+
+    def nbody_advance_particles(self, t):
+        self._chk(lib().gdf_nbody_advance_particles(self._h, float(t)))
+
+    def nbody_correction_terms(self, t):
+        self._chk(lib().gdf_nbody_correction_terms(self._h, float(t)))
+
+    def nbody_end_timestep(self, t):
+        self._chk(lib().gdf_nbody_end_timestep(self._h, float(t)))
 
     # ---- per-method entry points (TimeIntegration / NeighbourSearch of the reference) ----
     def advance_particles_block(self, n, t, timestep):

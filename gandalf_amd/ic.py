@@ -173,18 +173,24 @@ def plummer_sphere(N=131072, seed=4, a=1.0, mass=1.0, rmax_frac=0.999):
     return dict(r=r, v=v, m=m, u=np.zeros(N), params=params)
 
 
-def write_column_file(path, ic, stars=False, h=None):
+def write_column_file(path, ic, stars=False, h=None, star_ic=None):
     """Reference column snapshot (src/Common/SimulationIO.hpp:193-270): header Nhydro, Nstar,
     ndim, t; then x y z vx vy vz m h rho u per particle, at full precision.  With ``stars``
     every particle is written as a star (Nhydro = 0) and ``h`` gives the stars' smoothing
-    lengths, which the reference's N-body sums and timesteps use."""
+    lengths, which the reference's N-body sums and timesteps use.  ``star_ic`` (r, v, m, h)
+    adds stars after the gas."""
     r, v, m, u = ic["r"], ic["v"], ic["m"], ic["u"]
     N = r.shape[0]
     hh = np.zeros(N) if h is None else np.asarray(h, dtype=np.float64)
     cols = np.column_stack([r, v, m, hh, np.zeros(N), u])
+    ns = 0 if star_ic is None else star_ic["r"].shape[0]
     with open(path, "w") as f:
-        f.write(f"0\n{N}\n3\n0.0\n" if stars else f"{N}\n0\n3\n0.0\n")
+        f.write(f"0\n{N}\n3\n0.0\n" if stars else f"{N}\n{ns}\n3\n0.0\n")
         np.savetxt(f, cols, fmt="%.17g")
+        if ns:
+            sc = np.column_stack([star_ic["r"], star_ic["v"], star_ic["m"], star_ic["h"],
+                                  np.zeros(ns), np.zeros(ns)])
+            np.savetxt(f, sc, fmt="%.17g")
 
 
 def write_param_file(path, params, in_file, run_id="REF"):

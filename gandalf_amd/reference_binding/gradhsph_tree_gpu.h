@@ -27,6 +27,7 @@
 
 #include "Exception.h"
 #include "Integration.h"
+#include "Nbody.h"
 #include "SphNeighbourSearch.h"
 #include "gandalf_gpu.h"
 
@@ -40,7 +41,7 @@ inline void check(gdf_ctx* ctx, int rc) {
 template <int ndim, template <int> class ParticleType>
 class DeviceState {
  public:
-  explicit DeviceState(const gdf_config& cfg) : ctx(nullptr), N(0) {
+  explicit DeviceState(const gdf_config& cfg) : ctx(nullptr), N(0), Nstar(0) {
     const int rc = gdf_create(&cfg, &ctx);
     if (rc != GDF_OK) ExceptionHandler::getIstance().raise("gdf_create failed");
   }
@@ -70,6 +71,22 @@ class DeviceState {
     check(ctx, gdf_set_field(ctx, "u", &u[0]));
   }
 
+  // the stars' r, v, m, h (Nbody<ndim>::stardata) to the device before a pass that reads them:
+  // the h-rho pass (their zeta terms), the force passes (their pull on the gas) and the star-gas
+  // sums
+  void UploadStars(Nbody<ndim>* nbody) {
+    const int n = nbody ? nbody->Nstar : 0;
+    if (n == 0 && Nstar == 0) return;
+    std::vector<double> r(3*n + 1, 0.0), v(3*n + 1, 0.0), m(n + 1), h(n + 1);
+    for (int i = 0; i < n; i++) {
+      const NbodyParticle<ndim>& st = nbody->stardata[i];
+      for (int k = 0; k < ndim; k++) { r[3*i + k] = st.r[k]; v[3*i + k] = st.v[k]; }
+      m[i] = st.m; h[i] = st.h;
+    }
+    check(ctx, gdf_set_stars(ctx, n, &r[0], &v[0], &m[0], &h[0]));
+    Nstar = n;
+  }
+
   // a device field into the reference's particles: width 1 or 3 (vectors by component)
   template <typename Set>
   void Download(Hydrodynamics<ndim>* hydro, const char* name, int width, Set set) {
@@ -84,7 +101,7 @@ class DeviceState {
   }
 
   gdf_ctx* ctx;
-  int N;
+  int N, Nstar;
   std::vector<double> buf;
 };
 
@@ -131,9 +148,24 @@ class GradhSphTreeGpu : public SphNeighbourSearch<ndim> {
     for (int i = 0; i < n; i++) act[i] = State::Part(hydro, i).flags.check(active) ? 1.0 : 0.0;
     check(dev->ctx, gdf_set_field(dev->ctx, "active", &act[0]));
   }
-  virtual void UpdateAllStarGasForces(Hydrodynamics<ndim>*, Nbody<ndim>*, DomainBox<ndim>&,
-                                      Ewald<ndim>*) {
-    ExceptionHandler::getIstance().raise("neib_search = gpu: star-gas forces are not built");
+  // HydroTree::UpdateAllStarGasForces (HydroTree.cpp:552-654): the gas's pull on every active
+  // star, added to its a and gpot, through the brute-force lists (every gas particle direct)
+  virtual void UpdateAllStarGasForces(Hydrodynamics<ndim>*, Nbody<ndim>* nbody,
+                                      DomainBox<ndim>& simbox, Ewald<ndim>*) {
+    if (simbox.PeriodicGravity)
+      ExceptionHandler::getIstance().raise("neib_search = gpu: periodic gravity is not built");
+    dev->UploadStars(nbody);
+    check(dev->ctx, gdf_update_all_star_gas_forces(dev->ctx));
+    const int n = nbody->Nstar;
+    std::vector<double> a(3*n + 1), gpot(n + 1);
+    check(dev->ctx, gdf_get_star_field(dev->ctx, "a", &a[0]));
+    check(dev->ctx, gdf_get_star_field(dev->ctx, "gpot", &gpot[0]));
+    for (int i = 0; i < n; i++) {
+      NbodyParticle<ndim>& st = nbody->stardata[i];
+      if (!st.flags.check(active)) continue;
+      for (int k = 0; k < ndim; k++) st.a[k] += a[3*i + k];
+      st.gpot += gpot[i];
+    }
   }
   virtual double GetMaximumSmoothingLength() const {
     double hmax = 0.0;
@@ -157,7 +189,8 @@ class GradhSphTreeGpu : public SphNeighbourSearch<ndim> {
 
   // ---- SphNeighbourSearch<ndim> ----
   // GradhSphTree::UpdateAllSphProperties (src/GradhSph/GradhSphTree.cpp:83-271)
-  virtual void UpdateAllSphProperties(Sph<ndim>* sph, Nbody<ndim>*) {
+  virtual void UpdateAllSphProperties(Sph<ndim>* sph, Nbody<ndim>* nbody) {
+    dev->UploadStars(nbody);
     check(dev->ctx, gdf_update_all_sph_properties(dev->ctx, 0));
     const FLOAT krsqd = sph->kernp->kernrangesqd;
     dev->Download(sph, "h", 1, [krsqd](ParticleType<ndim>& p, const double* x) {
@@ -174,15 +207,17 @@ class GradhSphTreeGpu : public SphNeighbourSearch<ndim> {
     dev->Download(sph, "u", 1, [](ParticleType<ndim>& p, const double* x) { p.u = x[0]; });
   }
   // GradhSphTree::UpdateAllSphHydroForces (GradhSphTree.cpp:280-435)
-  virtual void UpdateAllSphHydroForces(Sph<ndim>* sph, Nbody<ndim>*, DomainBox<ndim>&) {
+  virtual void UpdateAllSphHydroForces(Sph<ndim>* sph, Nbody<ndim>* nbody, DomainBox<ndim>&) {
+    dev->UploadStars(nbody);
     check(dev->ctx, gdf_update_all_sph_hydro_forces(dev->ctx));
-    DownloadForces(sph, false);
+    DownloadForces(sph, nbody && nbody->Nstar > 0);
   }
   // GradhSphTree::UpdateAllSphForces (GradhSphTree.cpp:444-657); no Ewald (periodic gravity)
-  virtual void UpdateAllSphForces(Sph<ndim>* sph, Nbody<ndim>*, DomainBox<ndim>& simbox,
+  virtual void UpdateAllSphForces(Sph<ndim>* sph, Nbody<ndim>* nbody, DomainBox<ndim>& simbox,
                                   Ewald<ndim>*) {
     if (simbox.PeriodicGravity)
       ExceptionHandler::getIstance().raise("neib_search = gpu: periodic gravity is not built");
+    dev->UploadStars(nbody);
     check(dev->ctx, gdf_update_all_sph_forces(dev->ctx));
     DownloadForces(sph, true);
   }

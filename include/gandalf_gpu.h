@@ -71,6 +71,12 @@ typedef struct gdf_config {
    * the file's particle ids; otherwise both are reset as for "ic = file"
    * (SimulationIO.hpp:1385-1394, SphSimulation.cpp:215-217) */
   int restart;
+  /* stars beside the gas (gdf_set_stars): "nbody_softening" (1 = star-star sums and the stars'
+   * zeta terms over the mean smoothing length of the pair, 0 = star-star point masses and the
+   * gas particle's own h) and "conservative_sph_star_gravity" (the Hubber et al. 2013 zeta
+   * terms of the stars in the gas h-rho pass, GradhSph.cpp:282-307) */
+  int nbody_softening;
+  int conservative_sph_star_gravity;
 } gdf_config;
 
 typedef struct gdf_ctx gdf_ctx;
@@ -236,6 +242,36 @@ int gdf_gravity_group_counts(gdf_ctx* ctx, int32_t* ncell, int32_t* nnear);
  * smoothed = 1 NbodyLeapfrogKDK::CalculateDirectSmoothedGravForces
  * (src/Nbody/NbodyLeapfrogKDK.cpp:78-150) */
 int gdf_nbody_direct_forces(gdf_ctx* ctx, int smoothed);
+
+/* ---- stars beside the gas: SphSimulation with an N-body integrator (Nbody<3>::stardata,
+ * src/Headers/Nbody.h; the N-body half of SphSimulation::MainLoop, SphSimulation.cpp:595,
+ * 775-833, 881).  The stars ride in the gas context, in import order, in FP64.
+ * gdf_setup_simulation and gdf_main_loop then also give the gas the stars' pull (GradhSph::
+ * ComputeStarGravForces, GradhSph.cpp:699-743) and the stars' zeta terms, the stars the gas's
+ * pull (HydroTree::UpdateAllStarGasForces, HydroTree.cpp:552-654, every gas particle summed
+ * directly: the reference's brute-force search) and each other's, and advance the stars by
+ * NbodyLeapfrogKDK (src/Nbody/NbodyLeapfrogKDK.cpp:253-401), with one timestep level (Nlevels
+ * = 1); the per-method entry points below also serve a caller's block-timestep loop. ---- */
+/* Simulation::ImportArray's star half / the star lines of a column snapshot: nstar stars,
+ * r, v (nstar,3) row-major, m, h nstar doubles; replaces earlier stars */
+int gdf_set_stars(gdf_ctx* ctx, int nstar, const double* r, const double* v, const double* m,
+                  const double* h);
+int gdf_num_stars(const gdf_ctx* ctx);
+/* a star field in import order: r v a adot r0 v0 a0 (nstar,3), m h gpot (nstar) */
+int gdf_get_star_field(gdf_ctx* ctx, const char* name, double* out);
+/* HydroTree::UpdateAllStarGasForces: adds the gas's pull on every star to a and gpot */
+int gdf_update_all_star_gas_forces(gdf_ctx* ctx);
+/* the stars' zeroing and star-star sums of the MainLoop's N-body iteration (SphSimulation.cpp:
+ * 778-807): zero_first clears a, adot, gpot (before gdf_update_all_star_gas_forces); else the
+ * star-star sums (Nbody::CalculateDirectGravForces or NbodyLeapfrogKDK::
+ * CalculateDirectSmoothedGravForces, per nbody_softening) are added */
+int gdf_nbody_star_forces(gdf_ctx* ctx, int zero_first);
+/* NbodyLeapfrogKDK::AdvanceParticles, CorrectionTerms and EndTimestep(t) for the stars
+ * (NbodyLeapfrogKDK.cpp:253-370): a step runs from the stars' last end-step time (their
+ * import time before the first) to t */
+int gdf_nbody_advance_particles(gdf_ctx* ctx, double t);
+int gdf_nbody_correction_terms(gdf_ctx* ctx, double t);
+int gdf_nbody_end_timestep(gdf_ctx* ctx, double t);
 
 /* ---- serving pool: no counterpart in the reference, which runs one simulation per process.
  * Independent requests of one configuration class (one gdf_config) share a device context:

@@ -497,7 +497,7 @@ typedef struct { double h, rho, invomega, zeta, hfactor; } hres;
 /* returns 1 ok, 0 = h grew beyond hmax (retry), -1 final h > hmax (accepted), -2 failure */
 static int compute_h_one(const okern* kern, const orc_params* p, const double* xi, double m,
                          double h0, double hmax, int nn, const double* nx, const double* nm,
-                         hres* out) {
+                         const orc_stars* st, hres* out) {
   int iteration = 0;
   const int iteration_max = 30;
   double h_lower_bound = 0.0, h_upper_bound = hmax;
@@ -539,6 +539,16 @@ static int compute_h_one(const okern* kern, const orc_params* p, const double* x
   out->hfactor = pow(invh, 4);
   invomega = 1.0 - h_rho_deriv(h, m, rho)*invomega;
   invomega = 1.0/invomega;
+  /* the stars' conservative-gravity zeta terms (Hubber et al. 2013), GradhSph.cpp:282-303 */
+  if (st && st->conservative) {
+    for (int j=0; j<st->n; j++) {
+      const double isq = st->softening ? pow(2.0/(h + st->h[j]), 2) : 4.0*invh*invh;
+      const double dr0 = st->r[3*j] - xi[0], dr1 = st->r[3*j+1] - xi[1];
+      const double dr2 = st->r[3*j+2] - xi[2];
+      const double ssqd = (dr0*dr0 + dr1*dr1 + dr2*dr2)*isq;
+      zeta += st->m[j]*isq*K_wzeta_s2(kern, ssqd);
+    }
+  }
   zeta = h_rho_deriv(h, m, rho)*zeta*invomega;
   out->h = h; out->rho = rho; out->invomega = invomega; out->zeta = zeta;
   return h <= hmax ? 1 : -1;
@@ -549,7 +559,7 @@ static int compute_h_one(const okern* kern, const orc_params* p, const double* x
 static int compute_h_masked(const orc_params* p, int N, const double* r, const double* m,
                             int nopbc, double* h, double* rho, double* invomega, double* zeta,
                             double* hfactor, double* u, double* sound, double* pressure,
-                            const unsigned char* act) {
+                            const unsigned char* act, const orc_stars* st) {
   okern kern; kern_init(&kern, p);
   obox b; box_init(&b, p, nopbc);
   const double kr = kern.kernrange;
@@ -603,7 +613,7 @@ static int compute_h_masked(const orc_params* p, int N, const double* r, const d
           }
           free(lst.data);
         }
-        ok = compute_h_one(&kern, p, xi, m[i], h[i], hmax, nn, nx, nm, &res);
+        ok = compute_h_one(&kern, p, xi, m[i], h[i], hmax, nn, nx, nm, st, &res);
       } while (ok == 0);
       if (ok == -2) {
         #pragma omp atomic write
@@ -625,7 +635,80 @@ int orc_compute_h(const orc_params* p, int N, const double* r, const double* m, 
                   double* h, double* rho, double* invomega, double* zeta, double* hfactor,
                   double* u, double* sound, double* pressure) {
   return compute_h_masked(p, N, r, m, nopbc, h, rho, invomega, zeta, hfactor, u, sound,
-                          pressure, NULL);
+                          pressure, NULL, NULL);
+}
+
+int orc_compute_h_stars(const orc_params* p, int N, const double* r, const double* m,
+                        int nopbc, const orc_stars* st, double* h, double* rho,
+                        double* invomega, double* zeta, double* hfactor, double* u,
+                        double* sound, double* pressure) {
+  return compute_h_masked(p, N, r, m, nopbc, h, rho, invomega, zeta, hfactor, u, sound,
+                          pressure, NULL, st);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Stars beside the gas                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+/* GradhSph::ComputeStarGravForces (GradhSph.cpp:699-743) for every gas particle: the stars'
+ * kernel-softened pull over the mean smoothing length, added to a and gpot */
+void orc_gas_from_stars(const orc_params* p, int N, const double* r, const double* h,
+                        const orc_stars* st, double* a, double* gpot) {
+  okern kern; kern_init(&kern, p);
+  #pragma omp parallel for schedule(static)
+  for (int i=0; i<N; i++) {
+    for (int j=0; j<st->n; j++) {
+      double dr[3];
+      for (int k=0; k<3; k++) dr[k] = st->r[3*j+k] - r[3*i+k];
+      const double drsqd = dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2] + SMALL_NUMBER;
+      const double drmag = sqrt(drsqd);
+      const double invdrmag = 1.0/drmag;
+      const double invhmean = 2.0/(h[i] + st->h[j]);
+      const double paux = st->m[j]*invhmean*invhmean*K_wgrav(&kern, drmag*invhmean)*invdrmag;
+      for (int k=0; k<3; k++) a[3*i+k] += paux*dr[k];
+      gpot[i] += st->m[j]*invhmean*K_wpot(&kern, drmag*invhmean);
+    }
+  }
+  kern_free(&kern);
+}
+
+/* HydroTree::UpdateAllStarGasForces (HydroTree.cpp:552-654) with the brute-force search's lists
+ * (Tree.cpp:778-860 with one particle per cell): gas within (k h_s)/2 + (k/2) h_j of the star
+ * in the softened sums of NbodyLeapfrogKDK::CalculateDirectHydroForces (NbodyLeapfrogKDK.cpp:
+ * 178-200), the rest as point masses (:206-229); softened first, then direct, each in index
+ * order.  Writes a and gpot of every star. */
+void orc_stars_from_gas(const orc_params* p, const orc_stars* st, int N, const double* r,
+                        const double* m, const double* h, double* a, double* gpot) {
+  okern kern; kern_init(&kern, p);
+  const double kr = kern.kernrange;
+  for (int s=0; s<st->n; s++) {
+    const double* rs = &st->r[3*s];
+    double as[3] = {0.0, 0.0, 0.0}, pot = 0.0;
+    for (int pass=0; pass<2; pass++) {
+      for (int j=0; j<N; j++) {
+        double dr[3];
+        for (int k=0; k<3; k++) dr[k] = r[3*j+k] - rs[k];
+        const double drsqd = dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2];
+        const double rn = 0.5*(kr*st->h[s]) + 0.5*kr*h[j];
+        const int near = drsqd < pow(rn, 2);
+        if (near != (pass == 0)) continue;
+        const double drmag = sqrt(drsqd);
+        const double invdrmag = 1.0/drmag;
+        double paux;
+        if (near) {
+          const double invhmean = 2.0/(st->h[s] + h[j]);
+          paux = m[j]*invhmean*invhmean*K_wgrav(&kern, drmag*invhmean)*invdrmag;
+          pot += m[j]*invhmean*K_wpot(&kern, drmag*invhmean);
+        } else {
+          paux = m[j]*pow(invdrmag, 3);
+          pot += m[j]*invdrmag;
+        }
+        for (int k=0; k<3; k++) as[k] += paux*dr[k];
+      }
+    }
+    for (int k=0; k<3; k++) a[3*s+k] = as[k];
+    gpot[s] = pot;
+  }
+  kern_free(&kern);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -1403,7 +1486,40 @@ struct orc_sim {
   double *ptlast, *pdt, *dt_next, *dtw;
   unsigned char *active, *endstep;
   long long nfinished;   /* particles that ended a step in the last MainLoop */
+  /* stars beside the gas (orc_sim_set_stars) */
+  int ns, star_soft, star_cons;
+  double nbody_mult;
+  double *sr, *sv, *sa, *sadot, *sr0, *sv0, *sa0, *sm, *sh, *sgpot;
+  orc_stars st;
 };
+
+static const orc_stars* sim_stars(orc_sim* s) {
+  if (s->ns <= 0) return NULL;
+  s->st.n = s->ns; s->st.r = s->sr; s->st.m = s->sm; s->st.h = s->sh;
+  s->st.softening = s->star_soft; s->st.conservative = s->star_cons;
+  return &s->st;
+}
+
+int orc_sim_set_stars(orc_sim* s, int n, const double* r, const double* v, const double* m,
+                      const double* h, int softening, int conservative, double nbody_mult) {
+  if (n < 0) return -1;
+  double** f[] = {&s->sr, &s->sv, &s->sa, &s->sadot, &s->sr0, &s->sv0, &s->sa0};
+  for (unsigned k=0; k<sizeof(f)/sizeof(f[0]); k++) {
+    free(*f[k]);
+    *f[k] = calloc((size_t) 3*(n > 0 ? n : 1), sizeof(double));
+  }
+  free(s->sm); free(s->sh); free(s->sgpot);
+  s->sm = calloc((size_t) (n > 0 ? n : 1), sizeof(double));
+  s->sh = calloc((size_t) (n > 0 ? n : 1), sizeof(double));
+  s->sgpot = calloc((size_t) (n > 0 ? n : 1), sizeof(double));
+  s->ns = n; s->star_soft = softening; s->star_cons = conservative; s->nbody_mult = nbody_mult;
+  if (n > 0) {
+    memcpy(s->sr, r, sizeof(double)*3*n); memcpy(s->sr0, r, sizeof(double)*3*n);
+    memcpy(s->sv, v, sizeof(double)*3*n); memcpy(s->sv0, v, sizeof(double)*3*n);
+    memcpy(s->sm, m, sizeof(double)*n); memcpy(s->sh, h, sizeof(double)*n);
+  }
+  return 0;
+}
 
 orc_sim* orc_sim_create(const orc_params* p, int N, const double* r, const double* v,
                         const double* m, const double* u) {
@@ -1433,6 +1549,8 @@ void orc_sim_destroy(orc_sim* s) {
   free(s->u0); free(s->dudt0);
   free(s->level); free(s->levelneib); free(s->nstep); free(s->nlast); free(s->ptlast);
   free(s->pdt); free(s->dt_next); free(s->dtw); free(s->active); free(s->endstep);
+  free(s->sr); free(s->sv); free(s->sa); free(s->sadot); free(s->sr0); free(s->sv0);
+  free(s->sa0); free(s->sm); free(s->sh); free(s->sgpot);
   free(s);
 }
 
@@ -1452,7 +1570,7 @@ static int sim_forces(orc_sim* s, const unsigned char* act) {
   const int N = s->N;
   const orc_params* p = &s->p;
   if (compute_h_masked(p, N, s->r, s->m, 0, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
-                       s->u, s->sound, s->pressure, act)) return -1;
+                       s->u, s->sound, s->pressure, act, sim_stars(s))) return -1;
   zero_active(s, act);
   /* ComputeThermalProperties for every particle (nradstep = 1) */
   if (act) for (int i=0; i<N; i++) eos_thermal(p, s->rho[i], &s->u[i], &s->sound[i],
@@ -1478,7 +1596,54 @@ static int sim_forces(orc_sim* s, const unsigned char* act) {
       for (int k=0; k<3; k++) s->a[3*i+k] += s->atree[3*i+k];
     }
   }
+  if (s->ns > 0) {
+    /* the stars' pull on the gas, into a and atree (GradhSphTree.cpp:600-617) */
+    double* ga = calloc((size_t) 3*N, sizeof(double));
+    orc_gas_from_stars(p, N, s->r, s->h, sim_stars(s), ga, s->gpot);
+    for (int i=0; i<3*N; i++) { s->a[i] += ga[i]; s->atree[i] += ga[i]; }
+    free(ga);
+  }
   return 0;
+}
+
+/* the N-body iteration of the step (SphSimulation.cpp:775-807): the stars' a, adot, gpot
+ * cleared, the gas's pull (self-gravitating runs), then the star-star sums */
+static void star_forces(orc_sim* s) {
+  const int ns = s->ns;
+  if (ns <= 0) return;
+  if (s->p.self_gravity) {
+    orc_stars_from_gas(&s->p, sim_stars(s), s->N, s->r, s->m, s->h, s->sa, s->sgpot);
+  } else {
+    memset(s->sa, 0, sizeof(double)*3*ns);
+    memset(s->sgpot, 0, sizeof(double)*ns);
+  }
+  memset(s->sadot, 0, sizeof(double)*3*ns);
+  double* a = calloc((size_t) 3*ns, sizeof(double));
+  double* ad = calloc((size_t) 3*ns, sizeof(double));
+  double* gp = calloc((size_t) ns, sizeof(double));
+  if (s->star_soft) orc_nbody_direct_smoothed(&s->p, ns, s->sr, s->sv, s->sm, s->sh, a, ad, gp);
+  else orc_nbody_direct(ns, s->sr, s->sv, s->sm, a, ad, gp);
+  for (int i=0; i<3*ns; i++) { s->sa[i] += a[i]; s->sadot[i] += ad[i]; }
+  for (int i=0; i<ns; i++) s->sgpot[i] += gp[i];
+  free(a); free(ad); free(gp);
+}
+
+/* NbodyLeapfrogKDK::Timestep (NbodyLeapfrogKDK.cpp:386-401), minimum over the stars */
+static double star_timestep(const orc_sim* s) {
+  double dt = 1.0e300;
+  for (int i=0; i<s->ns; i++) {
+    const double* a = &s->sa[3*i];
+    const double amag = sqrt(a[0]*a[0] + a[1]*a[1] + a[2]*a[2]);
+    dt = fmin(dt, s->nbody_mult*sqrt(s->sh[i]/(amag + 1.0e-50)));
+  }
+  return dt;
+}
+
+/* NbodyLeapfrogKDK::EndTimestep (:333-370) */
+static void star_end(orc_sim* s) {
+  memcpy(s->sr0, s->sr, sizeof(double)*3*s->ns);
+  memcpy(s->sv0, s->sv, sizeof(double)*3*s->ns);
+  memcpy(s->sa0, s->sa, sizeof(double)*3*s->ns);
 }
 
 static int ipow2(int e) { return (int) pow(2.0, (double) e); }
@@ -1620,19 +1785,22 @@ int orc_sim_setup(orc_sim* s) {
   const orc_params* p = &s->p;
   const double hg = orc_h_guess(p, N, s->r);
   for (int i=0; i<N; i++) s->h[i] = hg;
-  if (orc_compute_h(p, N, s->r, s->m, 1, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
-                    s->u, s->sound, s->pressure)) return -1;
-  if (orc_compute_h(p, N, s->r, s->m, 0, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
-                    s->u, s->sound, s->pressure)) return -1;
+  if (compute_h_masked(p, N, s->r, s->m, 1, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
+                       s->u, s->sound, s->pressure, NULL, sim_stars(s))) return -1;
+  if (compute_h_masked(p, N, s->r, s->m, 0, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
+                       s->u, s->sound, s->pressure, NULL, sim_stars(s))) return -1;
   s->t = 0.0;
   s->n = 0; s->nresync = 0;
   for (int i=0; i<N; i++) { s->level[i] = 0; s->levelneib[i] = 0; s->ptlast[i] = s->t; }
   if (sim_forces(s, NULL)) return -1;
+  star_forces(s);             /* the initial N-body forces (SphSimulation.cpp:490-502) */
+  star_end(s);
   if (p->nlevels > 1) {
     block_timesteps(s);
     end_timestep(s);          /* pdt = 0: sets r0, v0, a0, u0, dudt0 and the first dt */
   } else {
     s->timestep = orc_timesteps(p, N, s->h, s->sound, s->div_v, s->a, s->u, s->dudt, NULL);
+    if (s->ns > 0) s->timestep = fmin(s->timestep, star_timestep(s));
     memcpy(s->r0, s->r, sizeof(double)*3*N);
     memcpy(s->v0, s->v, sizeof(double)*3*N);
     memcpy(s->a0, s->a, sizeof(double)*3*N);
@@ -1692,12 +1860,21 @@ int orc_sim_step(orc_sim* s) {
   const double dt = s->t - s->tlast;
   orc_advance(N, dt, p->energy_integration, s->r0, s->v0, s->a0, s->u0, s->dudt0, s->r, s->v,
               s->u);
+  /* NbodyLeapfrogKDK::AdvanceParticles (NbodyLeapfrogKDK.cpp:253-289) */
+  for (int i=0; i<3*s->ns; i++) {
+    s->sr[i] = s->sr0[i] + s->sv0[i]*dt + 0.5*s->sa0[i]*dt*dt;
+    s->sv[i] = s->sv0[i] + s->sa0[i]*dt;
+  }
   wrap_with_r0(s);
   if (sim_forces(s, NULL)) return -1;
-  const double newstep = orc_timesteps(p, N, s->h, s->sound, s->div_v, s->a, s->u, s->dudt,
-                                       NULL);
+  star_forces(s);
+  /* NbodyLeapfrogKDK::CorrectionTerms (:296-326) */
+  for (int i=0; i<3*s->ns; i++) s->sv[i] += 0.5*(s->sa[i] - s->sa0[i])*dt;
+  double newstep = orc_timesteps(p, N, s->h, s->sound, s->div_v, s->a, s->u, s->dudt, NULL);
+  if (s->ns > 0) newstep = fmin(newstep, star_timestep(s));
   orc_end_step(N, dtstep, p->energy_integration, s->a, s->dudt, s->v, s->u, s->r0, s->v0,
                s->a0, s->u0, s->dudt0, s->r);
+  star_end(s);
   s->tlast = s->t;
   s->timestep = newstep;
   s->nfinished = N;
@@ -1715,6 +1892,14 @@ int orc_sim_get(orc_sim* s, const char* f, double* out) {
   for (unsigned k=0; k<sizeof(tab)/sizeof(tab[0]); k++)
     if (!strcmp(tab[k].name, f)) {
       memcpy(out, tab[k].p, sizeof(double)*N*tab[k].w);
+      return 0;
+    }
+  struct { const char* name; double* p; int w; } stab[] = {
+    {"star_r", s->sr, 3}, {"star_v", s->sv, 3}, {"star_a", s->sa, 3}, {"star_adot", s->sadot, 3},
+    {"star_gpot", s->sgpot, 1}};
+  for (unsigned k=0; k<sizeof(stab)/sizeof(stab[0]); k++)
+    if (!strcmp(stab[k].name, f)) {
+      if (s->ns > 0) memcpy(out, stab[k].p, sizeof(double)*s->ns*stab[k].w);
       return 0;
     }
   struct { const char* name; int* p; } itab[] = {

@@ -91,6 +91,26 @@ void orc_nbody_direct_smoothed(const orc_params* p, int N, const double* r, cons
                                const double* m, const double* h, double* a, double* adot,
                                double* gpot);
 
+/* ---- stars beside the gas (Nbody<3>::stardata of an SphSimulation) ----
+ * r (n,3), m, h n doubles; softening = nbody_softening, conservative =
+ * conservative_sph_star_gravity */
+typedef struct {
+  int n;
+  const double *r, *m, *h;
+  int softening, conservative;
+} orc_stars;
+/* orc_compute_h with the stars' zeta terms (GradhSph.cpp:282-307) */
+int orc_compute_h_stars(const orc_params* p, int N, const double* r, const double* m,
+                        int nopbc, const orc_stars* st, double* h, double* rho,
+                        double* invomega, double* zeta, double* hfactor, double* u,
+                        double* sound, double* pressure);
+/* GradhSph::ComputeStarGravForces for every gas particle: adds to a and gpot */
+void orc_gas_from_stars(const orc_params* p, int N, const double* r, const double* h,
+                        const orc_stars* st, double* a, double* gpot);
+/* HydroTree::UpdateAllStarGasForces through the brute-force search: writes the stars' a, gpot */
+void orc_stars_from_gas(const orc_params* p, const orc_stars* st, int N, const double* r,
+                        const double* m, const double* h, double* a, double* gpot);
+
 /* ---- CPU twin of the device tree (Morton keys, LBVH, bottom-up moments, groups) ----
  * Arrays are sized: perm[N], parent[2N-1], left/right/first/last[N-1] (internal nodes),
  * nodes' moments [2N-1] in the unified index (internal 0..N-2, leaf l -> N-1+l).
@@ -148,6 +168,10 @@ typedef struct orc_sim orc_sim;
 orc_sim* orc_sim_create(const orc_params* p, int N, const double* r, const double* v,
                         const double* m, const double* u);
 void orc_sim_destroy(orc_sim* s);
+/* stars beside the gas, leapfrog KDK (NbodyLeapfrogKDK), before orc_sim_setup; orc_sim_get
+ * then also answers star_r star_v star_a star_adot star_gpot */
+int orc_sim_set_stars(orc_sim* s, int n, const double* r, const double* v, const double* m,
+                      const double* h, int softening, int conservative, double nbody_mult);
 int orc_sim_setup(orc_sim* s);
 int orc_sim_step(orc_sim* s);
 /* field names: r v a atree m h rho u dudt div_v gpot invomega zeta sound pressure hfactor */

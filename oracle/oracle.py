@@ -49,6 +49,20 @@ class Tree(C.Structure):
     ]
 
 
+class Stars(C.Structure):
+    """orc_stars: stars beside the gas (positions (n,3), masses, smoothing lengths)."""
+    _fields_ = [("n", C.c_int), ("r", C.c_void_p), ("m", C.c_void_p), ("h", C.c_void_p),
+                ("softening", C.c_int), ("conservative", C.c_int)]
+
+
+def stars(r, m, h, softening=1, conservative=1):
+    """An orc_stars view of the arrays; the arrays are kept on the returned object."""
+    r, m, h = _d(r).reshape(-1, 3), _d(m), _d(h)
+    st = Stars(r.shape[0], _p(r), _p(m), _p(h), int(softening), int(conservative))
+    st._keep = (r, m, h)
+    return st
+
+
 def _d(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
@@ -96,6 +110,12 @@ def lib():
         L.orc_sim_time.argtypes = [vp]; L.orc_sim_time.restype = d
         L.orc_sim_timestep.argtypes = [vp]; L.orc_sim_timestep.restype = d
         L.orc_sim_block_state.argtypes = [vp, vp, vp]
+        L.orc_compute_h_stars.argtypes = [pp, i, vp, vp, i, C.POINTER(Stars)] + [vp] * 8
+        L.orc_compute_h_stars.restype = i
+        L.orc_gas_from_stars.argtypes = [pp, i, vp, vp, C.POINTER(Stars), vp, vp]
+        L.orc_stars_from_gas.argtypes = [pp, C.POINTER(Stars), i, vp, vp, vp, vp, vp]
+        L.orc_sim_set_stars.argtypes = [vp, i, vp, vp, vp, vp, i, i, d]
+        L.orc_sim_set_stars.restype = i
         _lib = L
     return _lib
 
@@ -167,6 +187,41 @@ def compute_h(p: Params, r, m, h, u, nopbc=False):
     out["h"] = h
     out["u"] = u
     return out
+
+
+def compute_h_stars(p: Params, r, m, h, u, st, nopbc=False):
+    """compute_h with the stars' zeta terms (GradhSph.cpp:282-307)."""
+    r, m = _d(r), _d(m)
+    N = r.shape[0]
+    h = _d(h).copy(); u = _d(u).copy()
+    out = {k: np.zeros(N) for k in ("rho", "invomega", "zeta", "hfactor", "sound", "pressure")}
+    rc = lib().orc_compute_h_stars(C.byref(p), N, _p(r), _p(m), int(nopbc), C.byref(st), _p(h),
+                                   _p(out["rho"]), _p(out["invomega"]), _p(out["zeta"]),
+                                   _p(out["hfactor"]), _p(u), _p(out["sound"]),
+                                   _p(out["pressure"]))
+    if rc != 0:
+        raise RuntimeError("oracle h-rho iteration failed")
+    out["h"] = h
+    out["u"] = u
+    return out
+
+
+def gas_from_stars(p: Params, r, h, st):
+    """GradhSph::ComputeStarGravForces for every gas particle: (a, gpot) of the stars' pull."""
+    r, h = _d(r), _d(h)
+    N = r.shape[0]
+    a = np.zeros((N, 3)); gp = np.zeros(N)
+    lib().orc_gas_from_stars(C.byref(p), N, _p(r), _p(h), C.byref(st), _p(a), _p(gp))
+    return a, gp
+
+
+def stars_from_gas(p: Params, st, r, m, h):
+    """HydroTree::UpdateAllStarGasForces (brute-force lists): (a, gpot) of every star."""
+    r, m, h = _d(r), _d(m), _d(h)
+    a = np.zeros((st.n, 3)); gp = np.zeros(st.n)
+    lib().orc_stars_from_gas(C.byref(p), C.byref(st), r.shape[0], _p(r), _p(m), _p(h), _p(a),
+                             _p(gp))
+    return a, gp
 
 
 def hydro_forces(p: Params, variant, r, v, m, st):
@@ -320,12 +375,20 @@ class Sim:
         if lib().orc_sim_step(self._s) != 0:
             raise RuntimeError("oracle step failed")
 
+    def set_stars(self, r, v, m, h, softening=1, conservative=1, nbody_mult=0.1):
+        r, v, m, h = _d(r).reshape(-1, 3), _d(v).reshape(-1, 3), _d(m), _d(h)
+        self.Nstar = r.shape[0]
+        lib().orc_sim_set_stars(self._s, self.Nstar, _p(r), _p(v), _p(m), _p(h), int(softening),
+                                int(conservative), float(nbody_mult))
+
     def get(self, name):
-        w = 3 if name in self.FIELDS3 else 1
-        out = np.zeros(self.N * w)
+        n = getattr(self, "Nstar", 0) if name.startswith("star_") else self.N
+        w = 3 if name in self.FIELDS3 or name in ("star_r", "star_v", "star_a", "star_adot") \
+            else 1
+        out = np.zeros(max(n * w, 1))[: n * w]
         if lib().orc_sim_get(self._s, name.encode(), _p(out)) != 0:
             raise KeyError(name)
-        return out.reshape(self.N, 3) if w == 3 else out
+        return out.reshape(n, 3) if w == 3 else out
 
     @property
     def t(self):

@@ -39,6 +39,23 @@ def _load(fn):
     return {FIELDS[k]: a[k] for k in range(nf)}
 
 
+def _hybrid(N, Nstar, seed, softening):
+    """Gas and stars together (SphSimulation with an N-body integrator): the cold sphere's gas
+    at half the mass plus seeded stars inside it carrying the other half, through the reference's
+    brute-force neighbour search (exact gas gravity and exact star-gas sums), leapfrog KDK for
+    the stars, star-star sums unsoftened (nbody_softening = 0) or kernel-smoothed (1); the
+    conservative SPH-star gravity terms on (the reference default)."""
+    g = ic.cold_sphere(N=N, seed=seed)
+    g["m"] = g["m"] * 0.5
+    rng = np.random.default_rng(seed + 500)
+    rs = ic.uniform_sphere(Nstar, seed + 501, radius=0.8)
+    g["stars"] = {"r": rs, "v": 0.05 * rng.normal(size=(Nstar, 3)),
+                  "m": np.full(Nstar, 0.5 / Nstar), "h": 0.04 + 0.04 * rng.random(Nstar)}
+    g["params"] = dict(g["params"], nbody="lfkdk", nbody_softening=softening, nbody_mult=0.1,
+                       neib_search="bruteforce", gas_eos="energy_eqn")
+    return g
+
+
 CASES = {
     # hydro only, periodic jittered lattice, analytic M4
     "box16_m4": (lambda: ic.jittered_lattice_box(n_side=16, seed=11), {}, 10),
@@ -70,6 +87,9 @@ CASES = {
     "box16_m4_e100": (lambda: ic.jittered_lattice_box(n_side=16, seed=19), {}, 100),
     "sphere4k_m4_kd_e100": (lambda: ic.cold_sphere(N=4096, seed=20),
                             {"gas_eos": "energy_eqn"}, 100),
+    # gas and stars: the star-gas coupling, 3 steps
+    "hybrid1k_soft": (lambda: _hybrid(1024, 16, 22, 1), {}, 3),
+    "hybrid1k_hard": (lambda: _hybrid(1024, 16, 23, 0), {}, 3),
     # pure N-body: unsoftened and kernel-smoothed star-star direct sums, 5 KDK steps
     "plummer512_direct": (lambda: _plummer_stars(512, 17, 0), {}, 5),
     "plummer512_smoothed": (lambda: _plummer_stars(512, 18, 1), {}, 5),
@@ -102,8 +122,9 @@ def run_case(name, gen, over, nsteps, outdir):
     with tempfile.TemporaryDirectory() as td:
         col = os.path.join(td, "in.col")
         dat = os.path.join(td, "in.dat")
-        stars = g.get("stars", False)
-        ic.write_column_file(col, g, stars=stars, h=g.get("h"))
+        stars = g.get("stars", False) is True
+        hyb = g["stars"] if isinstance(g.get("stars"), dict) else None
+        ic.write_column_file(col, g, stars=stars, h=g.get("h"), star_ic=hyb)
         ic.write_param_file(dat, prm, col, "FIX")
         steps = max(nsteps, 0)
         subprocess.run([REF, dat, str(steps), os.path.join(td, "out"), str(max(steps, 1))],
@@ -141,6 +162,16 @@ def run_case(name, gen, over, nsteps, outdir):
                 arrs[f"s{nsteps}_su"] = np.frombuffer(f.read(), dtype=np.uint8)
             with open(os.path.join(td, f"out_s{nsteps}.sf"), "rb") as f:
                 arrs[f"s{nsteps}_sf"] = np.frombuffer(f.read(), dtype=np.uint8)
+        if hyb is not None:
+            for k, a in hyb.items():
+                arrs[f"star_in_{k}"] = a
+            for s in sorted({0, nsteps}):
+                b = open(os.path.join(td, f"out_stars_s{s}.bin"), "rb").read()
+                n, nf = np.frombuffer(b[:8], np.int32)
+                a = np.frombuffer(b[8:], np.float64).reshape(nf, n)
+                for k in range(nf):
+                    if STAR_FIELDS[k] not in ("m", "h"):
+                        arrs[f"star_s{s}_{STAR_FIELDS[k]}"] = a[k].copy()
         diag = np.loadtxt(os.path.join(td, "out_diag.txt"))
         arrs["diag"] = np.atleast_2d(diag)
     _save(name, g, prm, nsteps, arrs, outdir)
