@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = [
     "gdf_capi.hip", "gdf_tree.hip", "gdf_sph.hip", "gdf_gravity.hip", "gdf_integrate.hip",
     "gdf_pool.hip", "gdf_methods.hip", "gdf_stars.hip",
-    "host/gradhsph_simulation_gpu.cpp", "host/snapshot_io.cpp",
+    "host/gradhsph_simulation_gpu.cpp", "host/snapshot_io.cpp", "host/ewald_table.cpp",
 ]
 
 FLAGS = [

@@ -145,6 +145,24 @@ int alloc_particles(gdf_ctx* c, int N) {
   return GDF_OK;
 }
 
+int upload_ewald_table(gdf_ctx* c) {
+  PhysParams& p = c->phys;
+  std::vector<double> field;
+  const int rc = ewald_build_table(p.boxsize, c->cfg.gr_bhewaldseriesn, c->cfg.n_ewald_grid,
+                                   c->cfg.ewald_mult, c->cfg.efratio, field, p.eng, p.edI);
+  if (rc != GDF_OK) return fail(c, rc, "Ewald table: bad parameters");
+  const size_t n = field.size()/4;
+  std::vector<float> ff(field.size());
+  for (size_t i = 0; i < field.size(); i++) ff[i] = (float) field[i];
+  GDF_HIP(c, hipMalloc(&c->ewald_tab, sizeof(double4)*n));
+  GDF_HIP(c, hipMalloc(&c->ewald_tabf, sizeof(float4)*n));
+  GDF_HIP(c, hipMemcpy(c->ewald_tab, field.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
+  GDF_HIP(c, hipMemcpy(c->ewald_tabf, ff.data(), sizeof(float4)*n, hipMemcpyHostToDevice));
+  p.etab = c->ewald_tab;
+  p.etabf = c->ewald_tabf;
+  return GDF_OK;
+}
+
 int upload_kernel_tables(gdf_ctx* c) {
   // SmoothingKernel.h:583-604 tables: 6 linear in s, 3 in s^2; values from the oracle-free
   // host evaluation of the same polynomials (evaluated here in double)
@@ -291,6 +309,8 @@ int gdf_config_default(gdf_config* cfg) {
   cfg->nbody_mult = 0.1; cfg->fixed_dt = 0.0;
   cfg->Nlevels = 1; cfg->level_diff_max = 1;
   cfg->nbody_softening = 1; cfg->conservative_sph_star_gravity = 1;
+  cfg->n_ewald_grid = 16; cfg->gr_bhewaldseriesn = 10; cfg->ewald_mult = 1.0;
+  cfg->efratio = 1.0;
   return GDF_OK;
 }
 
@@ -310,6 +330,11 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
       cfg->cand_batches_init < 0 || cfg->cand_batches_init > 4096 || !(cfg->h_converge >= 0.0))
     return GDF_ERR_INVALID;
   if (cfg->Nlevels > 1 && (cfg->nbody || cfg->fixed_dt > 0.0)) return GDF_ERR_INVALID;
+  if (cfg->periodic && cfg->self_gravity && !cfg->nbody &&
+      (cfg->n_ewald_grid < 4 || cfg->n_ewald_grid > 256 || cfg->gr_bhewaldseriesn < 1 ||
+       cfg->gr_bhewaldseriesn > 64 || !(cfg->ewald_mult > 0.0) || !(cfg->efratio >= 1.0) ||
+       !(cfg->efratio <= 1.5)))
+    return GDF_ERR_INVALID;
   if ((cfg->nbody_softening != 0 && cfg->nbody_softening != 1) ||
       (cfg->conservative_sph_star_gravity != 0 && cfg->conservative_sph_star_gravity != 1))
     return GDF_ERR_INVALID;
@@ -352,6 +377,9 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
   p.hrho_itmax = cfg->hrho_iteration_max > 0 ? cfg->hrho_iteration_max : 30;
   c->nlevels = cfg->Nlevels; c->level_diff_max = cfg->level_diff_max;
   int rc = upload_kernel_tables(c);
+  // a periodic box with self-gravity carries the Ewald correction (Simulation.cpp:1183-1198)
+  p.ewald = cfg->periodic && cfg->self_gravity && !cfg->nbody;
+  if (rc == GDF_OK && p.ewald) rc = upload_ewald_table(c);
   if (rc == GDF_OK) {
     if (hipMalloc(&c->counters, sizeof(int)*32) != hipSuccess ||
         hipMalloc(&c->icount, sizeof(long long)*8) != hipSuccess ||
@@ -385,6 +413,8 @@ void gdf_destroy(gdf_ctx* c) {
   if (c->red) hipFree(c->red);
   if (c->ktab) hipFree(c->ktab);
   if (c->ktabf) hipFree(c->ktabf);
+  if (c->ewald_tab) hipFree(c->ewald_tab);
+  if (c->ewald_tabf) hipFree(c->ewald_tabf);
   void* segp[] = {c->d_seg_off, c->seg_root, c->seg_dt, c->seg_bnd};
   for (void* q : segp) if (q) hipFree(q);
   if (c->h_counters) hipHostFree(c->h_counters);

@@ -290,6 +290,64 @@ __device__ __forceinline__ void nearest_image(const PhysParams& p, T& dx, T& dy,
   if (dz > T(p.boxhalf[2])) dz -= T(p.boxsize[2]); else if (dz < -T(p.boxhalf[2])) dz += T(p.boxsize[2]);
 }
 
+// Ewald::CalculatePeriodicCorrection (src/Common/Ewald.cpp:500-650) for a fully periodic box:
+// the periodic correction of mass m at separation d = r_source - r_target.  A separation outside
+// the half box first swaps its Newtonian term for that of its nearest image (the reference's
+// repair of a non-nearest separation); the acceleration is trilinear in the table, the
+// potential the value at the lower corner node, signs restored per axis.
+template <typename T>
+__device__ __forceinline__ void ewald_corr(const PhysParams& p, T dx, T dy, T dz, T m, T& ax,
+                                           T& ay, T& az, T& pot) {
+  T d[3] = {dx, dy, dz};
+  const T half[3] = {T(p.boxhalf[0]), T(p.boxhalf[1]), T(p.boxhalf[2])};
+  if (fabs(d[0]) > half[0] || fabs(d[1]) > half[1] || fabs(d[2]) > half[2]) {
+    T inv = T(1)/sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2] + T(1.0e-20));
+    T inv3 = inv*inv*inv;
+    ax -= m*d[0]*inv3; ay -= m*d[1]*inv3; az -= m*d[2]*inv3;
+    pot -= m*inv;
+    for (int k = 0; k < 3; k++) {
+      if (d[k] > half[k]) d[k] -= T(2)*half[k];
+      else if (d[k] < -half[k]) d[k] += T(2)*half[k];
+    }
+    inv = T(1)/sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2] + T(1.0e-20));
+    inv3 = inv*inv*inv;
+    ax += m*d[0]*inv3; ay += m*d[1]*inv3; az += m*d[2]*inv3;
+    pot += m*inv;
+  }
+  int il[3];
+  T w[3];
+  for (int k = 0; k < 3; k++) {
+    const T b = T(p.edI[k])*fabs(d[k]);
+    // |d| <= L/2 keeps b in [0, eng - 2] already; the clamp keeps any input in the table
+    il[k] = max(0, min((int) b, p.eng[k] - 2));
+    w[k] = b - (T) il[k];
+  }
+  const int n0 = p.eng[0], n01 = p.eng[0]*p.eng[1];
+  const int b0 = il[0] + il[1]*n0 + il[2]*n01;
+  T g0 = T(0), g1 = T(0), g2 = T(0), p0;
+  auto corner = [&](int off, T wt) {
+    if (sizeof(T) == 4) {
+      const float4 t = p.etabf[b0 + off];
+      g0 += wt*(T) t.y; g1 += wt*(T) t.z; g2 += wt*(T) t.w;
+    } else {
+      const double4 t = p.etab[b0 + off];
+      g0 += wt*(T) t.y; g1 += wt*(T) t.z; g2 += wt*(T) t.w;
+    }
+  };
+  const T u0 = T(1) - w[0], u1 = T(1) - w[1], u2 = T(1) - w[2];
+  corner(0, u2*u1*u0);             corner(1, u2*u1*w[0]);
+  corner(n0, u2*w[1]*u0);          corner(n0 + 1, u2*w[1]*w[0]);
+  corner(n01, w[2]*u1*u0);         corner(n01 + 1, w[2]*u1*w[0]);
+  corner(n01 + n0, w[2]*w[1]*u0);  corner(n01 + n0 + 1, w[2]*w[1]*w[0]);
+  if (sizeof(T) == 4) p0 = (T) p.etabf[b0].x;
+  else p0 = (T) p.etab[b0].x;
+  pot += m*p0;
+  const T sx = d[0] > T(0) ? T(1) : (d[0] < T(0) ? T(-1) : T(0));
+  const T sy = d[1] > T(0) ? T(1) : (d[1] < T(0) ? T(-1) : T(0));
+  const T sz = d[2] > T(0) ? T(1) : (d[2] < T(0) ? T(-1) : T(0));
+  ax += m*g0*sx; ay += m*g1*sy; az += m*g2*sz;
+}
+
 // squared distance with the reference's operation order and no fused multiply-add, so
 // neighbour-set membership is decided exactly as the oracle decides it.  (HIP's __dadd_rn and
 // __dmul_rn are plain operators that the compiler may still contract; the pragma may not.)

@@ -131,8 +131,11 @@ __device__ __forceinline__ void quad_term(T dx, T dy, T dz, T mc, const T* q, T&
 // (same squared distance, same range test, so each pair lands in one of the two sums) and adds
 // the direct and multipole terms.  PLUMMER (pure N-body stars): every particle pair and every
 // multipole term is softened with the Plummer length eps, the star itself is skipped and
-// there is no self potential.
-template <bool MIXED, bool SOFT, bool PLUMMER>
+// there is no self potential.  EWALD (periodic box with self-gravity): every particle of the
+// walk's buckets (the hydro pass's neighbours and the target itself included) and every accepted
+// cell adds its periodic correction at its separation, as GradhSphTree::UpdateAllSphForces does
+// for all its neighbours and cells (GradhSphTree.cpp:558-578).
+template <bool MIXED, bool SOFT, bool PLUMMER, bool EWALD = false>
 __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
   __shared__ int pbuf[PCAP];
   __shared__ int cbuf[CCAP];
@@ -278,6 +281,8 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
               const float qq[5] = {qa.x, qa.y, qa.z, qa.w, s1[q]};
               quad_term<float>(rxi - c.x, ryi - c.y, rzi - c.z, c.w, qq, bx, by, bz, bp,
                                PLUMMER ? eps2f : 1.0e-20f);
+              if (EWALD) ewald_corr<float>(a.p, c.x - rxi, c.y - ryi, c.z - rzi, c.w, bx, by, bz,
+                                           bp);
             }
             fx.add(bx); fy.add(by); fz.add(bz); fp.add(bp);
           } else {
@@ -285,6 +290,8 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
               const double qq[5] = {sd[4][q], sd[5][q], sd[6][q], sd[7][q], sd[8][q]};
               quad_term<double>(me.x - sd[0][q], me.y - sd[1][q], me.z - sd[2][q], sd[3][q], qq,
                                 ax, ay, az, pot, PLUMMER ? eps2 : 1.0e-20);
+              if (EWALD) ewald_corr<double>(a.p, sd[0][q] - me.x, sd[1][q] - me.y,
+                                            sd[2][q] - me.z, sd[3][q], ax, ay, az, pot);
             }
           }
         }
@@ -339,6 +346,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
                 const float minv3 = mj*inv*inv*inv;
                 bx += minv3*dx; by += minv3*dy; bz += minv3*dz; bp += mj*inv;
               }
+              if (EWALD) ewald_corr<float>(a.p, dx, dy, dz, s.w, bx, by, bz, bp);
             }
             fx.add(bx); fy.add(by); fz.add(bz); fp.add(bp);
           } else {
@@ -361,6 +369,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
                 const double minv3 = mj*inv*inv*inv;
                 ax += minv3*dx; ay += minv3*dy; az += minv3*dz; pot += mj*inv;
               }
+              if (EWALD) ewald_corr<double>(a.p, dx, dy, dz, sd[3][q], ax, ay, az, pot);
             }
           }
         }
@@ -612,8 +621,14 @@ int gravity_launch(gdf_ctx* c) {
   // and the scatter-gather walk beside them waits for the gravity walk to drain
   const int grid = std::min(c->ngroups, kGravSlots/2);
   const int kt = ktime_begin(c, s2, 5);
-  if (c->cfg.precision == GDF_PREC_MIXED) k_gravity<true, false, false><<<grid, kWave, 0, s2>>>(a);
-  else k_gravity<false, false, false><<<grid, kWave, 0, s2>>>(a);
+  const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
+  if (c->phys.ewald) {
+    if (mixed) k_gravity<true, false, false, true><<<grid, kWave, 0, s2>>>(a);
+    else k_gravity<false, false, false, true><<<grid, kWave, 0, s2>>>(a);
+  } else {
+    if (mixed) k_gravity<true, false, false><<<grid, kWave, 0, s2>>>(a);
+    else k_gravity<false, false, false><<<grid, kWave, 0, s2>>>(a);
+  }
   GDF_HIP(c, hipGetLastError());
   ktime_end(c, s2, kt);
   GDF_HIP(c, hipMemcpyAsync(c->h_counters + 16, c->counters + 16, sizeof(int)*8,
@@ -657,8 +672,16 @@ int gravity_walk(gdf_ctx* c) {
     if (mixed) k_gravity<true, true, true><<<grid, kWave, 0, s>>>(a);
     else k_gravity<false, true, true><<<grid, kWave, 0, s>>>(a);
   } else if (c->phys.hydro_forces) {
-    if (mixed) k_gravity<true, false, false><<<grid, kWave, 0, s>>>(a);
-    else k_gravity<false, false, false><<<grid, kWave, 0, s>>>(a);
+    if (c->phys.ewald) {
+      if (mixed) k_gravity<true, false, false, true><<<grid, kWave, 0, s>>>(a);
+      else k_gravity<false, false, false, true><<<grid, kWave, 0, s>>>(a);
+    } else {
+      if (mixed) k_gravity<true, false, false><<<grid, kWave, 0, s>>>(a);
+      else k_gravity<false, false, false><<<grid, kWave, 0, s>>>(a);
+    }
+  } else if (c->phys.ewald) {
+    if (mixed) k_gravity<true, true, false, true><<<grid, kWave, 0, s>>>(a);
+    else k_gravity<false, true, false, true><<<grid, kWave, 0, s>>>(a);
   } else {
     if (mixed) k_gravity<true, true, false><<<grid, kWave, 0, s>>>(a);
     else k_gravity<false, true, false><<<grid, kWave, 0, s>>>(a);

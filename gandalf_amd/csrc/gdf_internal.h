@@ -82,6 +82,12 @@ struct PhysParams {
   double eps2;               // Plummer softening length squared of the N-body tree walk
   double nbody_mult;
   int hrho_itmax;            // fixed-point iterations before bisection (reference: 30)
+  // Ewald periodic gravity (periodic box with self-gravity): correction table of eng[0] x
+  // eng[1] x eng[2] nodes, x fastest, each (pot, ax, ay, az) of a unit mass; node spacing 1/edI
+  int ewald;
+  int eng[3];
+  double edI[3];
+  const double4* etab; const float4* etabf;
 };
 
 // Tree node arrays in the unified index (internal 0..N-2, leaf l -> N-1+l).
@@ -210,6 +216,10 @@ struct gdf_ctx {
   double snap_tlast = 0.0, snap_tlitelast = 0.0;
   std::vector<int> snap_porig;
 
+  // Ewald correction table (device copies, FP64 and FP32)
+  double4* ewald_tab = nullptr;
+  float4* ewald_tabf = nullptr;
+
   // stars beside the gas (gdf_set_stars): star-gas gravity both ways, star-star sums, the
   // stars' leapfrog KDK (SphSimulation::MainLoop's N-body half)
   int Nstar = 0, Nstar_cap = 0;
@@ -306,6 +316,11 @@ int stars_correction(gdf_ctx* c, double t);
 int stars_end_step(gdf_ctx* c, double t);
 int stars_timestep(gdf_ctx* c, double* dtmin);
 int stars_diagnostics(gdf_ctx* c, double* acc11);   // adds the stars' sums to acc
+
+// Ewald::Ewald's correction field of a fully periodic box (host/ewald_table.cpp): 4 doubles
+// (pot, ax, ay, az) per node, Ng nodes per axis, dI nodes per unit length
+int ewald_build_table(const double L[3], int nseries, int ngrid, double ewald_mult,
+                      double efratio, std::vector<double>& field, int Ng[3], double dI[3]);
 
 inline int nblocks(long long n, int bs) { return (int) ((n + bs - 1) / bs); }
 

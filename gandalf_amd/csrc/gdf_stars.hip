@@ -368,6 +368,8 @@ int gdf_set_stars(gdf_ctx* c, int nstar, const double* r, const double* v, const
     return c ? fail(c, GDF_ERR_INVALID, "gdf_set_stars: bad arguments") : GDF_ERR_INVALID;
   if (c->phys.nbody) return fail(c, GDF_ERR_INVALID, "gdf_set_stars: an N-body context holds "
                                                      "its stars as its particles");
+  if (c->phys.ewald && nstar > 0)
+    return fail(c, GDF_ERR_INVALID, "stars beside the gas: periodic gravity is not built");
   if (nstar > (1 << 20)) return fail(c, GDF_ERR_INVALID, "gdf_set_stars: too many stars");
   for (int i = 0; i < nstar; i++)
     if (!(m[i] > 0.0) || !(h[i] > 0.0))

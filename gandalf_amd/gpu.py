@@ -44,6 +44,8 @@ class Config(C.Structure):
         ("hrho_iteration_max", C.c_int), ("cand_batches_init", C.c_int),
         ("restart", C.c_int),
         ("nbody_softening", C.c_int), ("conservative_sph_star_gravity", C.c_int),
+        ("n_ewald_grid", C.c_int), ("gr_bhewaldseriesn", C.c_int),
+        ("ewald_mult", C.c_double), ("efratio", C.c_double),
     ]
 
 
@@ -164,6 +166,10 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
     c.nbody_eps = float(prm.get("nbody_eps", 0.0))
     c.nbody_softening = int(prm.get("nbody_softening", 1))
     c.conservative_sph_star_gravity = int(prm.get("conservative_sph_star_gravity", 1))
+    c.n_ewald_grid = int(prm.get("nEwaldGrid", 16))
+    c.gr_bhewaldseriesn = int(prm.get("gr_bhewaldseriesn", 10))
+    c.ewald_mult = float(prm.get("ewald_mult", 1.0))
+    c.efratio = float(prm.get("EFratio", 1.0))
     if c.nbody and c.nbody_eps <= 0.0:
         c.nbody_direct = 2 if c.nbody_softening else 1
     c.nbody_mult = float(prm.get("nbody_mult", 0.1))

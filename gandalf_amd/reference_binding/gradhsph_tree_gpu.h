@@ -212,11 +212,17 @@ class GradhSphTreeGpu : public SphNeighbourSearch<ndim> {
     check(dev->ctx, gdf_update_all_sph_hydro_forces(dev->ctx));
     DownloadForces(sph, nbody && nbody->Nstar > 0);
   }
-  // GradhSphTree::UpdateAllSphForces (GradhSphTree.cpp:444-657); no Ewald (periodic gravity)
+  // GradhSphTree::UpdateAllSphForces (GradhSphTree.cpp:444-657); periodic gravity with the
+  // device's own Ewald table when all three axes are periodic (the device context must be
+  // created with periodic = 1 and self_gravity = 1)
   virtual void UpdateAllSphForces(Sph<ndim>* sph, Nbody<ndim>* nbody, DomainBox<ndim>& simbox,
                                   Ewald<ndim>*) {
     if (simbox.PeriodicGravity)
-      ExceptionHandler::getIstance().raise("neib_search = gpu: periodic gravity is not built");
+      for (int k = 0; k < 3; k++)
+        if (simbox.boundary_lhs[k] != periodicBoundary ||
+            simbox.boundary_rhs[k] != periodicBoundary)
+          ExceptionHandler::getIstance().raise("neib_search = gpu: periodic gravity needs all "
+                                               "three axes periodic");
     dev->UploadStars(nbody);
     check(dev->ctx, gdf_update_all_sph_forces(dev->ctx));
     DownloadForces(sph, true);

@@ -20,8 +20,12 @@ import numpy as np
 from . import ic as ICS
 from .gpu import GpuPool, GpuSim
 
-# device bytes per resident particle (SoA state, permutation targets, tree, packed records,
-# candidate sets): the admission budget divided by this gives a pool's particle limit
+# device bytes per resident particle, rounded up from the pool context's allocations per
+# particle of capacity: 23 persistent FP64 fields and their permutation targets (368 B), 25
+# derived fields (200 B), integer state and ids (64 B), packed records and export scratch
+# (96 B), tree arrays over 2N nodes (~400 B), and the two candidate sets at their initial 16
+# batches of 64 per walk group (~512 B): ~1.65 KB; the admission budget divided by this gives
+# a pool's particle limit
 BYTES_PER_PARTICLE = 2048
 
 CLASSES = ("box", "sphere", "plummer")

@@ -77,6 +77,14 @@ typedef struct gdf_config {
    * terms of the stars in the gas h-rho pass, GradhSph.cpp:282-307) */
   int nbody_softening;
   int conservative_sph_star_gravity;
+  /* Ewald periodic gravity, on whenever the box is periodic and self_gravity is set
+   * (Simulation.cpp:1183-1198): "nEwaldGrid" table nodes per axis, "gr_bhewaldseriesn"
+   * image / Fourier series radius, "ewald_mult" splitting factor, "EFratio" table extent in
+   * half boxes (1 <= EFratio <= 1.5); Parameters.cpp:387-396 */
+  int n_ewald_grid;
+  int gr_bhewaldseriesn;
+  double ewald_mult;
+  double efratio;
 } gdf_config;
 
 typedef struct gdf_ctx gdf_ctx;
@@ -277,8 +285,9 @@ int gdf_nbody_end_timestep(gdf_ctx* ctx, double t);
  * Independent requests of one configuration class (one gdf_config) share a device context:
  * each is set up alone, then joins a batch whose steps advance every resident request by one
  * of its own global timesteps with one launch of each pass over all of them.  max_particles
- * bounds the resident particles (the admission limit on HBM footprint; see
- * gdf_pool_bytes_per_particle).  Requests need Nlevels = 1 and no direct N-body sums. */
+ * bounds the resident particles, the admission limit on HBM footprint (about 1.7 KB of device
+ * state per particle of capacity, plus 128 MiB of walk stacks per pool).  Requests need
+ * Nlevels = 1 and no direct N-body sums. */
 typedef struct gdf_pool gdf_pool;
 int gdf_pool_create(const gdf_config* cfg, int64_t max_particles, gdf_pool** out);
 void gdf_pool_destroy(gdf_pool* pool);

@@ -19,6 +19,7 @@
 
 /* Constants.h:61-90 (the reference's truncated literals are part of its results) */
 #define ORC_PI        3.14159265358979
+#define ORC_TWOPI     6.28318530717959   /* Constants.h:62 */
 #define ORC_INVPI     0.31830988618379
 #define ORC_ONETHIRD  0.33333333333333333333333
 #define SMALL_NUMBER  1.0e-20
@@ -918,6 +919,181 @@ void orc_direct_gravity(const orc_params* p, int N, const double* r, const doubl
   direct_gravity_masked(p, N, r, m, h, zeta, hfactor, atree, gpot, NULL);
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Ewald periodic gravity                                                                      */
+/* ------------------------------------------------------------------------------------------ */
+/* Ewald::AccShort (Ewald.cpp:841-850) */
+static double ewald_acc_short(double alpha, double proj, double r) {
+  const double r2 = pow(r, 2);
+  const double s1 = 2.0*alpha*proj*exp(-r2*pow(alpha, 2))/(sqrt(ORC_PI)*r2);
+  const double s2 = proj*erfc(alpha*r)/pow(r, 3);
+  return s1 + s2;
+}
+
+/* Ewald::AccLong3pPer (Ewald.cpp:1005-1013) */
+static double ewald_acc_long(int l, double linv_acc, double dzeta, double c3, double c4) {
+  return l*linv_acc*exp(-dzeta*c3)*sin(c4)/c3;
+}
+
+/* Ewald::Ewald (Ewald.cpp:42-420), fully periodic box (ewald_periodicity = 7): the correction
+ * field, component-major as the reference stores it (potential, then a_x, a_y, a_z, each
+ * Ng[0]*Ng[1]*Ng[2] nodes with x fastest) */
+int orc_ewald_table(const double L[3], int nseries, int ngrid, double ewald_mult,
+                    double efratio, double* field, int Ng[3], double dI[3]) {
+  const double lx = L[0], ly = L[1], lz = L[2];
+  const double linv = 1.0/lx;
+  const double ratio_p1 = ly/lx, ratio_p2 = lz/lx;
+  const int es_nrx = nseries, es_nry = (int) (1 + nseries/ratio_p1);
+  const int es_nrz = (int) (1 + nseries/ratio_p2);
+  const int es_nfx = nseries, es_nfy = (int) (1 + nseries*ratio_p1);
+  const int es_nfz = (int) (1 + nseries*ratio_p2);
+  const int es_radius2 = nseries*nseries;
+  const double cr1 = 1.0, cr2 = pow(ratio_p1, 2), cr3 = pow(ratio_p2, 2);
+  const double cf1 = 1.0, cf2 = 1.0/pow(ratio_p1, 2), cf3 = 1.0/pow(ratio_p2, 2);
+  const double ratio_pinv1 = 1.0/ratio_p1, ratio_pinv2 = 1.0/ratio_p2;
+  const double linv2 = linv*ratio_pinv1*ratio_pinv2/ORC_PI;
+  const double linv_acc = 2.0*pow(linv, 2)*ratio_pinv1*ratio_pinv2;
+  const double alpha = 2.0*linv*ewald_mult;
+  const double dzeta = ORC_PI*ORC_PI*linv*linv/(alpha*alpha);
+  double Lewald[3];
+  int Ncells[3];
+  for (int k=0; k<3; k++) {
+    Lewald[k] = efratio*0.5*L[k];
+    Ncells[k] = ngrid - 1;
+    dI[k] = (Ncells[k] - 1)/Lewald[k];
+    Ng[k] = Ncells[k] + 1;
+  }
+  if (!field) return 0;
+  const int one = Ng[0]*Ng[1]*Ng[2];
+  for (int i=0; i<4*one; i++) field[i] = 0.0;
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int k=0; k<Ng[2]; k++) {
+    for (int j=0; j<Ng[1]; j++) {
+      for (int i=0; i<Ng[0]; i++) {
+        const int e = k*Ng[1]*Ng[0] + j*Ng[0] + i;
+        const double x = i*Lewald[0]/(Ncells[0] - 1);
+        const double y = j*Lewald[1]/(Ncells[1] - 1);
+        const double z = k*Lewald[2]/(Ncells[2] - 1);
+        for (int ni=-es_nrx; ni<=es_nrx; ni++)
+          for (int nj=-es_nry; nj<=es_nry; nj++)
+            for (int nk=-es_nrz; nk<=es_nrz; nk++) {
+              if ((cr1*ni*ni + cr2*nj*nj + cr3*nk*nk) <= es_radius2) {
+                double rrel[4];
+                rrel[1] = x + ni*lx; rrel[2] = y + nj*ly; rrel[3] = z + nk*lz;
+                const double rni2 = pow(rrel[1], 2) + pow(rrel[2], 2) + pow(rrel[3], 2);
+                const double rni = sqrt(rni2);
+                field[e] = field[e] + erfc(alpha*rni)/rni;
+                for (int jj=1; jj<4; jj++) field[e + jj*one] += ewald_acc_short(alpha, rrel[jj], rni);
+              }
+            }
+        for (int hi=-es_nfx; hi<=es_nfx; hi++)
+          for (int hj=-es_nfy; hj<=es_nfy; hj++)
+            for (int hk=-es_nfz; hk<=es_nfz; hk++) {
+              if ((cf1*hi*hi + cf2*hj*hj + cf3*hk*hk) <= es_radius2 &&
+                  (pow(hi, 2) + pow(hj, 2) + pow(hk, 2)) > 0) {
+                const double c3 = pow(hi, 2) + pow(hj*ratio_pinv1, 2) + pow(hk*ratio_pinv2, 2);
+                const double c4 = ORC_TWOPI*linv*(hi*x + hj*y*ratio_pinv1 + hk*z*ratio_pinv2);
+                field[e] += linv2*exp(-dzeta*c3)*cos(c4)/c3;
+                field[e + one] += ewald_acc_long(hi, linv_acc, dzeta, c3, c4);
+                field[e + 2*one] += ratio_pinv1*(ewald_acc_long(hj, linv_acc, dzeta, c3, c4));
+                field[e + 3*one] += ratio_pinv2*(ewald_acc_long(hk, linv_acc, dzeta, c3, c4));
+              }
+            }
+        const double rni = sqrt(pow(x, 2) + pow(y, 2) + pow(z, 2));
+        field[e] -= 1.0/rni;
+        field[e + one] -= x/pow(rni, 3);
+        field[e + 2*one] -= y/pow(rni, 3);
+        field[e + 3*one] -= z/pow(rni, 3);
+      }
+    }
+  }
+  field[0] = 0.0; field[one] = 0.0; field[2*one] = 0.0; field[3*one] = 0.0;
+  return 0;
+}
+
+static inline double sgn_d(double v) { return (double) ((0.0 < v) - (v < 0.0)); }
+
+/* Ewald::CalculatePeriodicCorrection (Ewald.cpp:500-650), fully periodic box */
+void orc_ewald_corr(const orc_ewald* e, double m, const double* dr_in, double* acorr,
+                    double* gpotcorr) {
+  double dr[3] = {dr_in[0], dr_in[1], dr_in[2]};
+  const double lper[3] = {e->L[0], e->L[1], e->L[2]};
+  acorr[0] = acorr[1] = acorr[2] = 0.0;
+  *gpotcorr = 0.0;
+  if (fabs(dr[0]) > 0.5*lper[0] || fabs(dr[1]) > 0.5*lper[1] || fabs(dr[2]) > 0.5*lper[2]) {
+    double inv_dr = 1/sqrt(dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2] + SMALL_NUMBER);
+    double inv_dr3 = inv_dr*inv_dr*inv_dr;
+    for (int k=0; k<3; k++) acorr[k] -= m*dr[k]*inv_dr3;
+    *gpotcorr -= m*inv_dr;
+    for (int k=0; k<3; k++) {
+      const double size = 0.5*lper[k];
+      if (dr[k] > size) dr[k] -= 2*size;
+      else if (dr[k] < -size) dr[k] += 2*size;
+    }
+    inv_dr = 1/sqrt(dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2] + SMALL_NUMBER);
+    inv_dr3 = inv_dr*inv_dr*inv_dr;
+    for (int k=0; k<3; k++) acorr[k] += m*dr[k]*inv_dr3;
+    *gpotcorr += m*inv_dr;
+  }
+  int il[3];
+  double a[3];
+  for (int k=0; k<3; k++) {
+    const double b = e->dI[k]*fabs(dr[k]);
+    il[k] = (int) b;
+    a[k] = b - il[k];
+  }
+  double wf[8];
+  wf[0] = (1.0 - a[2])*(1.0 - a[1])*(1.0 - a[0]);
+  wf[1] = (1.0 - a[2])*(1.0 - a[1])*(      a[0]);
+  wf[2] = (1.0 - a[2])*(      a[1])*(1.0 - a[0]);
+  wf[3] = (1.0 - a[2])*(      a[1])*(      a[0]);
+  wf[4] = (      a[2])*(1.0 - a[1])*(1.0 - a[0]);
+  wf[5] = (      a[2])*(1.0 - a[1])*(      a[0]);
+  wf[6] = (      a[2])*(      a[1])*(1.0 - a[0]);
+  wf[7] = (      a[2])*(      a[1])*(      a[0]);
+  const int n0 = e->Ng[0], n01 = e->Ng[0]*e->Ng[1], one = n01*e->Ng[2];
+  const int i0 = il[0] + il[1]*n0 + il[2]*n01, i1 = i0 + n01;
+  double g[4];
+  for (int k=0; k<=3; k++) {
+    const double* f = e->field + k*one;
+    g[k] = wf[0]*f[i0] + wf[1]*f[i0 + 1] + wf[2]*f[i0 + n0] + wf[3]*f[i0 + n0 + 1] +
+           wf[4]*f[i1] + wf[5]*f[i1 + 1] + wf[6]*f[i1 + n0] + wf[7]*f[i1 + n0 + 1];
+  }
+  *gpotcorr += m*e->field[i0];
+  for (int k=0; k<3; k++) acorr[k] += m*g[k+1]*sgn_d(dr[k]);
+}
+
+/* the exact periodic gravity of a fully periodic box (GradhSphTree::UpdateAllSphForces with the
+ * brute-force search and Ewald, GradhSphTree.cpp:515-578): every pair's kernel-softened or
+ * direct term at its nearest image plus its Ewald correction, and the self potential */
+void orc_direct_gravity_periodic(const orc_params* p, const orc_ewald* e, int N,
+                                 const double* r, const double* m, const double* h,
+                                 const double* zeta, const double* hfactor, double* atree,
+                                 double* gpot) {
+  okern kern; kern_init(&kern, p);
+  obox b; box_init(&b, p, 0);
+  const double krsqd = kern.kernrangesqd;
+  const double wpot0 = K_wpot(&kern, 0.0);
+  #pragma omp parallel for schedule(dynamic, 16)
+  for (int i=0; i<N; i++) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    double pot = (m[i]/h[i])*wpot0;
+    for (int j=0; j<N; j++) {
+      double dr[3] = {r[3*j] - r[3*i], r[3*j+1] - r[3*i+1], r[3*j+2] - r[3*i+2]};
+      nearest(&b, dr);
+      grav_pair(&kern, krsqd, dr, m[j], h[i], h[j], zeta[i], zeta[j], hfactor[i], hfactor[j],
+                acc, &pot);
+      double ac[3], pc;
+      orc_ewald_corr(e, m[j], dr, ac, &pc);
+      for (int k=0; k<3; k++) acc[k] += ac[k];
+      pot += pc;
+    }
+    for (int k=0; k<3; k++) atree[3*i+k] = acc[k];
+    gpot[i] = pot;
+  }
+  kern_free(&kern);
+}
+
 void orc_direct_plummer(int N, const double* r, const double* m, double eps,
                         const int32_t* targets, int ntargets, double* a, double* pot) {
   const double eps2 = eps*eps;
@@ -939,7 +1115,6 @@ void orc_direct_plummer(int N, const double* r, const double* m, double eps,
   }
 }
 
-#define ORC_TWOPI 6.28318530717959   /* Constants.h:62 */
 
 /* Nbody::CalculateDirectGravForces (src/Nbody/Nbody.cpp:233-293) */
 void orc_nbody_direct(int N, const double* r, const double* v, const double* m, double* a,

@@ -91,6 +91,28 @@ void orc_nbody_direct_smoothed(const orc_params* p, int N, const double* r, cons
                                const double* m, const double* h, double* a, double* adot,
                                double* gpot);
 
+/* ---- Ewald periodic gravity (fully periodic box) ----
+ * Ewald::Ewald's correction field (Ewald.cpp:42-420, ewald_periodicity = 7): field holds
+ * 4*Ng[0]*Ng[1]*Ng[2] doubles, component-major (potential, a_x, a_y, a_z), or is NULL to get
+ * Ng and dI only */
+int orc_ewald_table(const double L[3], int nseries, int ngrid, double ewald_mult,
+                    double efratio, double* field, int Ng[3], double dI[3]);
+typedef struct {
+  int Ng[3];
+  double dI[3];
+  double L[3];
+  const double* field;
+} orc_ewald;
+/* Ewald::CalculatePeriodicCorrection (Ewald.cpp:500-650): the correction of mass m at
+ * dr = r_source - r_target */
+void orc_ewald_corr(const orc_ewald* e, double m, const double* dr, double* acorr,
+                    double* gpotcorr);
+/* exact periodic gravity: every pair's nearest-image term plus its Ewald correction */
+void orc_direct_gravity_periodic(const orc_params* p, const orc_ewald* e, int N,
+                                 const double* r, const double* m, const double* h,
+                                 const double* zeta, const double* hfactor, double* atree,
+                                 double* gpot);
+
 /* ---- stars beside the gas (Nbody<3>::stardata of an SphSimulation) ----
  * r (n,3), m, h n doubles; softening = nbody_softening, conservative =
  * conservative_sph_star_gravity */

@@ -49,6 +49,53 @@ class Tree(C.Structure):
     ]
 
 
+class Ewald(C.Structure):
+    """orc_ewald: the correction table of a fully periodic box."""
+    _fields_ = [("Ng", C.c_int * 3), ("dI", C.c_double * 3), ("L", C.c_double * 3),
+                ("field", C.c_void_p)]
+
+
+def ewald_table(L, nseries=10, ngrid=16, ewald_mult=1.0, efratio=1.0):
+    """Ewald::Ewald's correction field: an Ewald handle whose .table is (4, Ng_z, Ng_y, Ng_x)
+    (potential, a_x, a_y, a_z) and .spacing the node spacing per axis."""
+    Lv = _d(np.asarray(L, dtype=np.float64).reshape(3))
+    Ng = np.zeros(3, dtype=np.int32)
+    dI = np.zeros(3)
+    lib().orc_ewald_table(_p(Lv), int(nseries), int(ngrid), float(ewald_mult), float(efratio),
+                          None, _p(Ng), _p(dI))
+    field = np.zeros(4 * int(np.prod(Ng)))
+    lib().orc_ewald_table(_p(Lv), int(nseries), int(ngrid), float(ewald_mult), float(efratio),
+                          _p(field), _p(Ng), _p(dI))
+    e = Ewald()
+    for k in range(3):
+        e.Ng[k] = int(Ng[k]); e.dI[k] = float(dI[k]); e.L[k] = float(Lv[k])
+    e.field = _p(field)
+    e.table = field.reshape(4, int(Ng[2]), int(Ng[1]), int(Ng[0]))
+    e.spacing = 1.0 / dI
+    e._keep = field
+    return e
+
+
+def ewald_corr(e, m, dr):
+    """Ewald::CalculatePeriodicCorrection: (acceleration, potential) correction of mass m at
+    dr = r_source - r_target."""
+    a = np.zeros(3); pot = np.zeros(1)
+    lib().orc_ewald_corr(C.byref(e), float(m), _p(_d(np.asarray(dr, dtype=np.float64))), _p(a),
+                         _p(pot))
+    return a, float(pot[0])
+
+
+def direct_gravity_periodic(p: Params, e, r, m, st):
+    """Exact periodic gravity: nearest-image pair terms plus every pair's Ewald correction."""
+    r, m = _d(r), _d(m)
+    N = r.shape[0]
+    at = np.zeros((N, 3)); gp = np.zeros(N)
+    lib().orc_direct_gravity_periodic(C.byref(p), C.byref(e), N, _p(r), _p(m),
+                                      _p(_d(st["h"])), _p(_d(st["zeta"])),
+                                      _p(_d(st["hfactor"])), _p(at), _p(gp))
+    return at, gp
+
+
 class Stars(C.Structure):
     """orc_stars: stars beside the gas (positions (n,3), masses, smoothing lengths)."""
     _fields_ = [("n", C.c_int), ("r", C.c_void_p), ("m", C.c_void_p), ("h", C.c_void_p),
@@ -110,6 +157,10 @@ def lib():
         L.orc_sim_time.argtypes = [vp]; L.orc_sim_time.restype = d
         L.orc_sim_timestep.argtypes = [vp]; L.orc_sim_timestep.restype = d
         L.orc_sim_block_state.argtypes = [vp, vp, vp]
+        L.orc_ewald_table.argtypes = [vp, i, i, d, d, vp, vp, vp]
+        L.orc_ewald_table.restype = i
+        L.orc_ewald_corr.argtypes = [C.POINTER(Ewald), d, vp, vp, vp]
+        L.orc_direct_gravity_periodic.argtypes = [pp, C.POINTER(Ewald), i] + [vp] * 7
         L.orc_compute_h_stars.argtypes = [pp, i, vp, vp, i, C.POINTER(Stars)] + [vp] * 8
         L.orc_compute_h_stars.restype = i
         L.orc_gas_from_stars.argtypes = [pp, i, vp, vp, C.POINTER(Stars), vp, vp]
