@@ -28,6 +28,9 @@ namespace {
 
 constexpr int BS = 256;
 constexpr double NODE_SLACK = 1.0 + 1e-12; // node tests err on the inclusive side
+// the per-member filter of the candidate walks errs further on the inclusive side than any
+// target's own test (FP32 tests of the mixed path included)
+constexpr double KEEP_SLACK = 1.0 + 1e-4;
 constexpr int kInitBatches = 16;           // initial candidate capacity per group, in batches
 
 struct CandWalk {
@@ -135,13 +138,29 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
       }
       sp += 2*nopen;
       if (sp > kLdsStack) __threadfence_block();   // entries in the global overflow region
-      const int ecount = emit ? ec : 0;
-      const int incl = wave_incl_scan(ecount);
-      const int tot = __shfl(incl, 63, 64);
-      for (int k = 0; k < ecount; k++) {
-        const int slot = tail + incl - ecount + k;
-        if (slot < cap) idx[slot] = ef + k;
+      // a bucket's members that lie beyond the largest range from the group box fail every
+      // target's test below: they are dropped here, once per member, instead of being tested
+      // on all 64 lanes and staged by every pair pass (about a quarter of the members of the
+      // buckets a group reaches)
+      unsigned int keep = 0;
+      if (emit) {
+        for (int k = 0; k < ec; k++) {
+          const int j = ef + k;
+          double dx = a.x[j] - gc.x, dy = a.y[j] - gc.y, dz = a.z[j] - gc.z;
+          nearest_image<double>(pp, dx, dy, dz);
+          const double ex = fmax(fabs(dx) - gh.x, 0.0);
+          const double ey = fmax(fabs(dy) - gh.y, 0.0);
+          const double ez = fmax(fabs(dz) - gh.z, 0.0);
+          const double rr = GATHER ? kr*hG : kr*fmax(hG, a.hcand[j]);
+          if (ex*ex + ey*ey + ez*ez <= rr*rr*KEEP_SLACK) keep |= 1u << k;
+        }
       }
+      const int kcount = __popc(keep);
+      const int incl = wave_incl_scan(kcount);
+      const int tot = __shfl(incl, 63, 64);
+      int slot = tail + incl - kcount;
+      for (unsigned int m = keep; m; m &= m - 1, slot++)
+        if (slot < cap) idx[slot] = ef + __ffs(m) - 1;
       tail += tot;
     }
     if (overflow) {
