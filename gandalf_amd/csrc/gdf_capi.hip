@@ -311,6 +311,7 @@ int gdf_config_default(gdf_config* cfg) {
   cfg->nbody_softening = 1; cfg->conservative_sph_star_gravity = 1;
   cfg->n_ewald_grid = 16; cfg->gr_bhewaldseriesn = 10; cfg->ewald_mult = 1.0;
   cfg->efratio = 1.0;
+  cfg->ntreebuildstep = 1; cfg->ntreestockstep = 1;
   return GDF_OK;
 }
 
@@ -335,6 +336,7 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
        cfg->gr_bhewaldseriesn > 64 || !(cfg->ewald_mult > 0.0) || !(cfg->efratio >= 1.0) ||
        !(cfg->efratio <= 1.5)))
     return GDF_ERR_INVALID;
+  if (cfg->ntreebuildstep < 1 || cfg->ntreestockstep < 1) return GDF_ERR_INVALID;
   if ((cfg->nbody_softening != 0 && cfg->nbody_softening != 1) ||
       (cfg->conservative_sph_star_gravity != 0 && cfg->conservative_sph_star_gravity != 1))
     return GDF_ERR_INVALID;
@@ -469,7 +471,7 @@ int gdf_set_particles(gdf_ctx* c, int N, const double* r, const double* v, const
   for (int f = 0; f < I_NINT; f++) GDF_HIP(c, hipMemset(c->pi[f], 0, sizeof(int)*n));
   k_iota<<<nblocks(N, BS), BS, 0, c->stream>>>(N, c->iorig);
   GDF_HIP(c, hipStreamSynchronize(c->stream));
-  c->tree_valid = false; c->sg_valid = false; c->setup_done = false;
+  c->tree_valid = false; c->topo_valid = false; c->sg_valid = false; c->setup_done = false;
   c->t = c->timestep = c->tlast = 0.0; c->nsteps = 0;
   return GDF_OK;
 }
@@ -553,6 +555,17 @@ int gdf_update_all_sph_forces(gdf_ctx* c) {
   GDF_TRY(gravity_walk(c));
   GDF_TRY(integ_add_atree(c));
   return stars_on_gas(c);
+}
+
+int gdf_build_tree_step(gdf_ctx* c, int rebuild_tree, int n, int ntreebuildstep,
+                        int ntreestockstep) {
+  if (!c || c->N <= 0 || ntreebuildstep < 1 || ntreestockstep < 1) return GDF_ERR_INVALID;
+  if (rebuild_tree || !c->topo_valid || n%ntreebuildstep == 0) {
+    GDF_TRY(tree_build(c));
+    return 1;
+  }
+  GDF_TRY(tree_restock(c));
+  return 0;
 }
 
 int gdf_maximum_smoothing_length(gdf_ctx* c, double* hmax) {

@@ -148,7 +148,9 @@ struct gdf_ctx {
   int* grp_first = nullptr; int* grp_count = nullptr; int* grp_node = nullptr;
   int* nsel = nullptr; void* sel_tmp = nullptr; size_t sel_tmp_bytes = 0;
   int ngroups = 0;
-  bool tree_valid = false;
+  bool tree_valid = false;       // the tree matches the particles' current state
+  bool topo_valid = false;       // the tree's topology and groups match the particle order
+  bool rebuild_tree = true;      // SphSimulation's rebuild_tree: the next main loop rebuilds
 
   // per-group candidate sets: gather (h-rho search) and scatter-gather (forces)
   gdf::CandSet cs_g, cs_s;
@@ -245,6 +247,7 @@ int ctx_get_field_range(gdf_ctx* c, const char* name, int i0, int n, double* out
 int ctx_field_width(gdf_ctx* c, const char* name);
 int seg_upload(gdf_ctx* c);
 int tree_build(gdf_ctx* c);
+int tree_restock(gdf_ctx* c);   // refit of the kept topology to the current particle state
 int tree_refit_h(gdf_ctx* c);
 // kernel timing (ctx->timing): kind 5 gravity walk, 6 hydro pair kernel.  begin returns the
 // event pair's slot (-1 with timing off); collect waits for the recorded pairs and adds them

@@ -745,6 +745,7 @@ int seg_upload(gdf_ctx* c) {
   }
   GDF_HIP(c, hipGetLastError());
   c->tree_valid = false;
+  c->topo_valid = false;
   return GDF_OK;
 }
 
@@ -859,9 +860,32 @@ int tree_build(gdf_ctx* c) {
                                                        c->seg_root, c->grp_root);
   }
   c->tree_valid = true;
+  c->topo_valid = true;
   c->sg_valid = false;
   GDF_HIP(c, hipGetLastError());
   return integ_pack_posm(c);
+}
+
+// The kept topology and walk groups with every node's box, moments and hmax recomputed from the
+// particles' current state: the bottom-up pass of a build without the sort, the permutation and
+// the group selection.  Synchronises the stream, as a build does.
+int tree_restock(gdf_ctx* c) {
+  const int N = c->N;
+  hipStream_t s = c->stream;
+  if (!c->topo_valid) return fail(c, GDF_ERR_INVALID, "tree refit without a built tree");
+  NodeArgs na = node_args(c);
+  const double theta2 = c->phys.thetamaxsqd;
+  if (N > 1)
+    launch_refit<false>(c, na, theta2);
+  else
+    k_leaves<<<1, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M], c->pf[F_H],
+                              theta2, na, c->visit, c->parent);
+  GDF_HIP(c, hipGetLastError());
+  GDF_TRY(integ_pack_posm(c));
+  GDF_HIP(c, hipStreamSynchronize(s));
+  c->tree_valid = true;
+  c->sg_valid = false;
+  return GDF_OK;
 }
 
 int group_list(gdf_ctx* c, const int* status, int* list, int* ncand, int* gnc, int* gnn,

@@ -31,8 +31,11 @@ void record(gdf_ctx* c, int e) {
 }  // namespace
 
 template <int ndim>
-int GradhSphTreeGpu<ndim>::BuildTree(bool, int, int, int, double) {
-  return gdf::tree_build(ctx);
+int GradhSphTreeGpu<ndim>::BuildTree(bool rebuild_tree, int n, int ntreebuildstep,
+                                     int ntreestockstep, double) {
+  const int rc = gdf_build_tree_step(ctx, rebuild_tree ? 1 : 0, n, ntreebuildstep,
+                                     ntreestockstep);
+  return rc < 0 ? rc : GDF_OK;
 }
 
 template <int ndim>
@@ -171,6 +174,7 @@ int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   c->nsteps = nsteps0;
   c->setup_done = true;
+  c->rebuild_tree = true;          // the first main loop rebuilds (SphSimulation.cpp:339)
   return GDF_OK;
 }
 
@@ -192,7 +196,9 @@ int GradhSphSimulationGpu<ndim>::MainLoop() {
   GDF_TRY(hydroint.AdvanceParticles((int) c->nsteps, c->t, c->timestep));
   GDF_TRY(hydroint.CheckBoundaries());
   record(c, 1);
-  GDF_TRY(sphneib.BuildTree(true, (int) c->nsteps, 1, 1, c->timestep));
+  GDF_TRY(sphneib.BuildTree(c->rebuild_tree, (int) c->nsteps, c->cfg.ntreebuildstep,
+                            c->cfg.ntreestockstep, c->timestep));
+  c->rebuild_tree = false;
   record(c, 2);
   GDF_TRY(sphneib.UpdateAllSphProperties(false));
   record(c, 5);
@@ -248,7 +254,9 @@ int GradhSphSimulationGpu<ndim>::MainLoopBlock() {
   GDF_TRY(gdf::block_advance(c));
   GDF_TRY(hydroint.CheckBoundaries());
   record(c, 1);
-  GDF_TRY(sphneib.BuildTree(true, (int) c->nsteps, 1, 1, c->timestep));
+  GDF_TRY(sphneib.BuildTree(c->rebuild_tree, (int) c->nsteps, c->cfg.ntreebuildstep,
+                            c->cfg.ntreestockstep, c->timestep));
+  c->rebuild_tree = false;
   record(c, 2);
   // the tree build synchronised the stream: block_advance's count of active particles is in
   c->n_active = c->h_counters[12];

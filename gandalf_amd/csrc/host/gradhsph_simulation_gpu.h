@@ -19,7 +19,7 @@ template <int ndim>
 class GradhSphTreeGpu {
  public:
   explicit GradhSphTreeGpu(gdf_ctx* c) : ctx(c) {}
-  // HydroTree::BuildTree (HydroTree.cpp:310-372); the device tree is always rebuilt
+  // HydroTree::BuildTree (HydroTree.cpp:310-372): gdf_build_tree_step's schedule
   int BuildTree(bool rebuild_tree, int n, int ntreebuildstep, int ntreestockstep,
                 double timestep);
   // GradhSphTree::UpdateAllSphProperties (GradhSphTree.cpp:83-271)

@@ -46,6 +46,7 @@ class Config(C.Structure):
         ("nbody_softening", C.c_int), ("conservative_sph_star_gravity", C.c_int),
         ("n_ewald_grid", C.c_int), ("gr_bhewaldseriesn", C.c_int),
         ("ewald_mult", C.c_double), ("efratio", C.c_double),
+        ("ntreebuildstep", C.c_int), ("ntreestockstep", C.c_int),
     ]
 
 
@@ -106,6 +107,7 @@ PROTOTYPES = {
     "gdf_end_timestep_block": (_i, [_vp, _i, _d, _d]),
     "gdf_get_gather_neighbour_list": (_i, [_vp, _vp, _d, _i, _vp]),
     "gdf_snapshot_counters": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "gdf_build_tree_step": (_i, [_vp, _i, _i, _i, _i]),
     "gdf_set_stars": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "gdf_num_stars": (_i, [_vp]),
     "gdf_get_star_field": (_i, [_vp, C.c_char_p, _vp]),
@@ -170,6 +172,8 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
     c.gr_bhewaldseriesn = int(prm.get("gr_bhewaldseriesn", 10))
     c.ewald_mult = float(prm.get("ewald_mult", 1.0))
     c.efratio = float(prm.get("EFratio", 1.0))
+    c.ntreebuildstep = int(prm.get("ntreebuildstep", 1))
+    c.ntreestockstep = int(prm.get("ntreestockstep", 1))
     if c.nbody and c.nbody_eps <= 0.0:
         c.nbody_direct = 2 if c.nbody_softening else 1
     c.nbody_mult = float(prm.get("nbody_mult", 0.1))
@@ -239,6 +243,12 @@ class GpuSim:
     # ---- hot path ----
     def build_tree(self):
         self._chk(lib().gdf_build_tree(self._h))
+
+    def build_tree_step(self, rebuild_tree, n, ntreebuildstep, ntreestockstep):
+        """HydroTree::BuildTree's schedule: True after a rebuild, False after a refit."""
+        return bool(self._chk(lib().gdf_build_tree_step(self._h, int(bool(rebuild_tree)), int(n),
+                                                        int(ntreebuildstep),
+                                                        int(ntreestockstep))))
 
     def density(self, nopbc=False):
         self._chk(lib().gdf_update_all_sph_properties(self._h, int(nopbc)))

@@ -118,11 +118,13 @@ class GradhSphTreeGpu : public SphNeighbourSearch<ndim> {
   virtual ~GradhSphTreeGpu() {}
 
   // ---- NeighbourSearch<ndim> ----
-  // HydroTree::BuildTree (src/Tree/HydroTree.cpp:310-372): the device rebuilds every call
-  virtual void BuildTree(const bool, const int, const int, const int, const FLOAT,
-                         Hydrodynamics<ndim>* hydro) {
+  // HydroTree::BuildTree (src/Tree/HydroTree.cpp:310-372) with its schedule: rebuild, or
+  // refit of the kept tree on the other steps
+  virtual void BuildTree(const bool rebuild_tree, const int n, const int ntreebuildstep,
+                         const int ntreestockstep, const FLOAT, Hydrodynamics<ndim>* hydro) {
     dev->Upload(hydro);
-    check(dev->ctx, gdf_build_tree(dev->ctx));
+    check(dev->ctx, gdf_build_tree_step(dev->ctx, rebuild_tree ? 1 : 0, n, ntreebuildstep,
+                                        ntreestockstep));
   }
   virtual void BuildGhostTree(const bool, const int, const int, const int, const FLOAT,
                               Hydrodynamics<ndim>*) {}

@@ -85,6 +85,11 @@ typedef struct gdf_config {
   int gr_bhewaldseriesn;
   double ewald_mult;
   double efratio;
+  /* "ntreebuildstep", "ntreestockstep" (Parameters.cpp:311-312): the main loop rebuilds the tree
+   * every ntreebuildstep steps and refits the kept tree to the particles' new positions on the
+   * others (HydroTree::BuildTree, HydroTree.cpp:310-372) */
+  int ntreebuildstep;
+  int ntreestockstep;
 } gdf_config;
 
 typedef struct gdf_ctx gdf_ctx;
@@ -115,6 +120,14 @@ int gdf_num_particles(const gdf_ctx* ctx);
 /* ---- hot path: SphNeighbourSearch<3> (src/Headers/SphNeighbourSearch.h:77-94) ---- */
 /* HydroTree::BuildTree (src/Tree/HydroTree.cpp:310-372): Morton sort, LBVH, moments */
 int gdf_build_tree(gdf_ctx* ctx);
+/* HydroTree::BuildTree(rebuild_tree, n, ntreebuildstep, ntreestockstep) with its schedule: a
+ * full rebuild when rebuild_tree is set, n % ntreebuildstep == 0 or the particles changed since
+ * the last build; otherwise the kept tree is refitted to the current positions, masses and
+ * smoothing lengths (where the reference stocks the tree or extrapolates its cells, Tree.cpp:
+ * 172-198: a refit keeps every node's box exact, which exact neighbour sets need).  Returns 1
+ * after a rebuild, 0 after a refit, or a negative status. */
+int gdf_build_tree_step(gdf_ctx* ctx, int rebuild_tree, int n, int ntreebuildstep,
+                        int ntreestockstep);
 /* GradhSphTree::UpdateAllSphProperties (src/GradhSph/GradhSphTree.cpp:83-271);
  * nopbc = 1 ignores periodic images (the reference's first setup pass runs before its ghost
  * particles exist, SphSimulation.cpp:273-281) */
