@@ -93,6 +93,7 @@ bool field_ptrs(gdf_ctx* c, const char* name, double** p, int* w) {
 }
 
 void free_particles(gdf_ctx* c) {
+  build_graphs_free(c);
   for (int f = 0; f < F_NPERSIST; f++) { hipFree(c->pf[f]); hipFree(c->pf_alt[f]); c->pf[f] = c->pf_alt[f] = nullptr; }
   for (int f = 0; f < D_NDERIVED; f++) {
     hipFree(c->df[f]); hipFree(c->df_alt[f]); c->df[f] = c->df_alt[f] = nullptr;
@@ -312,6 +313,7 @@ int gdf_config_default(gdf_config* cfg) {
   cfg->n_ewald_grid = 16; cfg->gr_bhewaldseriesn = 10; cfg->ewald_mult = 1.0;
   cfg->efratio = 1.0;
   cfg->ntreebuildstep = 1; cfg->ntreestockstep = 1;
+  cfg->graph_capture = 0;
   return GDF_OK;
 }
 
@@ -336,7 +338,9 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
        cfg->gr_bhewaldseriesn > 64 || !(cfg->ewald_mult > 0.0) || !(cfg->efratio >= 1.0) ||
        !(cfg->efratio <= 1.5)))
     return GDF_ERR_INVALID;
-  if (cfg->ntreebuildstep < 1 || cfg->ntreestockstep < 1) return GDF_ERR_INVALID;
+  if (cfg->ntreebuildstep < 1 || cfg->ntreestockstep < 1 ||
+      (cfg->graph_capture != 0 && cfg->graph_capture != 1))
+    return GDF_ERR_INVALID;
   if ((cfg->nbody_softening != 0 && cfg->nbody_softening != 1) ||
       (cfg->conservative_sph_star_gravity != 0 && cfg->conservative_sph_star_gravity != 1))
     return GDF_ERR_INVALID;
@@ -555,6 +559,13 @@ int gdf_update_all_sph_forces(gdf_ctx* c) {
   GDF_TRY(gravity_walk(c));
   GDF_TRY(integ_add_atree(c));
   return stars_on_gas(c);
+}
+
+int gdf_graph_state(const gdf_ctx* c, int64_t* replays, int* in_use) {
+  if (!c) return GDF_ERR_INVALID;
+  if (replays) *replays = c->graph_replays;
+  if (in_use) *in_use = !c->cfg.graph_capture ? 0 : (c->graph_failed ? -c->graph_why : 1);
+  return GDF_OK;
 }
 
 int gdf_build_tree_step(gdf_ctx* c, int rebuild_tree, int n, int ntreebuildstep,

@@ -218,6 +218,16 @@ struct gdf_ctx {
   double snap_tlast = 0.0, snap_tlitelast = 0.0;
   std::vector<int> snap_porig;
 
+  // captured graphs of the tree build's launches, one per buffer arrangement (the build's
+  // swaps alternate between two); graph_failed: a capture failed, the context builds eagerly
+  static constexpr int kBuildGraphs = 2;
+  hipGraphExec_t bg_exec[kBuildGraphs] = {};
+  std::vector<const void*> bg_key[kBuildGraphs];
+  int bg_next = 0;
+  bool graph_failed = false;
+  int graph_why = 0;             // why the last capture was given up (0: it was not)
+  long long graph_replays = 0;
+
   // Ewald correction table (device copies, FP64 and FP32)
   double4* ewald_tab = nullptr;
   float4* ewald_tabf = nullptr;
@@ -247,7 +257,8 @@ int ctx_get_field_range(gdf_ctx* c, const char* name, int i0, int n, double* out
 int ctx_field_width(gdf_ctx* c, const char* name);
 int seg_upload(gdf_ctx* c);
 int tree_build(gdf_ctx* c);
-int tree_restock(gdf_ctx* c);   // refit of the kept topology to the current particle state
+int tree_restock(gdf_ctx* c);
+void build_graphs_free(gdf_ctx* c);   // the captured tree-build graphs   // refit of the kept topology to the current particle state
 int tree_refit_h(gdf_ctx* c);
 // kernel timing (ctx->timing): kind 5 gravity walk, 6 hydro pair kernel.  begin returns the
 // event pair's slot (-1 with timing off); collect waits for the recorded pairs and adds them

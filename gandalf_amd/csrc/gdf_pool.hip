@@ -214,6 +214,9 @@ int gdf_pool_create(const gdf_config* cfg, int64_t max_particles, gdf_pool** out
   if (!cfg || !out || max_particles <= 0) return GDF_ERR_INVALID;
   *out = nullptr;
   if (cfg->Nlevels != 1 || cfg->nbody_direct != 0) return GDF_ERR_INVALID;
+  // serving runs several batches from several host threads: the captured tree-build graphs
+  // are for a context driven by one thread alone
+  if (cfg->graph_capture) return GDF_ERR_INVALID;
   if (max_particles > (int64_t) 1 << 30) return GDF_ERR_INVALID;
   gdf_pool* P = new (std::nothrow) gdf_pool();
   if (!P) return GDF_ERR_NOMEM;

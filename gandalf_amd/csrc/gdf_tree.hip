@@ -11,8 +11,6 @@
 #include <hip/hip_runtime.h>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_select.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "gdf_internal.h"
 #include "gdf_device.h"
@@ -75,6 +73,7 @@ __global__ __launch_bounds__(BS) void k_bounds_final(int nb, const double* __res
 // ---- serving batches: per-segment bounds and segment-prefixed keys ----
 // partial-reduction blocks per segment: kMaxSeg x 16 x 6 partials fit the reduction buffer
 constexpr int kSegBlocks = 16;
+constexpr int kSelBlock = 256;      // workgroup of the flag selection; 16 flags per thread
 
 // grid (kSegBlocks, nseg): block (b, s) folds a strided share of segment s
 __global__ void k_seg_bounds_partial(const int* __restrict__ off, const double* __restrict__ x,
@@ -779,8 +778,7 @@ int tree_alloc(gdf_ctx* c) {
   size_t b1 = 0, b2 = 0;
   GDF_HIP(c, rocprim::radix_sort_pairs(nullptr, b1, c->keys, c->keys_alt, c->vals, c->vals_alt,
                                        (unsigned int) N, 0, 64, c->stream));
-  GDF_HIP(c, rocprim::select(nullptr, b2, rocprim::counting_iterator<int>(0),
-                             (unsigned char*) c->visit, c->grp_first, c->nsel, N, c->stream));
+  b2 = sizeof(int)*((size_t) nblocks(nblocks((long long) N, 16), kSelBlock) + 16);
   c->sort_tmp_bytes = b1;
   c->sel_tmp_bytes = b2;
   GDF_HIP(c, hipMalloc(&c->sort_tmp, b1 > 0 ? b1 : 16));
@@ -788,45 +786,131 @@ int tree_alloc(gdf_ctx* c) {
   return GDF_OK;
 }
 
-int tree_build(gdf_ctx* c) {
+namespace {
+
+// Order-preserving selection of the set bytes of n 0/1 byte flags (zero-padded to whole
+// 16-byte words): per-block counts, an exclusive scan of the block counts by one workgroup, and
+// a per-block scatter of the indices.  No memset and no library call, so the tree build's
+// launches capture into a graph of kernel nodes only.
+__device__ __forceinline__ int bytes_set(uint4 w) {
+  // each byte is 0 or 1: the byte sum of a word is its top byte after multiplying by 0x01010101
+  return (int) (((w.x*0x01010101u) >> 24) + ((w.y*0x01010101u) >> 24) +
+                ((w.z*0x01010101u) >> 24) + ((w.w*0x01010101u) >> 24));
+}
+
+__global__ __launch_bounds__(kSelBlock) void k_sel_count(int nw, const uint4* __restrict__ flags,
+                                                         int* __restrict__ bcount) {
+  __shared__ int red[kSelBlock];
+  const int w = blockIdx.x*kSelBlock + threadIdx.x;
+  red[threadIdx.x] = w < nw ? bytes_set(flags[w]) : 0;
+  __syncthreads();
+  for (int st = kSelBlock/2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bcount[blockIdx.x] = red[0];
+}
+
+// exclusive scan of nb block counts in place, the total into *total; one workgroup
+__global__ __launch_bounds__(kSelBlock) void k_sel_scan(int nb, int* __restrict__ bcount,
+                                                        int* __restrict__ total) {
+  __shared__ int part[kSelBlock];
+  const int per = (nb + kSelBlock - 1)/kSelBlock;
+  const int b0 = min(threadIdx.x*per, nb), b1 = min(b0 + per, nb);
+  int sum = 0;
+  for (int b = b0; b < b1; b++) sum += bcount[b];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int t = 0; t < kSelBlock; t++) { const int v = part[t]; part[t] = run; run += v; }
+    *total = run;
+  }
+  __syncthreads();
+  int run = part[threadIdx.x];
+  for (int b = b0; b < b1; b++) { const int v = bcount[b]; bcount[b] = run; run += v; }
+}
+
+__global__ __launch_bounds__(kSelBlock) void k_sel_scatter(int nw, const uint4* __restrict__ flags,
+                                                           const int* __restrict__ boff,
+                                                           int* __restrict__ out) {
+  __shared__ int pre[kSelBlock];
+  const int w = blockIdx.x*kSelBlock + threadIdx.x;
+  const uint4 x = w < nw ? flags[w] : make_uint4(0u, 0u, 0u, 0u);
+  const int cnt = bytes_set(x);
+  pre[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int st = 1; st < kSelBlock; st <<= 1) {          // inclusive scan (Hillis-Steele)
+    const int v = threadIdx.x >= st ? pre[threadIdx.x - st] : 0;
+    __syncthreads();
+    pre[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (cnt == 0) return;
+  int o = boff[blockIdx.x] + pre[threadIdx.x] - cnt;
+  const unsigned int word[4] = {x.x, x.y, x.z, x.w};
+  for (int k = 0; k < 16; k++)
+    if ((word[k >> 2] >> (8*(k & 3))) & 0xffu) out[o++] = 16*w + k;
+}
+
+// The byte flags of the group selection, cleared by a kernel (a memset would become a memset
+// node of the captured graph)
+__global__ void k_zero_words(int n, uint4* __restrict__ p) {
+  const int i = blockIdx.x*BS + threadIdx.x;
+  if (i < n) p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Steps 1-6 of a tree build: every launch that needs no value back on the host.  The host's
+// buffer swaps (double-buffered keys, indices and particle state) run with launch == false
+// too, so a replayed graph of these launches leaves the host state as the launches themselves
+// would.
+int build_body(gdf_ctx* c, hipStream_t s, bool launch) {
   const int N = c->N;
-  hipStream_t s = c->stream;
-  if (N <= 0) return fail(c, GDF_ERR_INVALID, "tree build without particles");
   const int nbp = std::min(nblocks(N, BS), 1024);
   // 1. bounds and Morton keys
-  if (c->nseg > 0) {
-    k_seg_bounds_partial<<<dim3(kSegBlocks, c->nseg), BS, 0, s>>>(c->d_seg_off, c->pf[F_X],
-                                                                  c->pf[F_Y], c->pf[F_Z], c->red);
-    k_seg_bounds_final<<<c->nseg, 64, 0, s>>>(c->red, c->seg_bnd);
-    k_morton_seg<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z],
-                                               c->pi[I_SEG], c->seg_bnd, c->keys, c->vals);
-  } else {
-    k_bounds_partial<<<nbp, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->red);
-    k_bounds_final<<<1, BS, 0, s>>>(nbp, c->red, c->red + 6*1024);
-    k_morton<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z],
-                                           c->red + 6*1024, c->keys, c->vals);
+  if (launch) {
+    if (c->nseg > 0) {
+      k_seg_bounds_partial<<<dim3(kSegBlocks, c->nseg), BS, 0, s>>>(c->d_seg_off, c->pf[F_X],
+                                                                    c->pf[F_Y], c->pf[F_Z],
+                                                                    c->red);
+      k_seg_bounds_final<<<c->nseg, 64, 0, s>>>(c->red, c->seg_bnd);
+      k_morton_seg<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z],
+                                                 c->pi[I_SEG], c->seg_bnd, c->keys, c->vals);
+    } else {
+      k_bounds_partial<<<nbp, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->red);
+      k_bounds_final<<<1, BS, 0, s>>>(nbp, c->red, c->red + 6*1024);
+      k_morton<<<nblocks(N, BS), BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z],
+                                             c->red + 6*1024, c->keys, c->vals);
+    }
+    // 2. stable radix sort of (key, index)
+    size_t bytes = c->sort_tmp_bytes;
+    GDF_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp, bytes, c->keys, c->keys_alt, c->vals,
+                                         c->vals_alt, (unsigned int) N, 0,
+                                         c->nseg > 0 ? 64 : 63, s));
   }
-  // 2. stable radix sort of (key, index)
-  size_t bytes = c->sort_tmp_bytes;
-  GDF_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp, bytes, c->keys, c->keys_alt, c->vals,
-                                       c->vals_alt, (unsigned int) N, 0,
-                                       c->nseg > 0 ? 64 : 63, s));
   std::swap(c->keys, c->keys_alt);
   std::swap(c->vals, c->vals_alt);
   // 3. permute persistent particle state into key order
-  PermArgs pa;
-  for (int f = 0; f < F_NPERSIST; f++) { pa.src[f] = c->pf[f]; pa.dst[f] = c->pf_alt[f]; }
-  for (int f = 0; f < I_NINT; f++) { pa.isrc[f] = c->pi[f]; pa.idst[f] = c->pi_alt[f]; }
-  k_permute<<<nblocks(N, BS), BS, 0, s>>>(N, c->vals, pa, c->iorig, c->iorig_alt);
+  if (launch) {
+    PermArgs pa;
+    for (int f = 0; f < F_NPERSIST; f++) { pa.src[f] = c->pf[f]; pa.dst[f] = c->pf_alt[f]; }
+    for (int f = 0; f < I_NINT; f++) { pa.isrc[f] = c->pi[f]; pa.idst[f] = c->pi_alt[f]; }
+    k_permute<<<nblocks(N, BS), BS, 0, s>>>(N, c->vals, pa, c->iorig, c->iorig_alt);
+  }
   for (int f = 0; f < F_NPERSIST; f++) std::swap(c->pf[f], c->pf_alt[f]);
   for (int f = 0; f < I_NINT; f++) std::swap(c->pi[f], c->pi_alt[f]);
   std::swap(c->iorig, c->iorig_alt);
   if (c->df_alt[0]) {
-    PermDArgs pd;
-    for (int k = 0; k < kNPermD; k++) { pd.src[k] = c->df[kPermD[k]]; pd.dst[k] = c->df_alt[kPermD[k]]; }
-    k_permute_d<<<nblocks(N, BS), BS, 0, s>>>(N, c->vals, pd);
+    if (launch) {
+      PermDArgs pd;
+      for (int k = 0; k < kNPermD; k++) {
+        pd.src[k] = c->df[kPermD[k]]; pd.dst[k] = c->df_alt[kPermD[k]];
+      }
+      k_permute_d<<<nblocks(N, BS), BS, 0, s>>>(N, c->vals, pd);
+    }
     for (int k = 0; k < kNPermD; k++) std::swap(c->df[kPermD[k]], c->df_alt[kPermD[k]]);
   }
+  if (!launch) return GDF_OK;
   // 4. topology
   if (N > 1)
     k_lbvh<<<nblocks(N - 1, BS), BS, 0, s>>>(N, c->keys, c->left, c->right, c->first, c->last,
@@ -840,15 +924,160 @@ int tree_build(gdf_ctx* c) {
     k_leaves<<<1, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M], c->pf[F_H],
                               theta2, na, c->visit, c->parent);
   // 6. walk groups in ascending first-particle order
-  GDF_HIP(c, hipMemsetAsync(c->visit, 0, (size_t) N, s));   // N byte flags
+  const int nw = (N + 15)/16;                // N byte flags, padded to 16 bytes
+  k_zero_words<<<nblocks(nw, BS), BS, 0, s>>>(nw, reinterpret_cast<uint4*>(c->visit));
   k_groups_mark<<<nblocks(2*N - 1, BS), BS, 0, s>>>(N, c->cfg.group_max, c->parent, c->first,
                                                     c->last, c->gstart,
                                                     (unsigned char*) c->visit);
-  bytes = c->sel_tmp_bytes;
-  GDF_HIP(c, rocprim::select(c->sel_tmp, bytes, rocprim::counting_iterator<int>(0),
-                             (const unsigned char*) c->visit, c->grp_first, c->nsel, (size_t) N, s));
+  const int nsb = nblocks(nw, kSelBlock);
+  int* bcount = reinterpret_cast<int*>(c->sel_tmp);
+  const uint4* fw = reinterpret_cast<const uint4*>(c->visit);
+  k_sel_count<<<nsb, kSelBlock, 0, s>>>(nw, fw, bcount);
+  k_sel_scan<<<1, kSelBlock, 0, s>>>(nsb, bcount, c->nsel);
+  k_sel_scatter<<<nsb, kSelBlock, 0, s>>>(nw, fw, bcount, c->grp_first);
   k_groups_fill<<<nblocks(N, BS), BS, 0, s>>>(N, c->nsel, c->grp_first, c->gstart, c->first,
                                               c->last, c->grp_node, c->grp_count);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
+// every buffer pointer the build swaps, in a fixed order: with the particle count, segment
+// count and configuration of the context it keys a captured graph
+std::vector<const void*> build_key(const gdf_ctx* c) {
+  std::vector<const void*> k = {c->keys, c->keys_alt, c->vals, c->vals_alt, c->iorig,
+                                c->iorig_alt, c->df_alt[0]};
+  for (int f = 0; f < F_NPERSIST; f++) { k.push_back(c->pf[f]); k.push_back(c->pf_alt[f]); }
+  for (int f = 0; f < I_NINT; f++) { k.push_back(c->pi[f]); k.push_back(c->pi_alt[f]); }
+  for (int i = 0; i < kNPermD; i++) {
+    k.push_back(c->df[kPermD[i]]); k.push_back(c->df_alt[kPermD[i]]);
+  }
+  // every other buffer and value the launches take
+  const void* other[] = {c->d_seg_off, c->seg_bnd, c->red, c->sort_tmp, c->sel_tmp, c->nsel,
+                         c->left, c->right, c->first, c->last, c->parent, c->visit, c->front,
+                         c->bbmin, c->bbmax, c->geo_c, c->geo_h, c->mp, c->gstart, c->grp_first,
+                         c->grp_count, c->grp_node};
+  for (const void* q : other) k.push_back(q);
+  uint64_t theta_bits;
+  std::memcpy(&theta_bits, &c->phys.thetamaxsqd, sizeof(theta_bits));
+  const uint64_t vals[] = {(uint64_t) c->N, (uint64_t) c->nseg, (uint64_t) c->sort_tmp_bytes,
+                           (uint64_t) c->sel_tmp_bytes, (uint64_t) c->cfg.group_max, theta_bits};
+  for (uint64_t v : vals) k.push_back(reinterpret_cast<const void*>((uintptr_t) v));
+  return k;
+}
+
+// a captured graph is used only if it holds kernel and copy nodes (no memset nodes, whose
+// replays are not trusted here)
+// returns 0, or 100 + the type of the first node of another kind
+int graph_nodes_check(hipGraph_t g) {
+  size_t n = 0;
+  if (hipGraphGetNodes(g, nullptr, &n) != hipSuccess) return 99;
+  std::vector<hipGraphNode_t> nodes(n);
+  if (n > 0 && hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess) return 99;
+  for (hipGraphNode_t nd : nodes) {
+    hipGraphNodeType t;
+    if (hipGraphNodeGetType(nd, &t) != hipSuccess) return 99;
+    if (t != hipGraphNodeTypeKernel && t != hipGraphNodeTypeMemcpy &&
+        t != hipGraphNodeTypeEmpty)
+      return 100 + (int) t;
+  }
+  return 0;
+}
+
+// the build's launches through a graph captured at the first build of each buffer
+// arrangement (the swaps alternate between two) and replayed after; a capture that fails or
+// holds other nodes leaves the context building eagerly
+int build_launches(gdf_ctx* c, hipStream_t s) {
+  if (!c->cfg.graph_capture || c->graph_failed) return build_body(c, s, true);
+  const std::vector<const void*> key = build_key(c);
+  for (int e = 0; e < gdf_ctx::kBuildGraphs; e++)
+    if (c->bg_exec[e] && c->bg_key[e] == key) {
+      GDF_HIP(c, hipGraphLaunch(c->bg_exec[e], s));
+      c->graph_replays++;
+      return build_body(c, s, false);
+    }
+  // capture: the host swaps happen during the capture, so they are undone afterwards (the
+  // replay below redoes them)
+  struct Saved {
+    double* pf[F_NPERSIST]; double* pf_alt[F_NPERSIST];
+    int* pi[I_NINT]; int* pi_alt[I_NINT];
+    double* df[D_NDERIVED]; double* df_alt[D_NDERIVED];
+    uint64_t* keys; uint64_t* keys_alt; int* vals; int* vals_alt; int* iorig; int* iorig_alt;
+  } sv;
+  std::copy(std::begin(c->pf), std::end(c->pf), sv.pf);
+  std::copy(std::begin(c->pf_alt), std::end(c->pf_alt), sv.pf_alt);
+  std::copy(std::begin(c->pi), std::end(c->pi), sv.pi);
+  std::copy(std::begin(c->pi_alt), std::end(c->pi_alt), sv.pi_alt);
+  std::copy(std::begin(c->df), std::end(c->df), sv.df);
+  std::copy(std::begin(c->df_alt), std::end(c->df_alt), sv.df_alt);
+  sv.keys = c->keys; sv.keys_alt = c->keys_alt; sv.vals = c->vals; sv.vals_alt = c->vals_alt;
+  sv.iorig = c->iorig; sv.iorig_alt = c->iorig_alt;
+  auto restore = [&]() {
+    std::copy(sv.pf, sv.pf + F_NPERSIST, c->pf);
+    std::copy(sv.pf_alt, sv.pf_alt + F_NPERSIST, c->pf_alt);
+    std::copy(sv.pi, sv.pi + I_NINT, c->pi);
+    std::copy(sv.pi_alt, sv.pi_alt + I_NINT, c->pi_alt);
+    std::copy(sv.df, sv.df + D_NDERIVED, c->df);
+    std::copy(sv.df_alt, sv.df_alt + D_NDERIVED, c->df_alt);
+    c->keys = sv.keys; c->keys_alt = sv.keys_alt; c->vals = sv.vals; c->vals_alt = sv.vals_alt;
+    c->iorig = sv.iorig; c->iorig_alt = sv.iorig_alt;
+  };
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  // why a capture was given up, kept for gdf_graph_state: 1 begin, 2 a launch, 3 end,
+  // 100 + node type of a node that is not a kernel or copy, 4 instantiation
+  int why = 0;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    why = 1;
+  } else {
+    const int rc = build_body(c, s, true);
+    const hipError_t ee = hipStreamEndCapture(s, &g);
+    if (rc != GDF_OK) why = 2;
+    else if (ee != hipSuccess || !g) why = 3;
+    else if (const int nt = graph_nodes_check(g)) why = nt;
+    else if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) why = 4;
+  }
+  const bool ok = why == 0;
+  c->graph_why = why;
+  if (g) (void) hipGraphDestroy(g);
+  // a capture invalidated part way (by another thread's call on the device, say) may leave the
+  // stream capturing: end it, and clear the error it left, before building eagerly
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+    hipGraph_t g2 = nullptr;
+    (void) hipStreamEndCapture(s, &g2);
+    if (g2) (void) hipGraphDestroy(g2);
+  }
+  (void) hipGetLastError();
+  restore();
+  if (!ok) {
+    if (ex) (void) hipGraphExecDestroy(ex);
+    c->graph_failed = true;
+    return build_body(c, s, true);
+  }
+  const int e = c->bg_next;
+  c->bg_next = (c->bg_next + 1)%gdf_ctx::kBuildGraphs;
+  if (c->bg_exec[e]) (void) hipGraphExecDestroy(c->bg_exec[e]);
+  c->bg_exec[e] = ex;
+  c->bg_key[e] = key;
+  GDF_HIP(c, hipGraphLaunch(ex, s));
+  return build_body(c, s, false);
+}
+
+}  // namespace
+
+void build_graphs_free(gdf_ctx* c) {
+  for (int e = 0; e < gdf_ctx::kBuildGraphs; e++) {
+    if (c->bg_exec[e]) (void) hipGraphExecDestroy(c->bg_exec[e]);
+    c->bg_exec[e] = nullptr;
+    c->bg_key[e].clear();
+  }
+}
+
+int tree_build(gdf_ctx* c) {
+  const int N = c->N;
+  hipStream_t s = c->stream;
+  if (N <= 0) return fail(c, GDF_ERR_INVALID, "tree build without particles");
+  GDF_TRY(build_launches(c, s));
   GDF_HIP(c, hipMemcpyAsync(c->h_counters, c->nsel, sizeof(int), hipMemcpyDeviceToHost, s));
   GDF_HIP(c, hipStreamSynchronize(s));
   c->ngroups = c->h_counters[0];

@@ -47,6 +47,7 @@ class Config(C.Structure):
         ("n_ewald_grid", C.c_int), ("gr_bhewaldseriesn", C.c_int),
         ("ewald_mult", C.c_double), ("efratio", C.c_double),
         ("ntreebuildstep", C.c_int), ("ntreestockstep", C.c_int),
+        ("graph_capture", C.c_int),
     ]
 
 
@@ -108,6 +109,7 @@ PROTOTYPES = {
     "gdf_get_gather_neighbour_list": (_i, [_vp, _vp, _d, _i, _vp]),
     "gdf_snapshot_counters": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "gdf_build_tree_step": (_i, [_vp, _i, _i, _i, _i]),
+    "gdf_graph_state": (_i, [_vp, C.POINTER(_i64), C.POINTER(_i)]),
     "gdf_set_stars": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "gdf_num_stars": (_i, [_vp]),
     "gdf_get_star_field": (_i, [_vp, C.c_char_p, _vp]),
@@ -174,6 +176,7 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
     c.efratio = float(prm.get("EFratio", 1.0))
     c.ntreebuildstep = int(prm.get("ntreebuildstep", 1))
     c.ntreestockstep = int(prm.get("ntreestockstep", 1))
+    c.graph_capture = int(prm.get("graph_capture", 0))
     if c.nbody and c.nbody_eps <= 0.0:
         c.nbody_direct = 2 if c.nbody_softening else 1
     c.nbody_mult = float(prm.get("nbody_mult", 0.1))
@@ -249,6 +252,13 @@ class GpuSim:
         return bool(self._chk(lib().gdf_build_tree_step(self._h, int(bool(rebuild_tree)), int(n),
                                                         int(ntreebuildstep),
                                                         int(ntreestockstep))))
+
+    def graph_state(self):
+        """(replays of the captured tree-build graphs, state: 1 in use, 0 off, negative: why a
+        capture was given up)"""
+        r, u = C.c_int64(), C.c_int()
+        self._chk(lib().gdf_graph_state(self._h, C.byref(r), C.byref(u)))
+        return r.value, u.value
 
     def density(self, nopbc=False):
         self._chk(lib().gdf_update_all_sph_properties(self._h, int(nopbc)))

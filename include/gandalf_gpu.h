@@ -90,6 +90,11 @@ typedef struct gdf_config {
    * others (HydroTree::BuildTree, HydroTree.cpp:310-372) */
   int ntreebuildstep;
   int ntreestockstep;
+  /* 1: the tree build's launches (bounds, keys, sort, permutation, topology, moments, group
+   * selection) run as a HIP graph captured once per buffer arrangement and replayed; 0 (the
+   * default): eager launches.  The results are the same bit for bit.  For a context driven by
+   * one host thread with no other thread using the device meanwhile; serving pools refuse it. */
+  int graph_capture;
 } gdf_config;
 
 typedef struct gdf_ctx gdf_ctx;
@@ -128,6 +133,12 @@ int gdf_build_tree(gdf_ctx* ctx);
  * after a rebuild, 0 after a refit, or a negative status. */
 int gdf_build_tree_step(gdf_ctx* ctx, int rebuild_tree, int n, int ntreebuildstep,
                         int ntreestockstep);
+/* state of the captured tree-build graphs (cfg.graph_capture; no counterpart in the
+ * reference): replays so far, and 1 if they are in use, 0 if capture is off, or minus the
+ * reason a capture was given up (the context then builds eagerly): 1 capture could not begin,
+ * 2 a launch failed, 3 capture could not end, 4 instantiation failed, 100 + the type of a
+ * node that is neither a kernel nor a copy */
+int gdf_graph_state(const gdf_ctx* ctx, int64_t* replays, int* in_use);
 /* GradhSphTree::UpdateAllSphProperties (src/GradhSph/GradhSphTree.cpp:83-271);
  * nopbc = 1 ignores periodic images (the reference's first setup pass runs before its ghost
  * particles exist, SphSimulation.cpp:273-281) */
