@@ -314,6 +314,7 @@ int gdf_config_default(gdf_config* cfg) {
   cfg->efratio = 1.0;
   cfg->ntreebuildstep = 1; cfg->ntreestockstep = 1;
   cfg->graph_capture = 0;
+  cfg->multipole = GDF_MULTIPOLE_QUADRUPOLE;
   return GDF_OK;
 }
 
@@ -339,7 +340,8 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
        !(cfg->efratio <= 1.5)))
     return GDF_ERR_INVALID;
   if (cfg->ntreebuildstep < 1 || cfg->ntreestockstep < 1 ||
-      (cfg->graph_capture != 0 && cfg->graph_capture != 1))
+      (cfg->graph_capture != 0 && cfg->graph_capture != 1) ||
+      cfg->multipole < GDF_MULTIPOLE_QUADRUPOLE || cfg->multipole > GDF_MULTIPOLE_FAST_QUADRUPOLE)
     return GDF_ERR_INVALID;
   if ((cfg->nbody_softening != 0 && cfg->nbody_softening != 1) ||
       (cfg->conservative_sph_star_gravity != 0 && cfg->conservative_sph_star_gravity != 1))
@@ -381,6 +383,7 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
   p.eps2 = cfg->nbody_eps*cfg->nbody_eps;
   p.nbody_mult = cfg->nbody_mult;
   p.hrho_itmax = cfg->hrho_iteration_max > 0 ? cfg->hrho_iteration_max : 30;
+  p.multipole = cfg->multipole;
   c->nlevels = cfg->Nlevels; c->level_diff_max = cfg->level_diff_max;
   int rc = upload_kernel_tables(c);
   // a periodic box with self-gravity carries the Ewald correction (Simulation.cpp:1183-1198)

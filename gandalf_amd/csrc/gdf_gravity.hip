@@ -125,6 +125,51 @@ __device__ __forceinline__ void quad_term(T dx, T dy, T dz, T mc, const T* q, T&
   pot += mc*invdrmag + T(0.5)*qscalar*invdr5;
 }
 
+// FastMultipoleForces::AddMonopoleContribution / AddQuadrupoleContribution
+// (NeighbourSearch.h:564-597, 675-723) with the second-order terms off: one accepted cell's
+// potential, acceleration and tidal tensor at the walk group's expansion point, added to
+// f = (pot, ax, ay, az, txx, txy, tyy, txz, tyz, tzz).  dr = cell - expansion point; the
+// reference's dphi takes the same increments as its ac, so one vector serves both.
+__device__ __forceinline__ void fast_add(double* f, double dx, double dy, double dz, double mc,
+                                         const double* q, bool quad) {
+  {
+    const double invdrmag = sqrt(1.0/(dx*dx + dy*dy + dz*dz));
+    const double invdrsqd = invdrmag*invdrmag;
+    const double mc3 = mc*invdrsqd*invdrmag;
+    f[0] += mc*invdrmag;
+    f[1] += mc3*dx; f[2] += mc3*dy; f[3] += mc3*dz;
+    f[4] += mc3*(3.0*dx*dx*invdrsqd - 1.0);
+    f[5] += mc3*(3.0*dx*dy*invdrsqd);
+    f[6] += mc3*(3.0*dy*dy*invdrsqd - 1.0);
+    f[7] += mc3*(3.0*dz*dx*invdrsqd);
+    f[8] += mc3*(3.0*dz*dy*invdrsqd);
+    f[9] += mc3*(3.0*dz*dz*invdrsqd - 1.0);
+  }
+  if (!quad) return;
+  // the quadrupole part is written with d = expansion point - cell
+  const double ex = -dx, ey = -dy, ez = -dz;
+  const double invdrsqd = 1.0/(ex*ex + ey*ey + ez*ez + 1.0e-20);
+  const double invdrmag = sqrt(invdrsqd);
+  const double invdr5 = invdrsqd*invdrsqd*invdrmag;
+  const double qzz = -(q[0] + q[2]);
+  const double qscalar = q[0]*ex*ex + q[2]*ey*ey + qzz*ez*ez +
+                         2.0*(q[1]*ex*ey + q[3]*ex*ez + q[4]*ey*ez);
+  const double qfactor = 2.5*qscalar*invdr5*invdrsqd;
+  double qx = (q[0]*ex + q[1]*ey + q[3]*ez)*invdr5;
+  double qy = (q[1]*ex + q[2]*ey + q[4]*ez)*invdr5;
+  double qz = (q[3]*ex + q[4]*ey + qzz*ez)*invdr5;
+  f[0] += 0.5*qscalar*invdr5;
+  f[1] += qx - qfactor*ex; f[2] += qy - qfactor*ey; f[3] += qz - qfactor*ez;
+  const double s5 = 5.0*invdrsqd;
+  qx *= s5; qy *= s5; qz *= s5;
+  f[4] += qfactor*(7.0*ex*ex*invdrsqd - 1.0) - (2.0*qx*ex - q[0]*invdr5);
+  f[5] += qfactor*(7.0*ex*ey*invdrsqd) - (qx*ey + qy*ex - q[1]*invdr5);
+  f[6] += qfactor*(7.0*ey*ey*invdrsqd - 1.0) - (2.0*qy*ey - q[2]*invdr5);
+  f[7] += qfactor*(7.0*ex*ez*invdrsqd) - (qx*ez + qz*ex - q[3]*invdr5);
+  f[8] += qfactor*(7.0*ey*ez*invdrsqd) - (qy*ez + qz*ey - q[4]*invdr5);
+  f[9] += qfactor*(7.0*ez*ez*invdrsqd - 1.0) - (2.0*qz*ez - qzz*invdr5);
+}
+
 // SOFT: the walk also evaluates the kernel-softened neighbour pairs (no hydro pass ran).
 // Otherwise the hydro pass has already written the softened gravity of every scatter-gather
 // pair into atree/gpot (GradhSph::ComputeSphHydroGravForces), the walk masks exactly those pairs
@@ -134,8 +179,16 @@ __device__ __forceinline__ void quad_term(T dx, T dy, T dz, T mc, const T* q, T&
 // there is no self potential.  EWALD (periodic box with self-gravity): every particle of the
 // walk's buckets (the hydro pass's neighbours and the target itself included) and every accepted
 // cell adds its periodic correction at its separation, as GradhSphTree::UpdateAllSphForces does
-// for all its neighbours and cells (GradhSphTree.cpp:558-578).
-template <bool MIXED, bool SOFT, bool PLUMMER, bool EWALD = false>
+// for all its neighbours and cells (GradhSphTree.cpp:558-578).  FAST (multipole fast_monopole or
+// fast_quadrupole, gas walks): the accepted cells are summed once per group into the Taylor
+// coefficients about the group's centre of mass (each lane takes the cells of its batch slot,
+// the lanes' sums are reduced at the end of the walk) and every target applies the first-order
+// series (ComputeFastMonopoleForces / ComputeFastQuadrupoleForces, NeighbourSearch.h:769-830).
+// A cell is then accepted only if its centre of mass also lies beyond the group's extent about
+// that point divided by theta, so that the series converges for every target.  multipole
+// monopole: the cells' moments are staged as zeros (the quadrupole term reduces to
+// ComputeCellMonopoleForces, NeighbourSearch.h:350-377).
+template <bool MIXED, bool SOFT, bool PLUMMER, bool EWALD = false, bool FAST = false>
 __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
   __shared__ int pbuf[PCAP];
   __shared__ int cbuf[CCAP];
@@ -170,6 +223,20 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
     }
     const double4 gc = a.geo_c[G], gh = a.geo_h[G];
     const double hmaxG = gc.w;
+    // fast multipoles: expansion point (group centre of mass, relative to the group centre), the
+    // squared distance beyond which a cell's series converges for the whole group, and this
+    // lane's share of the coefficients
+    double rcx = 0.0, rcy = 0.0, rcz = 0.0, rg2t = 0.0;
+    double fc[10];
+    if (FAST) {
+#pragma clang fp contract(off)
+      const double* mG = a.mp + 10*(size_t) G;
+      rcx = mG[0] - gc.x; rcy = mG[1] - gc.y; rcz = mG[2] - gc.z;
+      const double ex = fabs(rcx) + gh.x, ey = fabs(rcy) + gh.y, ez = fabs(rcz) + gh.z;
+      rg2t = (ex*ex + ey*ey + ez*ez)/a.p.thetamaxsqd;
+      for (int k = 0; k < 10; k++) fc[k] = 0.0;
+    }
+    const bool mono = !PLUMMER && a.p.multipole == GDF_MULTIPOLE_MONOPOLE;
     double4 me = make_double4(0, 0, 0, 0), mr = make_double4(1, 0, 0, 0);
     if (has) { me = a.posm[i]; mr = a.grec[i]; }
     const double r2i = range2_rn(krsqd, mr.x);
@@ -216,7 +283,15 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
           const double ey = fmax(fabs(dy) - gh.y, 0.0);
           const double ez = fmax(fabs(dz) - gh.z, 0.0);
           const double dmac2 = ex*ex + ey*ey + ez*ez;
-          if (dmac2 >= nh.w) {
+          bool accept = dmac2 >= nh.w;
+          if (FAST && accept) {
+            const double* mc = a.mp + 10*(size_t) node;
+            double cx = mc[0] - gc.x, cy = mc[1] - gc.y, cz = mc[2] - gc.z;
+            nearest_image<double>(a.p, cx, cy, cz);
+            cx -= rcx; cy -= rcy; cz -= rcz;
+            accept = cx*cx + cy*cy + cz*cz >= rg2t;
+          }
+          if (accept) {
             if (ec == 1) near = true; else cell = true;
           } else {
             if (bucket) near = true; else open = true;
@@ -255,32 +330,44 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
           const int c = cbuf[(chead + lane) & (CCAP - 1)];
           const double* m = a.mp + 10*(size_t) c;
           double cx = m[0], cy = m[1], cz = m[2];
+          if (FAST) {
+#pragma clang fp contract(off)
+            // the cell relative to the expansion point, with the node test's replica
+            double dx = cx - gc.x, dy = cy - gc.y, dz = cz - gc.z;
+            nearest_image<double>(a.p, dx, dy, dz);
+            fast_add(fc, dx - rcx, dy - rcy, dz - rcz, m[3], m + 4,
+                     a.p.multipole == GDF_MULTIPOLE_FAST_QUADRUPOLE);
+          }
           if (a.p.periodic) {
             // nearest periodic replica of the cell relative to the group centre
             double dx = cx - gc.x, dy = cy - gc.y, dz = cz - gc.z;
             nearest_image<double>(a.p, dx, dy, dz);
             cx = gc.x + dx; cy = gc.y + dy; cz = gc.z + dz;
           }
+          const double qs = mono ? 0.0 : 1.0;      // monopole: no quadrupole moments
           if (MIXED) {
             s4a[lane] = make_float4((float) (cx - gc.x), (float) (cy - gc.y),
                                     (float) (cz - gc.z), (float) m[3]);
-            s4b[lane] = make_float4((float) m[4], (float) m[5], (float) m[6], (float) m[7]);
-            s1[lane] = (float) m[8];
+            s4b[lane] = make_float4((float) (qs*m[4]), (float) (qs*m[5]), (float) (qs*m[6]),
+                                    (float) (qs*m[7]));
+            s1[lane] = (float) (qs*m[8]);
           } else {
-            sd[0][lane] = cx; sd[1][lane] = cy; sd[2][lane] = cz;
-            for (int k = 3; k < 9; k++) sd[k][lane] = m[k];
+            sd[0][lane] = cx; sd[1][lane] = cy; sd[2][lane] = cz; sd[3][lane] = m[3];
+            for (int k = 4; k < 9; k++) sd[k][lane] = qs*m[k];
           }
         }
         __syncthreads();
-        if (has) {
+        // fast multipoles leave only the periodic corrections to the per-target sweep
+        if (has && (!FAST || EWALD)) {
           if (MIXED) {
             float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
             #pragma unroll 4
             for (int q = 0; q < nb; q++) {
               const float4 c = s4a[q], qa = s4b[q];
               const float qq[5] = {qa.x, qa.y, qa.z, qa.w, s1[q]};
-              quad_term<float>(rxi - c.x, ryi - c.y, rzi - c.z, c.w, qq, bx, by, bz, bp,
-                               PLUMMER ? eps2f : 1.0e-20f);
+              if (!FAST)
+                quad_term<float>(rxi - c.x, ryi - c.y, rzi - c.z, c.w, qq, bx, by, bz, bp,
+                                 PLUMMER ? eps2f : 1.0e-20f);
               if (EWALD) ewald_corr<float>(a.p, c.x - rxi, c.y - ryi, c.z - rzi, c.w, bx, by, bz,
                                            bp);
             }
@@ -288,8 +375,9 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
           } else {
             for (int q = 0; q < nb; q++) {
               const double qq[5] = {sd[4][q], sd[5][q], sd[6][q], sd[7][q], sd[8][q]};
-              quad_term<double>(me.x - sd[0][q], me.y - sd[1][q], me.z - sd[2][q], sd[3][q], qq,
-                                ax, ay, az, pot, PLUMMER ? eps2 : 1.0e-20);
+              if (!FAST)
+                quad_term<double>(me.x - sd[0][q], me.y - sd[1][q], me.z - sd[2][q], sd[3][q],
+                                  qq, ax, ay, az, pot, PLUMMER ? eps2 : 1.0e-20);
               if (EWALD) ewald_corr<double>(a.p, sd[0][q] - me.x, sd[1][q] - me.y,
                                             sd[2][q] - me.z, sd[3][q], ax, ay, az, pot);
             }
@@ -381,8 +469,25 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
       if (lane == 0) atomicOr(&a.flags[2], 1);
       continue;
     }
+    if (FAST) {
+      // the lanes' coefficient sums, reduced by a butterfly (every lane ends with the same sums)
+      for (int k = 0; k < 10; k++) {
+        double v = fc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        fc[k] = v;
+      }
+    }
     if (has) {
       if (MIXED) { ax = (double) fx.s; ay = (double) fy.s; az = (double) fz.s; pot = (double) fp.s; }
+      if (FAST) {
+        // ApplyForcesTaylor (NeighbourSearch.h:735-760), first order about the expansion point
+        const double* mG = a.mp + 10*(size_t) G;
+        const double d0 = me.x - mG[0], d1 = me.y - mG[1], d2 = me.z - mG[2];
+        ax += fc[1] + fc[4]*d0 + fc[5]*d1 + fc[7]*d2;
+        ay += fc[2] + fc[5]*d0 + fc[6]*d1 + fc[8]*d2;
+        az += fc[3] + fc[7]*d0 + fc[8]*d1 + fc[9]*d2;
+        pot += fc[0] + fc[1]*d0 + fc[2]*d1 + fc[3]*d2;
+      }
       if (SOFT) {
         a.atx[i] = ax; a.aty[i] = ay; a.atz[i] = az;
         a.gpot[i] = pot0 + pot;
@@ -398,7 +503,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
       a.gnc[g] = ncell;
       a.gnn[g] = nnear;
       atomicAdd((unsigned long long*) &a.icount[2], (unsigned long long) nnear*nact);
-      atomicAdd((unsigned long long*) &a.icount[3], (unsigned long long) ncell*nact);
+      atomicAdd((unsigned long long*) &a.icount[3], (unsigned long long) ncell*(FAST ? 1 : nact));
     }
   }
 }
@@ -562,6 +667,15 @@ int nbody_direct_stars(gdf_ctx* c, int smoothed) {
 
 namespace {
 
+// gas walks: the fast-multipole variant when the multipole option asks for it
+template <bool MIXED, bool SOFT, bool EWALD>
+void launch_gas(const GravArgs& a, int grid, hipStream_t s) {
+  if (a.p.multipole >= GDF_MULTIPOLE_FAST_MONOPOLE)
+    k_gravity<MIXED, SOFT, false, EWALD, true><<<grid, kWave, 0, s>>>(a);
+  else
+    k_gravity<MIXED, SOFT, false, EWALD, false><<<grid, kWave, 0, s>>>(a);
+}
+
 GravArgs grav_args(gdf_ctx* c) {
   GravArgs a;
   a.N = c->N; a.ngroups = c->ngroups; a.nleaf = c->phys.nleaf;
@@ -623,11 +737,11 @@ int gravity_launch(gdf_ctx* c) {
   const int kt = ktime_begin(c, s2, 5);
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
   if (c->phys.ewald) {
-    if (mixed) k_gravity<true, false, false, true><<<grid, kWave, 0, s2>>>(a);
-    else k_gravity<false, false, false, true><<<grid, kWave, 0, s2>>>(a);
+    if (mixed) launch_gas<true, false, true>(a, grid, s2);
+    else launch_gas<false, false, true>(a, grid, s2);
   } else {
-    if (mixed) k_gravity<true, false, false><<<grid, kWave, 0, s2>>>(a);
-    else k_gravity<false, false, false><<<grid, kWave, 0, s2>>>(a);
+    if (mixed) launch_gas<true, false, false>(a, grid, s2);
+    else launch_gas<false, false, false>(a, grid, s2);
   }
   GDF_HIP(c, hipGetLastError());
   ktime_end(c, s2, kt);
@@ -673,18 +787,18 @@ int gravity_walk(gdf_ctx* c) {
     else k_gravity<false, true, true><<<grid, kWave, 0, s>>>(a);
   } else if (c->phys.hydro_forces) {
     if (c->phys.ewald) {
-      if (mixed) k_gravity<true, false, false, true><<<grid, kWave, 0, s>>>(a);
-      else k_gravity<false, false, false, true><<<grid, kWave, 0, s>>>(a);
+      if (mixed) launch_gas<true, false, true>(a, grid, s);
+      else launch_gas<false, false, true>(a, grid, s);
     } else {
-      if (mixed) k_gravity<true, false, false><<<grid, kWave, 0, s>>>(a);
-      else k_gravity<false, false, false><<<grid, kWave, 0, s>>>(a);
+      if (mixed) launch_gas<true, false, false>(a, grid, s);
+      else launch_gas<false, false, false>(a, grid, s);
     }
   } else if (c->phys.ewald) {
-    if (mixed) k_gravity<true, true, false, true><<<grid, kWave, 0, s>>>(a);
-    else k_gravity<false, true, false, true><<<grid, kWave, 0, s>>>(a);
+    if (mixed) launch_gas<true, true, true>(a, grid, s);
+    else launch_gas<false, true, true>(a, grid, s);
   } else {
-    if (mixed) k_gravity<true, true, false><<<grid, kWave, 0, s>>>(a);
-    else k_gravity<false, true, false><<<grid, kWave, 0, s>>>(a);
+    if (mixed) launch_gas<true, true, false>(a, grid, s);
+    else launch_gas<false, true, false>(a, grid, s);
   }
   GDF_HIP(c, hipGetLastError());
   ktime_end(c, s, kt);

@@ -82,6 +82,7 @@ struct PhysParams {
   double eps2;               // Plummer softening length squared of the N-body tree walk
   double nbody_mult;
   int hrho_itmax;            // fixed-point iterations before bisection (reference: 30)
+  int multipole;             // GDF_MULTIPOLE_*: the gravity walk's accepted-cell terms
   // Ewald periodic gravity (periodic box with self-gravity): correction table of eng[0] x
   // eng[1] x eng[2] nodes, x fastest, each (pot, ax, ay, az) of a unit mass; node spacing 1/edI
   int ewald;

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "libgandalf_gpu.so")
 GDF_KERNEL = {"m4": 0, "quintic": 1}
 GDF_EOS = {"energy_eqn": 0, "isothermal": 1, "barotropic": 2}
 PREC = {"fp64": 0, "mixed": 1}
+# the reference's "multipole" parameter values (Parameters.cpp:309), as GDF_MULTIPOLE_*
+MULTIPOLE = {"quadrupole": 0, "monopole": 1, "fast_monopole": 2, "fast_quadrupole": 3}
 FIELDS3 = ("r", "v", "a", "atree", "r0", "v0", "a0", "adot")
 
 
@@ -47,7 +49,7 @@ class Config(C.Structure):
         ("n_ewald_grid", C.c_int), ("gr_bhewaldseriesn", C.c_int),
         ("ewald_mult", C.c_double), ("efratio", C.c_double),
         ("ntreebuildstep", C.c_int), ("ntreestockstep", C.c_int),
-        ("graph_capture", C.c_int),
+        ("graph_capture", C.c_int), ("multipole", C.c_int),
     ]
 
 
@@ -177,6 +179,7 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
     c.ntreebuildstep = int(prm.get("ntreebuildstep", 1))
     c.ntreestockstep = int(prm.get("ntreestockstep", 1))
     c.graph_capture = int(prm.get("graph_capture", 0))
+    c.multipole = MULTIPOLE[prm.get("multipole", "quadrupole")]
     if c.nbody and c.nbody_eps <= 0.0:
         c.nbody_direct = 2 if c.nbody_softening else 1
     c.nbody_mult = float(prm.get("nbody_mult", 0.1))

@@ -95,7 +95,18 @@ typedef struct gdf_config {
    * default): eager launches.  The results are the same bit for bit.  For a context driven by
    * one host thread with no other thread using the device meanwhile; serving pools refuse it. */
   int graph_capture;
+  /* "multipole" (Parameters.cpp:309): the gravity walk's accepted-cell terms.
+   * GDF_MULTIPOLE_QUADRUPOLE (the default) and GDF_MULTIPOLE_MONOPOLE evaluate every cell at
+   * every target (ComputeCellQuadrupoleForces / ComputeCellMonopoleForces); the fast options sum
+   * the cells once per walk group at the group's centre of mass and apply a first-order Taylor
+   * series to each target (ComputeFastMonopoleForces / ComputeFastQuadrupoleForces,
+   * GradhSphTree.cpp:587-593).  A cell is then accepted only beyond the group's extent about
+   * that point divided by theta.  Gas walks only: the N-body tree walk keeps quadrupoles. */
+  int multipole;
 } gdf_config;
+
+enum { GDF_MULTIPOLE_QUADRUPOLE = 0, GDF_MULTIPOLE_MONOPOLE = 1, GDF_MULTIPOLE_FAST_MONOPOLE = 2,
+       GDF_MULTIPOLE_FAST_QUADRUPOLE = 3 };
 
 typedef struct gdf_ctx gdf_ctx;
 

@@ -1433,6 +1433,99 @@ void orc_cell_quad_force(const double* com, const double* q, double mc, const do
   quad_force(com, q, mc, rp, agrav, gpot);
 }
 
+/* Monopole cell force (ComputeCellMonopoleForces, NeighbourSearch.h:350-377): the cell's mass at
+ * its centre of mass, dr = cell.r - rp */
+static inline void mono_force(const double* com, double mc, const double* rp, double* agrav,
+                              double* gpot) {
+  double dr[3];
+  for (int k=0; k<3; k++) dr[k] = com[k] - rp[k];
+  const double drsqd = dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2] + SMALL_NUMBER;
+  const double invdrsqd = 1.0/drsqd;
+  const double invdrmag = sqrt(invdrsqd);
+  const double invdr3 = invdrsqd*invdrmag;
+  *gpot += mc*invdrmag;
+  for (int k=0; k<3; k++) agrav[k] += mc*dr[k]*invdr3;
+}
+
+void orc_cell_mono_force(const double* com, double mc, const double* rp, double* agrav,
+                         double* gpot) {
+  mono_force(com, mc, rp, agrav, gpot);
+}
+
+/* FastMultipoleForces (NeighbourSearch.h:478-760) with its second-order terms off (HoT = false):
+ * the accepted cells' potential, acceleration and tidal tensor at one expansion point rc,
+ * summed once per walk group and applied to each target as a first-order Taylor series about rc.
+ * f[0] = pot, f[1..3] = acceleration (the reference's dphi takes the same increments as ac, so
+ * one vector serves both), f[4..9] = tensor (xx, xy, yy, xz, yz, zz).
+ * dr = cell.r - rc; quad = 0 adds the monopole part only (AddMonopoleContribution, :564-597),
+ * quad = 1 the monopole and quadrupole parts (AddQuadrupoleContribution, :675-723). */
+static void fast_add(double* f, const double* dr, double mc, const double* q, int quad) {
+  {
+    const double invdrmag = sqrt(1.0/(dr[0]*dr[0] + dr[1]*dr[1] + dr[2]*dr[2]));
+    const double invdrsqd = invdrmag*invdrmag;
+    const double invdr3 = invdrsqd*invdrmag;
+    f[0] += mc*invdrmag;
+    const double mc3 = mc*invdr3;
+    for (int k=0; k<3; k++) f[1+k] += mc3*dr[k];
+    f[4] += mc3*(3.0*dr[0]*dr[0]*invdrsqd - 1);
+    f[5] += mc3*(3.0*dr[0]*dr[1]*invdrsqd);
+    f[6] += mc3*(3.0*dr[1]*dr[1]*invdrsqd - 1);
+    f[7] += mc3*(3.0*dr[2]*dr[0]*invdrsqd);
+    f[8] += mc3*(3.0*dr[2]*dr[1]*invdrsqd);
+    f[9] += mc3*(3.0*dr[2]*dr[2]*invdrsqd - 1);
+  }
+  if (!quad) return;
+  double d[3];
+  for (int k=0; k<3; k++) d[k] = -dr[k];          /* rc - cell.r */
+  const double drsqd = d[0]*d[0] + d[1]*d[1] + d[2]*d[2] + SMALL_NUMBER;
+  const double invdrsqd = 1.0/drsqd;
+  const double invdrmag = sqrt(invdrsqd);
+  const double invdr5 = invdrsqd*invdrsqd*invdrmag;
+  const double qzz = -(q[0] + q[2]);
+  const double qscalar = q[0]*d[0]*d[0] + q[2]*d[1]*d[1] + qzz*d[2]*d[2] +
+    2.0*(q[1]*d[0]*d[1] + q[3]*d[0]*d[2] + q[4]*d[1]*d[2]);
+  const double qfactor = 2.5*qscalar*invdr5*invdrsqd;
+  double qx[3] = {(q[0]*d[0] + q[1]*d[1] + q[3]*d[2])*invdr5,
+                  (q[1]*d[0] + q[2]*d[1] + q[4]*d[2])*invdr5,
+                  (q[3]*d[0] + q[4]*d[1] + qzz*d[2])*invdr5};
+  f[0] += 0.5*qscalar*invdr5;
+  for (int k=0; k<3; k++) f[1+k] += qx[k] - qfactor*d[k];
+  for (int k=0; k<3; k++) qx[k] *= 5*invdrsqd;
+  f[4] += qfactor*(7*d[0]*d[0]*invdrsqd - 1);
+  f[5] += qfactor*(7*d[0]*d[1]*invdrsqd);
+  f[6] += qfactor*(7*d[1]*d[1]*invdrsqd - 1);
+  f[7] += qfactor*(7*d[0]*d[2]*invdrsqd);
+  f[8] += qfactor*(7*d[1]*d[2]*invdrsqd);
+  f[9] += qfactor*(7*d[2]*d[2]*invdrsqd - 1);
+  f[4] -= 2.0*qx[0]*d[0] - q[0]*invdr5;
+  f[5] -= qx[0]*d[1] + qx[1]*d[0] - q[1]*invdr5;
+  f[6] -= 2.0*qx[1]*d[1] - q[2]*invdr5;
+  f[7] -= qx[0]*d[2] + qx[2]*d[0] - q[3]*invdr5;
+  f[8] -= qx[1]*d[2] + qx[2]*d[1] - q[4]*invdr5;
+  f[9] -= 2.0*qx[2]*d[2] - qzz*invdr5;
+}
+
+/* ApplyForcesTaylor (NeighbourSearch.h:735-760, first order): d = r - rc */
+static inline void fast_apply(const double* f, const double* d, double* agrav, double* gpot) {
+  agrav[0] += f[1] + f[4]*d[0] + f[5]*d[1] + f[7]*d[2];
+  agrav[1] += f[2] + f[5]*d[0] + f[6]*d[1] + f[8]*d[2];
+  agrav[2] += f[3] + f[7]*d[0] + f[8]*d[1] + f[9]*d[2];
+  *gpot += f[0] + f[1]*d[0] + f[2]*d[1] + f[3]*d[2];
+}
+
+/* the fast expansion of n cells (com n x 3 relative to rc, q n x 5, m n) evaluated at the
+ * points d (np x 3, relative to rc): for its known-answer tests */
+void orc_fast_multipole(int n, const double* dcom, const double* q, const double* m, int quad,
+                        int np, const double* d, double* coef, double* agrav, double* gpot) {
+  for (int k=0; k<10; k++) coef[k] = 0.0;
+  for (int c=0; c<n; c++) fast_add(coef, &dcom[3*c], m[c], &q[5*c], quad);
+  for (int i=0; i<np; i++) {
+    agrav[3*i] = agrav[3*i+1] = agrav[3*i+2] = 0.0;
+    gpot[i] = 0.0;
+    fast_apply(coef, &d[3*i], &agrav[3*i], &gpot[i]);
+  }
+}
+
 #define GRAV_NEIB_SLACK (1.0 + 1.0e-4)
 
 /* act (optional, input order): groups without an active particle are skipped (counts 0) and
@@ -1470,6 +1563,18 @@ static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const do
         halfG[k] = 0.5*(t->bbmax[3*G+k] - t->bbmin[3*G+k]);
       }
       const double hmaxG = t->hmax[G];
+      /* fast multipoles: the expansion point is the group's centre of mass (as the reference's
+         is its walking leaf cell's, GradhSphTree.cpp:587-593); an accepted cell must also lie
+         beyond the group's own extent about that point divided by theta, so that the Taylor
+         series converges for every target of the group */
+      const int fast = p->multipole >= 2;
+      double rcr[3], rg2 = 0.0;
+      for (int k=0; k<3; k++) {
+        rcr[k] = t->com[3*G+k] - cG[k];
+        const double e = fabs(rcr[k]) + halfG[k];
+        rg2 += e*e;
+      }
+      const double rg2t = rg2/p->thetamaxsqd;
       int sp = 0, nc = 0, np = 0;
       stack[sp++] = 0;   /* root: internal 0, or the single leaf when N == 1 */
       while (sp) {
@@ -1498,7 +1603,16 @@ static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const do
             double ek = fabs(dr[k]) - halfG[k];
             if (ek > 0.0) dmac2 += ek*ek;
           }
-          if (dmac2 >= t->cdistsqd[c]) {
+          int accept = dmac2 >= t->cdistsqd[c];
+          if (accept && fast) {
+            double dc[3];
+            for (int k=0; k<3; k++) dc[k] = t->com[3*c+k] - cG[k];
+            nearest(&b, dc);
+            double dcom2 = 0.0;
+            for (int k=0; k<3; k++) { dc[k] -= rcr[k]; dcom2 += dc[k]*dc[k]; }
+            accept = dcom2 >= rg2t;
+          }
+          if (accept) {
             if (cnt == 1) leafadd = 1; else cells[nc++] = c;
           } else {
             if (cnt <= nleaf || c >= N - 1 || N == 1) leafadd = 1; else open = 1;
@@ -1514,6 +1628,17 @@ static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const do
       }
       if (ncell) ncell[g] = nc;
       if (nnear) nnear[g] = np;
+      double fc[10] = {0.0};
+      if (fast) {
+        for (int q=0; q<nc; q++) {
+          const int c = cells[q];
+          double dc[3];
+          for (int k=0; k<3; k++) dc[k] = t->com[3*c+k] - cG[k];
+          nearest(&b, dc);
+          for (int k=0; k<3; k++) dc[k] -= rcr[k];
+          fast_add(fc, dc, t->mass[c], &t->quad[5*c], p->multipole == 3);
+        }
+      }
       for (int tt=0; tt<t->group_count[g]; tt++) {
         const int i = t->perm[t->group_first[g] + tt];
         if (act && !act[i]) continue;
@@ -1526,7 +1651,7 @@ static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const do
           grav_pair(&kern, krsqd, d, m[j], h[i], h[j], zeta[i], zeta[j], hfactor[i],
                     hfactor[j], acc, &pot);
         }
-        for (int q=0; q<nc; q++) {
+        for (int q=0; q<nc && !fast; q++) {
           const int c = cells[q];
           double rp[3] = {r[3*i], r[3*i+1], r[3*i+2]};
           if (b.periodic) {
@@ -1535,7 +1660,13 @@ static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const do
             nearest(&b, d);
             for (int k=0; k<3; k++) rp[k] = t->com[3*c+k] - d[k];
           }
-          quad_force(&t->com[3*c], &t->quad[5*c], t->mass[c], rp, acc, &pot);
+          if (p->multipole == 1) mono_force(&t->com[3*c], t->mass[c], rp, acc, &pot);
+          else quad_force(&t->com[3*c], &t->quad[5*c], t->mass[c], rp, acc, &pot);
+        }
+        if (fast) {
+          const double d[3] = {r[3*i] - t->com[3*G], r[3*i+1] - t->com[3*G+1],
+                               r[3*i+2] - t->com[3*G+2]};
+          fast_apply(fc, d, acc, &pot);
         }
         for (int k=0; k<3; k++) atree[3*i+k] = acc[k];
         gpot[i] = pot;

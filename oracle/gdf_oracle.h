@@ -35,6 +35,8 @@ typedef struct {
   int level_diff_max;      /* max. level difference to neighbours (Simulation.cpp:1209) */
   int grav_direct;         /* 1: the sim driver's gravity is the exact direct sum, as the
                               reference's neib_search = bruteforce (BruteForceTree.cpp:214) */
+  int multipole;           /* cell terms of the gravity walk (Parameters.cpp:309): 0 quadrupole,
+                              1 monopole, 2 fast_monopole, 3 fast_quadrupole */
 } orc_params;
 
 enum { ORC_W0 = 0, ORC_W1, ORC_WOMEGA, ORC_WZETA, ORC_WGRAV, ORC_WPOT,
@@ -167,6 +169,15 @@ void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
  * and traceless moments q[5] acting on a point rp; adds to agrav[3] and *gpot */
 void orc_cell_quad_force(const double* com, const double* q, double mc, const double* rp,
                          double* agrav, double* gpot);
+/* one monopole cell term (ComputeCellMonopoleForces, NeighbourSearch.h:350-377) */
+void orc_cell_mono_force(const double* com, double mc, const double* rp, double* agrav,
+                         double* gpot);
+/* the fast multipole expansion (FastMultipoleForces, NeighbourSearch.h:478-760, first order) of
+ * n cells at dcom (n x 3, relative to the expansion point), moments q (n x 5), masses m; quad = 0
+ * monopole parts only, 1 with the quadrupole parts.  coef[10] = (pot, a[3], tensor xx xy yy xz
+ * yz zz); agrav (np x 3) and gpot (np) = the series at the points d (np x 3, relative) */
+void orc_fast_multipole(int n, const double* dcom, const double* q, const double* m, int quad,
+                        int np, const double* d, double* coef, double* agrav, double* gpot);
 
 /* ---- integrator (SphLeapfrogKDK.cpp:76-272) and timesteps (SphIntegration.cpp:81-134) ---- */
 void orc_advance(int N, double dt, int energy_integration, const double* r0,

@@ -31,6 +31,7 @@ class Params(C.Structure):
         ("boxmin", C.c_double * 3), ("boxmax", C.c_double * 3),
         ("courant_mult", C.c_double), ("accel_mult", C.c_double), ("energy_mult", C.c_double),
         ("nlevels", C.c_int), ("level_diff_max", C.c_int), ("grav_direct", C.c_int),
+        ("multipole", C.c_int),
     ]
 
 
@@ -145,6 +146,8 @@ def lib():
         L.orc_tree_refit_h.argtypes = [pp, C.POINTER(Tree), vp]
         L.orc_tree_gravity.argtypes = [pp, C.POINTER(Tree)] + [vp] * 9
         L.orc_cell_quad_force.argtypes = [vp, vp, d, vp, vp, vp]
+        L.orc_cell_mono_force.argtypes = [vp, d, vp, vp, vp]
+        L.orc_fast_multipole.argtypes = [i, vp, vp, vp, i, i, vp, vp, vp, vp]
         L.orc_timesteps.argtypes = [pp, i] + [vp] * 7
         L.orc_timesteps.restype = d
         L.orc_diagnostics.argtypes = [pp, i] + [vp] * 6
@@ -169,6 +172,10 @@ def lib():
         L.orc_sim_set_stars.restype = i
         _lib = L
     return _lib
+
+
+# the reference's "multipole" parameter values (Parameters.cpp:309)
+MULTIPOLE = {"quadrupole": 0, "monopole": 1, "fast_monopole": 2, "fast_quadrupole": 3}
 
 
 def params_from_dict(prm: dict) -> Params:
@@ -201,6 +208,7 @@ def params_from_dict(prm: dict) -> Params:
     p.nlevels = int(prm.get("Nlevels", 1))
     p.level_diff_max = int(prm.get("level_diff_max", 1))
     p.grav_direct = 1 if prm.get("neib_search", "kdtree") == "bruteforce" else 0
+    p.multipole = MULTIPOLE[prm.get("multipole", "quadrupole")]
     return p
 
 
@@ -320,6 +328,26 @@ def nbody_direct_smoothed(p: Params, r, v, m, h):
     lib().orc_nbody_direct_smoothed(C.byref(p), N, _p(r), _p(v), _p(m), _p(h), _p(a), _p(ad),
                                     _p(gp))
     return a, ad, gp
+
+
+def cell_mono_force(com, mc, rp):
+    """The oracle's monopole cell term at one point: (acceleration, potential)."""
+    a = np.zeros(3)
+    pot = np.zeros(1)
+    lib().orc_cell_mono_force(_p(_d(com)), float(mc), _p(_d(rp)), _p(a), _p(pot))
+    return a, float(pot[0])
+
+
+def fast_multipole(dcom, q, m, quad, d):
+    """The fast multipole expansion of cells (centres dcom relative to the expansion point,
+    moments q, masses m) evaluated at points d relative to it: (coef[10], acc, pot)."""
+    dcom = _d(np.atleast_2d(dcom)); q = _d(np.atleast_2d(q)); m = _d(np.atleast_1d(m))
+    d = _d(np.atleast_2d(d))
+    coef = np.zeros(10)
+    acc = np.zeros((d.shape[0], 3)); pot = np.zeros(d.shape[0])
+    lib().orc_fast_multipole(dcom.shape[0], _p(dcom), _p(q), _p(m), int(quad), d.shape[0],
+                             _p(d), _p(coef), _p(acc), _p(pot))
+    return coef, acc, pot
 
 
 def cell_quad_force(com, q, mc, rp):

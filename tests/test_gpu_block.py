@@ -41,6 +41,8 @@ def run_pair(case, precision="fp64", nsteps=None, over=None):
     ("blob16_m4_blk", {}),
     # the cloud through the device tree and the oracle's tree twin (same cells and MAC)
     ("cloud4k_quintic_blk", {"neib_search": "kdtree"}),
+    # the same with the fast multipole expansion per walk group
+    ("cloud4k_quintic_blk", {"neib_search": "kdtree", "multipole": "fast_quadrupole"}),
 ])
 def test_block_steps_match_oracle(case, over):
     s, o, states = run_pair(case, over=over)
