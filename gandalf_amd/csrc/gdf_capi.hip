@@ -104,7 +104,7 @@ void free_particles(gdf_ctx* c) {
                   c->first, c->last, c->parent, c->visit, c->front, c->bbmin, c->bbmax, c->geo_c,
                   c->geo_h, c->mp, c->gstart, c->grp_first, c->grp_count, c->grp_node, c->nsel,
                   c->sel_tmp, c->stacks, c->grav_nc,
-                  c->grav_nn, c->tmpd, c->grp_root, c->stacks2, c->glist};
+                  c->grav_nn, c->tmpd, c->grp_root, c->stacks2, c->glist, c->grp_gmac};
   for (void* p : ptrs) if (p) hipFree(p);
   c->iorig = c->iorig_alt = nullptr; c->posm = c->hrecA = c->hrecB = c->grec = nullptr;
   c->status = nullptr; c->keys = c->keys_alt = nullptr; c->vals = c->vals_alt = nullptr;
@@ -114,6 +114,7 @@ void free_particles(gdf_ctx* c) {
   c->stacks = nullptr;
   sph_free_lists(c);
   c->grav_nc = c->grav_nn = nullptr; c->tmpd = nullptr; c->grp_root = nullptr;
+  c->grp_gmac = nullptr;
   c->stacks2 = nullptr; c->glist = nullptr;
   c->Ncap = 0; c->N = 0;
 }
@@ -315,6 +316,7 @@ int gdf_config_default(gdf_config* cfg) {
   cfg->ntreebuildstep = 1; cfg->ntreestockstep = 1;
   cfg->graph_capture = 0;
   cfg->multipole = GDF_MULTIPOLE_QUADRUPOLE;
+  cfg->gravity_mac = GDF_MAC_GEOMETRIC; cfg->macerror = 1.0e-4;
   return GDF_OK;
 }
 
@@ -341,7 +343,9 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
     return GDF_ERR_INVALID;
   if (cfg->ntreebuildstep < 1 || cfg->ntreestockstep < 1 ||
       (cfg->graph_capture != 0 && cfg->graph_capture != 1) ||
-      cfg->multipole < GDF_MULTIPOLE_QUADRUPOLE || cfg->multipole > GDF_MULTIPOLE_FAST_QUADRUPOLE)
+      cfg->multipole < GDF_MULTIPOLE_QUADRUPOLE || cfg->multipole > GDF_MULTIPOLE_FAST_QUADRUPOLE ||
+      cfg->gravity_mac < GDF_MAC_GEOMETRIC || cfg->gravity_mac > GDF_MAC_EIGENMAC ||
+      !(cfg->macerror > 0.0))
     return GDF_ERR_INVALID;
   if ((cfg->nbody_softening != 0 && cfg->nbody_softening != 1) ||
       (cfg->conservative_sph_star_gravity != 0 && cfg->conservative_sph_star_gravity != 1))
@@ -384,6 +388,7 @@ int gdf_create(const gdf_config* cfg, gdf_ctx** out) {
   p.nbody_mult = cfg->nbody_mult;
   p.hrho_itmax = cfg->hrho_iteration_max > 0 ? cfg->hrho_iteration_max : 30;
   p.multipole = cfg->multipole;
+  p.gravity_mac = cfg->gravity_mac; p.macerror = cfg->macerror;
   c->nlevels = cfg->Nlevels; c->level_diff_max = cfg->level_diff_max;
   int rc = upload_kernel_tables(c);
   // a periodic box with self-gravity carries the Ewald correction (Simulation.cpp:1183-1198)
@@ -580,6 +585,18 @@ int gdf_build_tree_step(gdf_ctx* c, int rebuild_tree, int n, int ntreebuildstep,
   }
   GDF_TRY(tree_restock(c));
   return 0;
+}
+
+int gdf_set_opening_criterion(gdf_ctx* c, int mac) {
+  if (!c || mac < GDF_MAC_GEOMETRIC || mac > GDF_MAC_EIGENMAC) return GDF_ERR_INVALID;
+  c->phys.gravity_mac = mac;
+  c->cfg.gravity_mac = mac;
+  return GDF_OK;
+}
+
+int gdf_opening_criterion(const gdf_ctx* c) {
+  if (!c) return GDF_ERR_INVALID;
+  return c->phys.gravity_mac;
 }
 
 int gdf_maximum_smoothing_length(gdf_ctx* c, double* hmax) {

@@ -14,7 +14,8 @@
 //     particles) or opened;
 //   * geometric MAC (Tree.h:413-432) evaluated from the nearest point of G: accept the
 //     multipole iff dist(G, centre_cell)^2 >= cdistsqd = max(rmax^2, hmax^2)/theta^2, a
-//     conservative form of the reference's centre-to-centre test;
+//     conservative form of the reference's centre-to-centre test; the gadget2 and eigenmac
+//     criteria then open more cells, from the same distance (mac_opens);
 //   * otherwise open, or send the bucket's particles to the direct list.
 // As in the reference, the kernel-softened gravity of the scatter-gather neighbours is summed by
 // the hydro pass (GradhSph::ComputeSphHydroGravForces); the walk adds the direct 1/r^2 terms of
@@ -46,6 +47,7 @@ struct GravArgs {
   int32_t* gnc; int32_t* gnn;
   const int* act;         // block timesteps: only active targets (NULL = all)
   const int* grp_root;    // serving batches: walk from the group's segment subtree (NULL = root)
+  const double* gmac;     // per group: factor of the gadget2 / eigenmac criterion
   // non-NULL: the walk runs beside the hydro pass and writes its own sums here (acceleration,
   // potential, self term) instead of adding to the hydro pass's atx/gpot
   double* gwx; double* gwy; double* gwz; double* gwp; double* gwp0;
@@ -123,6 +125,25 @@ __device__ __forceinline__ void quad_term(T dx, T dy, T dz, T mc, const T* q, T&
   ay += -mono*dy + (q[1]*dx + q[2]*dy + q[4]*dz)*invdr5 - qfactor*dy;
   az += -mono*dz + (q[3]*dx + q[4]*dy - (q[0] + q[2])*dz)*invdr5 - qfactor*dz;
   pot += mc*invdrmag + T(0.5)*qscalar*invdr5;
+}
+
+// Tree::open_cell_for_gravity (Tree.h:413-432) past its geometric test, for a cell at squared
+// distance d2 from the walk group's box: gadget2 opens a cell whose m rmax^2 / d^4 exceeds
+// macerror times the group's smallest |atree| (gfac); eigenmac one nearer than
+// (0.5 lambda / macerror)^(2/3) (KDTree.cpp:1058-1079, lambda = 2 sqrt(-p/3) bounding the
+// quadrupole's largest eigenvalue) times the group's largest gpot^(-2/3) (gfac)
+__device__ __forceinline__ bool mac_opens(const PhysParams& p, const double4& nh, const double* mp,
+                                          double d2, double gfac) {
+#pragma clang fp contract(off)
+  if (p.gravity_mac == GDF_MAC_GADGET2) {
+    const double rmax = sqrt(nh.x*nh.x + nh.y*nh.y + nh.z*nh.z);
+    return d2*d2*gfac*p.macerror < rmax*rmax*mp[3];
+  }
+  const double* q = mp + 4;
+  const double pq = q[0]*q[2] - (q[0] + q[2])*(q[0] + q[2]) - q[1]*q[1] - q[3]*q[3] - q[4]*q[4];
+  const double lambda = pq >= 0.0 ? 0.0 : 2.0*sqrt(-pq/3.0);
+  const double mac = pow(0.5*lambda/p.macerror, 0.66666666666666);
+  return d2 < mac*gfac;
 }
 
 // FastMultipoleForces::AddMonopoleContribution / AddQuadrupoleContribution
@@ -237,6 +258,8 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
       for (int k = 0; k < 10; k++) fc[k] = 0.0;
     }
     const bool mono = !PLUMMER && a.p.multipole == GDF_MULTIPOLE_MONOPOLE;
+    const bool gmac = !PLUMMER && a.p.gravity_mac != GDF_MAC_GEOMETRIC;
+    const double gfac = gmac ? a.gmac[g] : 0.0;
     double4 me = make_double4(0, 0, 0, 0), mr = make_double4(1, 0, 0, 0);
     if (has) { me = a.posm[i]; mr = a.grec[i]; }
     const double r2i = range2_rn(krsqd, mr.x);
@@ -284,6 +307,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
           const double ez = fmax(fabs(dz) - gh.z, 0.0);
           const double dmac2 = ex*ex + ey*ey + ez*ez;
           bool accept = dmac2 >= nh.w;
+          if (gmac && accept) accept = !mac_opens(a.p, nh, a.mp + 10*(size_t) node, dmac2, gfac);
           if (FAST && accept) {
             const double* mc = a.mp + 10*(size_t) node;
             double cx = mc[0] - gc.x, cy = mc[1] - gc.y, cz = mc[2] - gc.z;
@@ -688,6 +712,7 @@ GravArgs grav_args(gdf_ctx* c) {
   a.icount = c->icount; a.gnc = c->grav_nc; a.gnn = c->grav_nn;
   a.act = c->nlevels > 1 ? c->pi[I_ACT] : nullptr;
   a.grp_root = c->nseg > 0 ? c->grp_root : nullptr;
+  a.gmac = c->grp_gmac;
   a.gwx = a.gwy = a.gwz = a.gwp = a.gwp0 = nullptr;
   a.glist = nullptr; a.ncap = c->Ncap;
   a.p = c->phys;

@@ -83,6 +83,8 @@ struct PhysParams {
   double nbody_mult;
   int hrho_itmax;            // fixed-point iterations before bisection (reference: 30)
   int multipole;             // GDF_MULTIPOLE_*: the gravity walk's accepted-cell terms
+  int gravity_mac;           // GDF_MAC_*: its opening criterion
+  double macerror;
   // Ewald periodic gravity (periodic box with self-gravity): correction table of eng[0] x
   // eng[1] x eng[2] nodes, x fastest, each (pot, ax, ay, az) of a unit mass; node spacing 1/edI
   int ewald;
@@ -195,6 +197,9 @@ struct gdf_ctx {
   int* d_seg_off = nullptr;      // [kMaxSeg + 1]
   int* seg_root = nullptr;       // [kMaxSeg]
   int* grp_root = nullptr;       // [Ncap]: subtree root of each walk group's segment
+  // [Ncap]: each walk group's factor of the gadget2 / eigenmac opening criterion, from its
+  // particles' atree / gpot at the last tree build or refit
+  double* grp_gmac = nullptr;
   double* seg_dt = nullptr;      // [kMaxSeg]: drift / kick length of the segment's step
   double* seg_bnd = nullptr;     // [4 kMaxSeg]: lower corner and scale of its Morton grid
 

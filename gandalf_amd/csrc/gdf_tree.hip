@@ -221,6 +221,48 @@ __global__ void k_seg_roots(int N, int nseg, const int* __restrict__ off,
   }
 }
 
+// The walking group's factor of the gadget2 / eigenmac opening criterion (the reference's
+// walking leaf cell's amin / macfactor, stocked from its particles, KDTree.cpp:900-903):
+// gadget2 the smallest |atree|, eigenmac the largest gpot^(-2/3).  One wavefront per group;
+// perm (non-NULL) maps a group's sorted slots to the force arrays' order when the build did not
+// permute them.
+__global__ __launch_bounds__(BS) void k_group_mac(int ngroups, const int* __restrict__ grp_first,
+                                                  const int* __restrict__ grp_count,
+                                                  const int* __restrict__ perm,
+                                                  const double* __restrict__ ax,
+                                                  const double* __restrict__ ay,
+                                                  const double* __restrict__ az,
+                                                  const double* __restrict__ gpot, int mode,
+                                                  double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int g = blockIdx.x*(BS/kWave) + (int) threadIdx.x/kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (g >= ngroups) return;
+  const bool gadget = mode == GDF_MAC_GADGET2;
+  double v = gadget ? 9.9e30 : 0.0;
+  if (lane < grp_count[g]) {
+    int i = grp_first[g] + lane;
+    if (perm) i = perm[i];
+    v = gadget ? sqrt(ax[i]*ax[i] + ay[i]*ay[i] + az[i]*az[i])
+               : pow(gpot[i], -0.66666666666666666666666);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, kWave);
+    v = gadget ? fmin(v, w) : fmax(v, w);
+  }
+  if (lane == 0) out[g] = v;
+}
+
+int group_mac(gdf_ctx* c, const int* perm) {
+  if (c->phys.gravity_mac == GDF_MAC_GEOMETRIC || c->ngroups <= 0) return GDF_OK;
+  constexpr int gpb = BS/kWave;
+  k_group_mac<<<(c->ngroups + gpb - 1)/gpb, BS, 0, c->stream>>>(
+      c->ngroups, c->grp_first, c->grp_count, perm, c->df[D_ATX], c->df[D_ATY], c->df[D_ATZ],
+      c->df[D_GPOT], c->phys.gravity_mac, c->grp_gmac);
+  GDF_HIP(c, hipGetLastError());
+  return GDF_OK;
+}
+
 __global__ void k_grp_roots(int ngroups, const int* __restrict__ grp_first,
                             const int* __restrict__ seg, const int* __restrict__ root,
                             int* __restrict__ grp_root) {
@@ -773,6 +815,7 @@ int tree_alloc(gdf_ctx* c) {
   GDF_HIP(c, hipMalloc(&c->grp_count, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->grp_node, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->grp_root, sizeof(int)*N));
+  GDF_HIP(c, hipMalloc(&c->grp_gmac, sizeof(double)*N));
   GDF_HIP(c, hipMalloc(&c->nsel, sizeof(int)*4));
   // temporary storage of the sort and the group compaction
   size_t b1 = 0, b2 = 0;
@@ -1088,6 +1131,8 @@ int tree_build(gdf_ctx* c) {
     k_grp_roots<<<nblocks(c->ngroups, BS), BS, 0, s>>>(c->ngroups, c->grp_first, c->pi[I_SEG],
                                                        c->seg_root, c->grp_root);
   }
+  // the force arrays follow the new order only when the build permuted them
+  GDF_TRY(group_mac(c, c->df_alt[0] ? nullptr : c->vals));
   c->tree_valid = true;
   c->topo_valid = true;
   c->sg_valid = false;
@@ -1110,6 +1155,7 @@ int tree_restock(gdf_ctx* c) {
     k_leaves<<<1, BS, 0, s>>>(N, c->pf[F_X], c->pf[F_Y], c->pf[F_Z], c->pf[F_M], c->pf[F_H],
                               theta2, na, c->visit, c->parent);
   GDF_HIP(c, hipGetLastError());
+  GDF_TRY(group_mac(c, nullptr));
   GDF_TRY(integ_pack_posm(c));
   GDF_HIP(c, hipStreamSynchronize(s));
   c->tree_valid = true;

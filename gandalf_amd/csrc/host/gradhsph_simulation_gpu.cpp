@@ -157,7 +157,19 @@ int GradhSphSimulationGpu<ndim>::PostInitialConditionsSetup() {
   GDF_TRY(sphneib.UpdateAllSphProperties(false));
   GDF_TRY(sphneib.BuildTree(true, 0, 1, 1, 0.0));
   GDF_TRY(sphneib.UpdateAllSphProperties(false));
-  GDF_TRY(ComputeForces());
+  // a relative opening criterion needs forces to start from: the first pass uses the geometric
+  // one, then the tree is built again (stocking those forces) and the forces recomputed without
+  // a new h-rho pass (SphSimulation.cpp:380-470)
+  const int mac = c->phys.gravity_mac;
+  const bool twice = c->phys.self_gravity && mac != GDF_MAC_GEOMETRIC;
+  if (twice) c->phys.gravity_mac = GDF_MAC_GEOMETRIC;
+  const int frc = ComputeForces();
+  c->phys.gravity_mac = mac;
+  GDF_TRY(frc);
+  if (twice) {
+    GDF_TRY(sphneib.BuildTree(true, 0, 1, 1, 0.0));
+    GDF_TRY(ComputeForces());
+  }
   c->t = t0;
   c->tlast = t0;
   GDF_TRY(star_forces(c, false));      // the initial N-body forces (SphSimulation.cpp:490-502)

@@ -18,6 +18,8 @@ GDF_EOS = {"energy_eqn": 0, "isothermal": 1, "barotropic": 2}
 PREC = {"fp64": 0, "mixed": 1}
 # the reference's "multipole" parameter values (Parameters.cpp:309), as GDF_MULTIPOLE_*
 MULTIPOLE = {"quadrupole": 0, "monopole": 1, "fast_monopole": 2, "fast_quadrupole": 3}
+# and of its "gravity_mac" parameter (Parameters.cpp:308), as GDF_MAC_*
+GRAVITY_MAC = {"geometric": 0, "gadget2": 1, "eigenmac": 2}
 FIELDS3 = ("r", "v", "a", "atree", "r0", "v0", "a0", "adot")
 
 
@@ -50,6 +52,7 @@ class Config(C.Structure):
         ("ewald_mult", C.c_double), ("efratio", C.c_double),
         ("ntreebuildstep", C.c_int), ("ntreestockstep", C.c_int),
         ("graph_capture", C.c_int), ("multipole", C.c_int),
+        ("gravity_mac", C.c_int), ("macerror", C.c_double),
     ]
 
 
@@ -112,6 +115,8 @@ PROTOTYPES = {
     "gdf_snapshot_counters": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "gdf_build_tree_step": (_i, [_vp, _i, _i, _i, _i]),
     "gdf_graph_state": (_i, [_vp, C.POINTER(_i64), C.POINTER(_i)]),
+    "gdf_set_opening_criterion": (_i, [_vp, _i]),
+    "gdf_opening_criterion": (_i, [_vp]),
     "gdf_set_stars": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "gdf_num_stars": (_i, [_vp]),
     "gdf_get_star_field": (_i, [_vp, C.c_char_p, _vp]),
@@ -180,6 +185,8 @@ def config_from_params(prm: dict, precision: str = "fp64", group_max: int = 64) 
     c.ntreestockstep = int(prm.get("ntreestockstep", 1))
     c.graph_capture = int(prm.get("graph_capture", 0))
     c.multipole = MULTIPOLE[prm.get("multipole", "quadrupole")]
+    c.gravity_mac = GRAVITY_MAC[prm.get("gravity_mac", "geometric")]
+    c.macerror = float(prm.get("macerror", 1.0e-4))
     if c.nbody and c.nbody_eps <= 0.0:
         c.nbody_direct = 2 if c.nbody_softening else 1
     c.nbody_mult = float(prm.get("nbody_mult", 0.1))
@@ -255,6 +262,14 @@ class GpuSim:
         return bool(self._chk(lib().gdf_build_tree_step(self._h, int(bool(rebuild_tree)), int(n),
                                                         int(ntreebuildstep),
                                                         int(ntreestockstep))))
+
+    def set_opening_criterion(self, mac):
+        """NeighbourSearch::SetOpeningCriterion: "geometric", "gadget2" or "eigenmac"."""
+        self._chk(lib().gdf_set_opening_criterion(self._h, GRAVITY_MAC[mac]))
+
+    def opening_criterion(self):
+        names = {v: k for k, v in GRAVITY_MAC.items()}
+        return names[self._chk(lib().gdf_opening_criterion(self._h))]
 
     def graph_state(self):
         """(replays of the captured tree-build graphs, state: 1 in use, 0 off, negative: why a

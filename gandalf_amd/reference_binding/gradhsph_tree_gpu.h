@@ -110,10 +110,9 @@ class GradhSphTreeGpu : public SphNeighbourSearch<ndim> {
  public:
   typedef DeviceState<ndim, ParticleType> State;
 
-  GradhSphTreeGpu(State* state, MAC_Type mac) : dev(state), mac_type(mac), timing(0),
+  GradhSphTreeGpu(State* state, MAC_Type mac) : dev(state), mac_type(geometric), timing(0),
                                                  neibcheck(false) {
-    if (mac != geometric)
-      ExceptionHandler::getIstance().raise("neib_search = gpu: gravity_mac must be geometric");
+    SetOpeningCriterion(mac);
   }
   virtual ~GradhSphTreeGpu() {}
 
@@ -182,10 +181,13 @@ class GradhSphTreeGpu : public SphNeighbourSearch<ndim> {
   virtual void UpdateTimestepsLimitsFromDistantParticles(Hydrodynamics<ndim>*, const bool) {
     ExceptionHandler::getIstance().raise("neib_search = gpu: grad-h SPH only");
   }
+  // HydroTree::Get/SetOpeningCriterion (src/Tree/HydroTree.cpp:196-205); the reference's
+  // MAC_Type (Tree.h:52-56) numbers the criteria geometric, eigenmac, gadget2
   virtual MAC_Type GetOpeningCriterion() const { return mac_type; }
   virtual void SetOpeningCriterion(MAC_Type mac) {
-    if (mac != geometric)
-      ExceptionHandler::getIstance().raise("neib_search = gpu: gravity_mac must be geometric");
+    const int dm = mac == eigenmac ? GDF_MAC_EIGENMAC
+                 : mac == gadget2 ? GDF_MAC_GADGET2 : GDF_MAC_GEOMETRIC;
+    check(dev->ctx, gdf_set_opening_criterion(dev->ctx, dm));
     mac_type = mac;
   }
 

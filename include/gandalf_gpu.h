@@ -103,10 +103,21 @@ typedef struct gdf_config {
    * GradhSphTree.cpp:587-593).  A cell is then accepted only beyond the group's extent about
    * that point divided by theta.  Gas walks only: the N-body tree walk keeps quadrupoles. */
   int multipole;
+  /* "gravity_mac", "macerror" (Parameters.cpp:308, 314): the cell opening criterion of the
+   * gravity walk (Tree::open_cell_for_gravity, Tree.h:413-432).  GDF_MAC_GEOMETRIC (the default)
+   * accepts a cell beyond max(rmax, hmax)/theta; GDF_MAC_GADGET2 also opens a cell with
+   * m rmax^2 / d^4 above macerror times the smallest |atree| of the walk group's particles,
+   * GDF_MAC_EIGENMAC one nearer than (0.5 lambda / macerror)^(2/3) (lambda bounding its
+   * quadrupole's largest eigenvalue) times the largest gpot^(-2/3) of the group's particles;
+   * both read atree and gpot as they are at the tree build (KDTree.cpp:900-903, 1058-1079).
+   * d is the distance from the walk group's box to the cell. */
+  int gravity_mac;
+  double macerror;
 } gdf_config;
 
 enum { GDF_MULTIPOLE_QUADRUPOLE = 0, GDF_MULTIPOLE_MONOPOLE = 1, GDF_MULTIPOLE_FAST_MONOPOLE = 2,
        GDF_MULTIPOLE_FAST_QUADRUPOLE = 3 };
+enum { GDF_MAC_GEOMETRIC = 0, GDF_MAC_GADGET2 = 1, GDF_MAC_EIGENMAC = 2 };
 
 typedef struct gdf_ctx gdf_ctx;
 
@@ -144,6 +155,14 @@ int gdf_build_tree(gdf_ctx* ctx);
  * after a rebuild, 0 after a refit, or a negative status. */
 int gdf_build_tree_step(gdf_ctx* ctx, int rebuild_tree, int n, int ntreebuildstep,
                         int ntreestockstep);
+/* NeighbourSearch::Get/SetOpeningCriterion (src/Headers/NeighbourSearch.h:99-100,
+ * HydroTree.cpp:196-205): the gravity walk's criterion, GDF_MAC_*.  The gadget2 and eigenmac
+ * criteria read each walk group's atree / gpot as stocked by the last tree build or refit, so a
+ * caller switching to one builds the tree afterwards (as the reference's setup does,
+ * SphSimulation.cpp:380-470).  The setter returns GDF_OK or GDF_ERR_INVALID; the getter the
+ * criterion, or a negative status. */
+int gdf_set_opening_criterion(gdf_ctx* ctx, int gravity_mac);
+int gdf_opening_criterion(const gdf_ctx* ctx);
 /* state of the captured tree-build graphs (cfg.graph_capture; no counterpart in the
  * reference): replays so far, and 1 if they are in use, 0 if capture is off, or minus the
  * reason a capture was given up (the context then builds eagerly): 1 capture could not begin,

@@ -1527,13 +1527,66 @@ void orc_fast_multipole(int n, const double* dcom, const double* q, const double
 }
 
 #define GRAV_NEIB_SLACK (1.0 + 1.0e-4)
+#define TWOTHIRDS 0.66666666666666666666666
+
+/* The walking group's factor of the gadget2 and eigenmac criteria, from its particles' state at
+ * the tree stock (KDTree.cpp:900-903, the walking leaf cell's amin and macfactor): gadget2 the
+ * smallest |atree|, eigenmac the largest gpot^(-2/3).  NULL for the geometric criterion. */
+static double* group_mac_factors(const orc_params* p, const orc_tree* t, const double* atree,
+                                 const double* gpot) {
+  if (p->gravity_mac == 0) return NULL;
+  double* out = malloc(sizeof(double)*(t->ngroups > 0 ? t->ngroups : 1));
+  for (int g=0; g<t->ngroups; g++) {
+    double v = p->gravity_mac == 1 ? 9.9e30 : 0.0;
+    for (int tt=0; tt<t->group_count[g]; tt++) {
+      const int i = t->perm[t->group_first[g] + tt];
+      if (p->gravity_mac == 1) {
+        const double* a = &atree[3*i];
+        v = fmin(v, sqrt(a[0]*a[0] + a[1]*a[1] + a[2]*a[2]));
+      } else {
+        v = fmax(v, pow(gpot[i], -TWOTHIRDS));
+      }
+    }
+    out[g] = v;
+  }
+  return out;
+}
+
+/* Tree::open_cell_for_gravity (Tree.h:413-432) past its geometric test: gadget2 opens a cell
+ * whose m rmax^2 / d^4 exceeds macerror |a|; eigenmac one nearer than its (0.5 lambda /
+ * macerror)^(2/3) (KDTree.cpp:1058-1079, lambda bounding the quadrupole's largest eigenvalue)
+ * times the group's factor.  d^2 is the group box's nearest-point distance, as in the geometric
+ * test. */
+static int mac_opens(const orc_params* p, const double* half, double mc, const double* q,
+                     double dmac2, double gfac) {
+  if (p->gravity_mac == 1) {
+    const double rmax = sqrt(half[0]*half[0] + half[1]*half[1] + half[2]*half[2]);
+    return dmac2*dmac2*gfac*p->macerror < rmax*rmax*mc;
+  }
+  if (p->gravity_mac == 2) {
+    const double pq = q[0]*q[2] - (q[0] + q[2])*(q[0] + q[2]) - q[1]*q[1] - q[3]*q[3] -
+                      q[4]*q[4];
+    const double lambda = pq >= 0.0 ? 0.0 : 2.0*sqrt(-pq/3.0);
+    const double mac = pow(0.5*lambda/p->macerror, 0.66666666666666);
+    return dmac2 < mac*gfac;
+  }
+  return 0;
+}
+
+/* the criterion above for one cell (box half-widths half, mass mc, moments q[5]), for its
+ * known-answer tests */
+int orc_cell_mac_opens(const orc_params* p, const double* half, double mc, const double* q,
+                       double dmac2, double gfac) {
+  return mac_opens(p, half, mc, q, dmac2, gfac);
+}
 
 /* act (optional, input order): groups without an active particle are skipped (counts 0) and
  * only active targets are written */
 static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const double* r,
                                 const double* m, const double* h, const double* zeta,
                                 const double* hfactor, double* atree, double* gpot,
-                                int32_t* ncell, int32_t* nnear, const unsigned char* act) {
+                                int32_t* ncell, int32_t* nnear, const unsigned char* act,
+                                const double* gmacv) {
   okern kern; kern_init(&kern, p);
   obox b; box_init(&b, p, 0);
   const int N = t->N;
@@ -1575,6 +1628,7 @@ static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const do
         rg2 += e*e;
       }
       const double rg2t = rg2/p->thetamaxsqd;
+      const double gfac = gmacv ? gmacv[g] : 0.0;
       int sp = 0, nc = 0, np = 0;
       stack[sp++] = 0;   /* root: internal 0, or the single leaf when N == 1 */
       while (sp) {
@@ -1604,6 +1658,8 @@ static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const do
             if (ek > 0.0) dmac2 += ek*ek;
           }
           int accept = dmac2 >= t->cdistsqd[c];
+          if (accept && gmacv)
+            accept = !mac_opens(p, half, t->mass[c], &t->quad[5*c], dmac2, gfac);
           if (accept && fast) {
             double dc[3];
             for (int k=0; k<3; k++) dc[k] = t->com[3*c+k] - cG[k];
@@ -1680,8 +1736,11 @@ static void tree_gravity_masked(const orc_params* p, const orc_tree* t, const do
 void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
                       const double* m, const double* h, const double* zeta,
                       const double* hfactor, double* atree, double* gpot,
-                      int32_t* ncell, int32_t* nnear) {
-  tree_gravity_masked(p, t, r, m, h, zeta, hfactor, atree, gpot, ncell, nnear, NULL);
+                      int32_t* ncell, int32_t* nnear, const double* atree_prev,
+                      const double* gpot_prev) {
+  double* gm = group_mac_factors(p, t, atree_prev, gpot_prev);
+  tree_gravity_masked(p, t, r, m, h, zeta, hfactor, atree, gpot, ncell, nnear, NULL, gm);
+  free(gm);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -1785,6 +1844,8 @@ struct orc_sim {
   double t, timestep, tlast;
   double *r, *v, *a, *atree, *m, *h, *rho, *u, *dudt, *div_v, *gpot, *invomega, *zeta;
   double *sound, *pressure, *hfactor, *r0, *v0, *a0, *u0, *dudt0;
+  /* atree and gpot at the step's tree stock, for the gadget2 and eigenmac criteria */
+  double *mac_a, *mac_pot;
   /* block timesteps (Particle.h:140-167, Simulation.h:187-215) */
   int n, nresync, level_max, level_step;
   double dt_max;
@@ -1838,7 +1899,7 @@ orc_sim* orc_sim_create(const orc_params* p, int N, const double* r, const doubl
   ALLOC(r0, 3*N); ALLOC(v0, 3*N); ALLOC(a0, 3*N); ALLOC(u0, N); ALLOC(dudt0, N);
   ALLOC(level, N); ALLOC(levelneib, N); ALLOC(nstep, N); ALLOC(nlast, N);
   ALLOC(ptlast, N); ALLOC(pdt, N); ALLOC(dt_next, N); ALLOC(dtw, N);
-  ALLOC(active, N); ALLOC(endstep, N);
+  ALLOC(active, N); ALLOC(endstep, N); ALLOC(mac_a, 3*N); ALLOC(mac_pot, N);
 #undef ALLOC
   memcpy(s->r, r, sizeof(double)*3*N);
   memcpy(s->v, v, sizeof(double)*3*N);
@@ -1855,9 +1916,17 @@ void orc_sim_destroy(orc_sim* s) {
   free(s->u0); free(s->dudt0);
   free(s->level); free(s->levelneib); free(s->nstep); free(s->nlast); free(s->ptlast);
   free(s->pdt); free(s->dt_next); free(s->dtw); free(s->active); free(s->endstep);
+  free(s->mac_a); free(s->mac_pot);
   free(s->sr); free(s->sv); free(s->sa); free(s->sadot); free(s->sr0); free(s->sv0);
   free(s->sa0); free(s->sm); free(s->sh); free(s->sgpot);
   free(s);
+}
+
+/* the tree stock's copy of atree and gpot (KDTree.cpp:900-903), taken where the step builds its
+ * tree: before the setup's first force pass and at the start of every MainLoop */
+static void mac_snapshot(orc_sim* s) {
+  memcpy(s->mac_a, s->atree, sizeof(double)*3*s->N);
+  memcpy(s->mac_pot, s->gpot, sizeof(double)*s->N);
 }
 
 /* Sph::ZeroAccelerations (Sph.cpp:126-140), active particles */
@@ -1872,11 +1941,12 @@ static void zero_active(orc_sim* s, const unsigned char* act) {
 
 /* density, thermal state and forces of the active particles (all when act is NULL):
  * the body of SphSimulation::MainLoop's force loop (SphSimulation.cpp:653-720) */
-static int sim_forces(orc_sim* s, const unsigned char* act) {
+static int sim_forces(orc_sim* s, const unsigned char* act, int with_h) {
   const int N = s->N;
   const orc_params* p = &s->p;
-  if (compute_h_masked(p, N, s->r, s->m, 0, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
-                       s->u, s->sound, s->pressure, act, sim_stars(s))) return -1;
+  if (with_h && compute_h_masked(p, N, s->r, s->m, 0, s->h, s->rho, s->invomega, s->zeta,
+                                 s->hfactor, s->u, s->sound, s->pressure, act, sim_stars(s)))
+    return -1;
   zero_active(s, act);
   /* ComputeThermalProperties for every particle (nradstep = 1) */
   if (act) for (int i=0; i<N; i++) eos_thermal(p, s->rho[i], &s->u[i], &s->sound[i],
@@ -1894,8 +1964,10 @@ static int sim_forces(orc_sim* s, const unsigned char* act) {
     orc_tree tr;
     orc_tree_alloc(&tr, N);
     orc_tree_build(p, &tr, s->r, s->m, s->h, 64);
+    double* gm = group_mac_factors(p, &tr, s->mac_a, s->mac_pot);
     tree_gravity_masked(p, &tr, s->r, s->m, s->h, s->zeta, s->hfactor, s->atree, s->gpot,
-                        NULL, NULL, act);
+                        NULL, NULL, act, gm);
+    free(gm);
     orc_tree_free(&tr);
     for (int i=0; i<N; i++) {
       if (act && !act[i]) continue;
@@ -2091,6 +2163,7 @@ int orc_sim_setup(orc_sim* s) {
   const orc_params* p = &s->p;
   const double hg = orc_h_guess(p, N, s->r);
   for (int i=0; i<N; i++) s->h[i] = hg;
+  mac_snapshot(s);
   if (compute_h_masked(p, N, s->r, s->m, 1, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
                        s->u, s->sound, s->pressure, NULL, sim_stars(s))) return -1;
   if (compute_h_masked(p, N, s->r, s->m, 0, s->h, s->rho, s->invomega, s->zeta, s->hfactor,
@@ -2098,7 +2171,18 @@ int orc_sim_setup(orc_sim* s) {
   s->t = 0.0;
   s->n = 0; s->nresync = 0;
   for (int i=0; i<N; i++) { s->level[i] = 0; s->levelneib[i] = 0; s->ptlast[i] = s->t; }
-  if (sim_forces(s, NULL)) return -1;
+  /* a relative opening criterion needs forces to start from: the first pass uses the geometric
+     one, then the tree is stocked again with those forces and the forces recomputed, without a
+     new h-rho pass (SphSimulation.cpp:380-470) */
+  const int mac = s->p.gravity_mac;
+  const int twice = p->self_gravity && !p->grav_direct && mac != 0;
+  if (twice) s->p.gravity_mac = 0;
+  if (sim_forces(s, NULL, 1)) return -1;
+  if (twice) {
+    s->p.gravity_mac = mac;
+    mac_snapshot(s);
+    if (sim_forces(s, NULL, 0)) return -1;
+  }
   star_forces(s);             /* the initial N-body forces (SphSimulation.cpp:490-502) */
   star_end(s);
   if (p->nlevels > 1) {
@@ -2146,7 +2230,7 @@ static int sim_step_block(orc_sim* s) {
   wrap_with_r0(s);
   int activecount;
   do {
-    if (sim_forces(s, s->active)) return -1;
+    if (sim_forces(s, s->active, 1)) return -1;
     memset(s->active, 0, (size_t) N);
     activecount = check_timesteps(s);
   } while (activecount > 0);
@@ -2160,6 +2244,7 @@ static int sim_step_block(orc_sim* s) {
 int orc_sim_step(orc_sim* s) {
   const int N = s->N;
   const orc_params* p = &s->p;
+  mac_snapshot(s);
   if (p->nlevels > 1) return sim_step_block(s);
   const double dtstep = s->timestep;
   s->t = s->t + s->timestep;
@@ -2172,7 +2257,7 @@ int orc_sim_step(orc_sim* s) {
     s->sv[i] = s->sv0[i] + s->sa0[i]*dt;
   }
   wrap_with_r0(s);
-  if (sim_forces(s, NULL)) return -1;
+  if (sim_forces(s, NULL, 1)) return -1;
   star_forces(s);
   /* NbodyLeapfrogKDK::CorrectionTerms (:296-326) */
   for (int i=0; i<3*s->ns; i++) s->sv[i] += 0.5*(s->sa[i] - s->sa0[i])*dt;

@@ -37,6 +37,9 @@ typedef struct {
                               reference's neib_search = bruteforce (BruteForceTree.cpp:214) */
   int multipole;           /* cell terms of the gravity walk (Parameters.cpp:309): 0 quadrupole,
                               1 monopole, 2 fast_monopole, 3 fast_quadrupole */
+  int gravity_mac;         /* opening criterion (Parameters.cpp:308): 0 geometric, 1 gadget2,
+                              2 eigenmac */
+  double macerror;         /* their error parameter (Parameters.cpp:314) */
 } orc_params;
 
 enum { ORC_W0 = 0, ORC_W1, ORC_WOMEGA, ORC_WZETA, ORC_WGRAV, ORC_WPOT,
@@ -160,15 +163,24 @@ void orc_tree_refit_h(const orc_params* p, orc_tree* t, const double* h);
 
 /* tree gravity with the device walk's acceptance rules; r,m,h,zeta,hfactor in input order.
  * If ncell/nnear are non-null they receive per-group interaction counts. */
+/* atree_prev (N x 3) and gpot_prev (N): the particles' tree acceleration and potential when the
+ * tree was stocked, for the gadget2 and eigenmac criteria (NULL for the geometric one) */
 void orc_tree_gravity(const orc_params* p, const orc_tree* t, const double* r,
                       const double* m, const double* h, const double* zeta,
                       const double* hfactor, double* atree, double* gpot,
-                      int32_t* ncell, int32_t* nnear);
+                      int32_t* ncell, int32_t* nnear, const double* atree_prev,
+                      const double* gpot_prev);
 
 /* one quadrupole cell term (NeighbourSearch.h:423-475): a cell of mass mc, centre of mass com
  * and traceless moments q[5] acting on a point rp; adds to agrav[3] and *gpot */
 void orc_cell_quad_force(const double* com, const double* q, double mc, const double* rp,
                          double* agrav, double* gpot);
+/* Tree::open_cell_for_gravity (Tree.h:413-432) past the geometric test, for p->gravity_mac
+ * gadget2 (1) or eigenmac (2): 1 if a cell (box half-widths half, mass mc, moments q[5]) at
+ * squared distance dmac2 is opened, given the walking group's factor gfac (smallest |atree| or
+ * largest gpot^(-2/3)) */
+int orc_cell_mac_opens(const orc_params* p, const double* half, double mc, const double* q,
+                       double dmac2, double gfac);
 /* one monopole cell term (ComputeCellMonopoleForces, NeighbourSearch.h:350-377) */
 void orc_cell_mono_force(const double* com, double mc, const double* rp, double* agrav,
                          double* gpot);

@@ -31,7 +31,7 @@ class Params(C.Structure):
         ("boxmin", C.c_double * 3), ("boxmax", C.c_double * 3),
         ("courant_mult", C.c_double), ("accel_mult", C.c_double), ("energy_mult", C.c_double),
         ("nlevels", C.c_int), ("level_diff_max", C.c_int), ("grav_direct", C.c_int),
-        ("multipole", C.c_int),
+        ("multipole", C.c_int), ("gravity_mac", C.c_int), ("macerror", C.c_double),
     ]
 
 
@@ -144,9 +144,10 @@ def lib():
         L.orc_tree_build.argtypes = [pp, C.POINTER(Tree), vp, vp, vp, i]
         L.orc_tree_build.restype = i
         L.orc_tree_refit_h.argtypes = [pp, C.POINTER(Tree), vp]
-        L.orc_tree_gravity.argtypes = [pp, C.POINTER(Tree)] + [vp] * 9
+        L.orc_tree_gravity.argtypes = [pp, C.POINTER(Tree)] + [vp] * 11
         L.orc_cell_quad_force.argtypes = [vp, vp, d, vp, vp, vp]
         L.orc_cell_mono_force.argtypes = [vp, d, vp, vp, vp]
+        L.orc_cell_mac_opens.argtypes = [pp, vp, d, vp, d, d]
         L.orc_fast_multipole.argtypes = [i, vp, vp, vp, i, i, vp, vp, vp, vp]
         L.orc_timesteps.argtypes = [pp, i] + [vp] * 7
         L.orc_timesteps.restype = d
@@ -176,6 +177,8 @@ def lib():
 
 # the reference's "multipole" parameter values (Parameters.cpp:309)
 MULTIPOLE = {"quadrupole": 0, "monopole": 1, "fast_monopole": 2, "fast_quadrupole": 3}
+# and of its "gravity_mac" parameter (Parameters.cpp:308)
+GRAVITY_MAC = {"geometric": 0, "gadget2": 1, "eigenmac": 2}
 
 
 def params_from_dict(prm: dict) -> Params:
@@ -209,6 +212,8 @@ def params_from_dict(prm: dict) -> Params:
     p.level_diff_max = int(prm.get("level_diff_max", 1))
     p.grav_direct = 1 if prm.get("neib_search", "kdtree") == "bruteforce" else 0
     p.multipole = MULTIPOLE[prm.get("multipole", "quadrupole")]
+    p.gravity_mac = GRAVITY_MAC[prm.get("gravity_mac", "geometric")]
+    p.macerror = float(prm.get("macerror", 1.0e-4))
     return p
 
 
@@ -330,6 +335,12 @@ def nbody_direct_smoothed(p: Params, r, v, m, h):
     return a, ad, gp
 
 
+def cell_mac_opens(p: Params, half, mc, q, dmac2, gfac) -> bool:
+    """The gadget2 / eigenmac opening test of one cell (p.gravity_mac selects it)."""
+    return bool(lib().orc_cell_mac_opens(C.byref(p), _p(_d(half)), float(mc), _p(_d(q)),
+                                         float(dmac2), float(gfac)))
+
+
 def cell_mono_force(com, mc, rp):
     """The oracle's monopole cell term at one point: (acceleration, potential)."""
     a = np.zeros(3)
@@ -402,14 +413,18 @@ class TreeTwin:
         h = _d(h)
         lib().orc_tree_refit_h(C.byref(self.p), C.byref(self._t), _p(h))
 
-    def gravity(self, r, m, st):
+    def gravity(self, r, m, st, atree_prev=None, gpot_prev=None):
+        """Tree gravity; atree_prev / gpot_prev: the particles' tree acceleration and potential
+        at the tree stock, which the gadget2 / eigenmac criteria read (zero when omitted)."""
         r, m = _d(r), _d(m)
         N = r.shape[0]
         at = np.zeros((N, 3)); gp = np.zeros(N)
         nc = np.zeros(self._t.ngroups, dtype=np.int32); nn = np.zeros_like(nc)
+        ap = _d(np.zeros((N, 3)) if atree_prev is None else atree_prev)
+        gq = _d(np.zeros(N) if gpot_prev is None else gpot_prev)
         lib().orc_tree_gravity(C.byref(self.p), C.byref(self._t), _p(r), _p(m),
                                _p(_d(st["h"])), _p(_d(st["zeta"])), _p(_d(st["hfactor"])),
-                               _p(at), _p(gp), _p(nc), _p(nn))
+                               _p(at), _p(gp), _p(nc), _p(nn), _p(ap), _p(gq))
         return at, gp, nc, nn
 
 

@@ -43,6 +43,8 @@ def run_pair(case, precision="fp64", nsteps=None, over=None):
     ("cloud4k_quintic_blk", {"neib_search": "kdtree"}),
     # the same with the fast multipole expansion per walk group
     ("cloud4k_quintic_blk", {"neib_search": "kdtree", "multipole": "fast_quadrupole"}),
+    # and with the gadget2 criterion reading the forces stocked at each step's tree build
+    ("cloud4k_quintic_blk", {"neib_search": "kdtree", "gravity_mac": "gadget2"}),
 ])
 def test_block_steps_match_oracle(case, over):
     s, o, states = run_pair(case, over=over)
