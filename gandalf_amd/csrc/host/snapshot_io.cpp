@@ -207,10 +207,76 @@ int gdf_format_sf(const char* path, int nhydro, int nstar, const int64_t* counts
   return f.good() ? GDF_OK : GDF_ERR_INVALID;
 }
 
+// Seren lite layout (WriteSerenLiteSnapshotFile, SimulationIO.hpp:2264-2497): the unformatted
+// file's header with a FLOAT size of 4, single-precision header reals, no porig or velocity
+// arrays, r, m, h, rho and u as floats, and 4-byte star records; its header keeps the snapshot
+// count and step count only (no lite counters).
+int gdf_format_slite(const char* path, int nhydro, int nstar, const int64_t* counts2,
+                     const double* times3, double h_fac, const double* r, const double* v,
+                     const double* m, const double* h, const double* rho, const double* u) {
+  if (!path || !counts2 || !times3 || nhydro < 0 || nstar < 0 ||
+      (nhydro + nstar > 0 && (!r || !v || !m || !h)) || (nhydro > 0 && (!rho || !u)))
+    return GDF_ERR_INVALID;
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return GDF_ERR_INVALID;
+  int idata[50] = {};
+  long ilpdata[50] = {};
+  float rdata[50] = {};
+  double ddata[50] = {};
+  idata[0] = nhydro; idata[1] = nstar; idata[4] = nhydro;
+  struct Arr { const char* id; int w, unit; };
+  const Arr hyd[] = {{"r", 3, 1}, {"m", 1, 2}, {"h", 1, 1}, {"rho", 1, 6}, {"u", 1, 20}};
+  const int ndata = (nhydro > 0 ? 5 : 0) + (nstar > 0 ? 1 : 0);
+  idata[19] = 0;                       // no unit names: dimensionless
+  idata[20] = ndata;
+  ilpdata[0] = (long) counts2[0]; ilpdata[1] = (long) counts2[1];
+  rdata[0] = (float) h_fac;
+  ddata[0] = times3[0]; ddata[1] = times3[1]; ddata[2] = times3[2];
+  su_str(f, kSuTag);
+  su_put(f, (int) sizeof(float));
+  for (int k = 0; k < 3; k++) su_put(f, 3);
+  for (int k = 0; k < 50; k++) su_put(f, idata[k]);
+  for (int k = 0; k < 50; k++) su_put(f, ilpdata[k]);
+  for (int k = 0; k < 50; k++) su_put(f, rdata[k]);
+  for (int k = 0; k < 50; k++) su_put(f, ddata[k]);
+  if (nhydro > 0) for (const Arr& a : hyd) su_str(f, a.id);
+  if (nstar > 0) su_str(f, "sink_v1");
+  if (nhydro > 0)
+    for (const Arr& a : hyd) {
+      const int td[5] = {a.w, 1, nhydro, 4, a.unit};
+      for (int k = 0; k < 5; k++) su_put(f, td[k]);
+    }
+  if (nstar > 0) {
+    const int td[5] = {1, 1, nstar, 7, 0};
+    for (int k = 0; k < 5; k++) su_put(f, td[k]);
+  }
+  const size_t n = (size_t) nhydro;
+  auto put = [&f](const double* a, size_t cnt) {
+    for (size_t i = 0; i < cnt; i++) su_put(f, (float) a[i]);
+  };
+  if (nhydro > 0) { put(r, 3*n); put(m, n); put(h, n); put(rho, n); put(u, n); }
+  if (nstar > 0) {
+    const int head[6] = {2, 2, 0, kSinkLen, 0, 0};
+    for (int k = 0; k < 6; k++) su_put(f, head[k]);
+    for (int i = 0; i < nstar; i++) {
+      const size_t j = n + (size_t) i;
+      su_put(f, 1); su_put(f, 1); su_put(f, i + 1); su_put(f, 0);
+      float sd[kSinkLen] = {};
+      for (int k = 0; k < 3; k++) {
+        sd[1 + k] = (float) r[3*j + k]; sd[4 + k] = (float) v[3*j + k];
+      }
+      sd[7] = (float) m[j]; sd[8] = (float) h[j];
+      for (int k = 0; k < kSinkLen; k++) su_put(f, sd[k]);
+    }
+  }
+  return f.good() ? GDF_OK : GDF_ERR_INVALID;
+}
+
 int gdf_write_snapshot(gdf_ctx* c, const char* path, const char* format) {
   if (!c || !path || !format || c->N <= 0) return GDF_ERR_INVALID;
   const std::string fmt(format);
-  if (fmt != "column" && fmt != "su" && fmt != "sf")
+  const bool lite = fmt == "slite" || fmt == "seren_lite";
+  if (fmt != "column" && fmt != "su" && fmt != "sf" && !lite)
     return gdf::fail(c, GDF_ERR_INVALID, std::string("unsupported snapshot format ") + format);
   const size_t n = (size_t) c->N;
   std::vector<double> r(3*n), v(3*n), m(n), h(n), rho(n), u(n);
@@ -234,7 +300,11 @@ int gdf_write_snapshot(gdf_ctx* c, const char* path, const char* format) {
     const int64_t counts[3] = {c->snap_nout, c->nsteps, c->snap_nlite};
     const double times[4] = {c->t, c->snap_tlast, mmean, c->snap_tlitelast};
     const int* porig = c->snap_porig.size() == (size_t) nh ? c->snap_porig.data() : nullptr;
-    rc = (fmt == "su" ? gdf_format_su : gdf_format_sf)(path, nh, ns, counts, times,
+    if (lite)
+      rc = gdf_format_slite(path, nh, ns, counts, times, c->phys.h_fac, r.data(), v.data(),
+                            m.data(), h.data(), rho.data(), u.data());
+    else
+      rc = (fmt == "su" ? gdf_format_su : gdf_format_sf)(path, nh, ns, counts, times,
                                                         c->phys.h_fac, porig, r.data(), v.data(),
                                                         m.data(), h.data(), rho.data(), u.data());
   }

@@ -88,6 +88,7 @@ PROTOTYPES = {
     "gdf_format_column": (_i, [C.c_char_p, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gdf_format_su": (_i, [C.c_char_p, _i, _i, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gdf_format_sf": (_i, [C.c_char_p, _i, _i, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gdf_format_slite": (_i, [C.c_char_p, _i, _i, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gdf_pool_create": (_i, [C.POINTER(Config), _i64, C.POINTER(_vp)]),
     "gdf_pool_destroy": (None, [_vp]),
     "gdf_pool_last_error": (C.c_char_p, [_vp]),

@@ -12,7 +12,8 @@ reference's (src/Common/Parameters.cpp:160-575) for the keys this path reads.  E
 The particle state lives on the GPU (one device context, gandalf_amd.gpu.GpuSim); this layer
 keeps only the simulation clock and the output bookkeeping on the host, as the reference's
 Simulation object does.  Supported: ndim = 3, sim = "gradhsph" (or "sph") and "nbody";
-ic = "file" (column, su, sf), "python" and restarts from ``<run_id>.restart``.
+ic = "file" (column, su, sf), "python" and restarts from ``<run_id>.restart``; lite snapshots
+(slite) are written when litesnap is set.
 """
 from __future__ import annotations
 
@@ -63,6 +64,8 @@ _STRING = {
     "gpu_precision": "fp64",
 }
 _FORMATS = ("column", "su", "sf")
+# formats written only (the reference reads no lite snapshot)
+_WRITE_FORMATS = _FORMATS + ("slite", "seren_lite")
 
 
 class Parameters:
@@ -129,6 +132,8 @@ class SimulationBase:
         self.Noutlitesnap = 0
         self.tsnaplast = 0.0
         self.tsnapnext = 0.0
+        self.tlitesnaplast = 0.0
+        self.tlitesnapnext = 0.0
         self.nlastrestart = 0
         self.restart = False
         self.diag = None
@@ -180,6 +185,9 @@ class SimulationBase:
         self.tend = p.floatparams["tend"]
         self.dt_snap = p.floatparams["dt_snap"]
         self.tsnapnext = p.floatparams["tsnapfirst"]
+        self.litesnap = p.intparams["litesnap"]
+        self.dt_litesnap = p.floatparams["dt_litesnap"]
+        self.tlitesnapnext = p.floatparams["tlitesnapfirst"]
         self.dt_python = p.floatparams["dt_python"]
         self.Nstepsmax = p.intparams["Nstepsmax"]
         self.noutputstep = p.intparams["noutputstep"]
@@ -267,6 +275,7 @@ class SimulationBase:
         if self.restart:
             # SimulationIO.hpp:2553-2555: the next snapshot follows the restored one
             self.tsnapnext = self.tsnaplast + self.dt_snap
+            self.tlitesnapnext = self.tlitesnaplast + self.dt_litesnap
         self.sim.setup()
         self.t, self.timestep, self.Nsteps = self.sim.time()
         self._sync_block_clock()
@@ -330,6 +339,12 @@ class SimulationBase:
             print(f"t : {self.t}    dt : {self.timestep}    Nsteps : {self.Nsteps}")
         filename = ""
         form = self.simparams.stringparams["out_file_form"]
+        # a lite snapshot for movies (Simulation.cpp:527-540)
+        if self.litesnap == 1 and self.t >= self.tlitesnapnext:
+            self.Noutlitesnap += 1
+            self.tlitesnaplast = self.tlitesnapnext
+            self.tlitesnapnext += self.dt_litesnap
+            self.WriteSnapshotFile(f"{self.run_id}.slite.{self.Noutlitesnap:05d}", "slite")
         if self.t >= self.tsnapnext:
             self.Noutsnap += 1
             self.tsnaplast = self.tsnapnext
@@ -355,9 +370,10 @@ class SimulationBase:
 
     def WriteSnapshotFile(self, filename, form):
         """Simulation::WriteSnapshotFile (SimulationIO.hpp:60-150)."""
-        if form not in _FORMATS:
+        if form not in _WRITE_FORMATS:
             raise GandalfError(f"Error: snapshot format {form} is not available")
-        self.sim.snapshot_counters(self.Noutsnap, self.Noutlitesnap, self.tsnaplast, 0.0)
+        self.sim.snapshot_counters(self.Noutsnap, self.Noutlitesnap, self.tsnaplast,
+                                   self.tlitesnaplast)
         self.sim.write_snapshot(filename, form)
         return True
 
@@ -370,7 +386,8 @@ class SimulationBase:
             raise GandalfError(f"Error: snapshot file {filename} not found")
         self.sim.read_snapshot(filename, form)
         self.t, self.timestep, self.Nsteps = self.sim.time()
-        self.Noutsnap, self.Noutlitesnap, self.tsnaplast, _ = self.sim.snapshot_counters()
+        self.Noutsnap, self.Noutlitesnap, self.tsnaplast, self.tlitesnaplast = \
+            self.sim.snapshot_counters()
         return True
 
     def CalculateDiagnostics(self):

@@ -242,9 +242,10 @@ int gdf_block_state(const gdf_ctx* ctx, int64_t* iout5, double* dout1);
  * src/Common/SimulationIO.hpp:60-150): format "column" (WriteColumnSnapshotFile /
  * ReadColumnSnapshotFile, :444-548, :193-272), "su" (WriteSerenUnformSnapshotFile /
  * ReadSerenUnformSnapshotFile, :2009-2249, :1294-1540) or "sf" (WriteSerenFormSnapshotFile /
- * ReadSerenFormSnapshotFile, :993-1222, :601-925).  Reading sets the particles (r, v, m,
- * h, u; rho as a field) and the time, and "su" the step count; an N-body context reads and
- * writes the star block. */
+ * ReadSerenFormSnapshotFile, :993-1222, :601-925); writing also "slite" / "seren_lite"
+ * (WriteSerenLiteSnapshotFile, :2264-2497; the reference has no reader for it).  Reading sets
+ * the particles (r, v, m, h, u; rho as a field) and the time, and "su" the step count; an
+ * N-body context reads and writes the star block. */
 int gdf_write_snapshot(gdf_ctx* ctx, const char* path, const char* format);
 int gdf_read_snapshot(gdf_ctx* ctx, const char* path, const char* format);
 /* the snapshot counters the Seren writers put in their headers and a restart read restores
@@ -272,6 +273,12 @@ int gdf_format_sf(const char* path, int nhydro, int nstar, const int64_t* counts
                   const double* times4, double h_fac, const int* porig, const double* r,
                   const double* v, const double* m, const double* h, const double* rho,
                   const double* u);
+/* the Seren lite writer on host arrays (Simulation::WriteSerenLiteSnapshotFile,
+ * src/Common/SimulationIO.hpp:2264-2497: single-precision r, m, h, rho, u and star records):
+ * counts2 = {Noutsnap, Nsteps}, times3 = {t, tsnaplast, mmean} */
+int gdf_format_slite(const char* path, int nhydro, int nstar, const int64_t* counts2,
+                     const double* times3, double h_fac, const double* r, const double* v,
+                     const double* m, const double* h, const double* rho, const double* u);
 /* simulation time, current timestep and step count */
 int gdf_time(const gdf_ctx* ctx, double* t, double* timestep, int64_t* nsteps);
 

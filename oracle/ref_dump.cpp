@@ -11,7 +11,8 @@
 // usage: ref_dump <paramfile> <nsteps> <outprefix> [dump_every]
 //   writes <outprefix>_s<step>.bin (step 0 = after setup; gas), <outprefix>_stars_s<step>.bin
 //   (stars: r v a adot m h gpot dt) and <outprefix>_diag.txt, and the
-//   reference's own column and Seren unformatted snapshots of each dumped step after setup
+//   reference's own column, Seren unformatted, formatted and lite snapshots of each dumped
+//   step after setup
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -163,6 +164,8 @@ int main(int argc, char** argv)
       sim->WriteSerenUnformSnapshotFile(prefix + name);
       snprintf(name, sizeof(name), "_s%d.sf", s);
       sim->WriteSerenFormSnapshotFile(prefix + name);
+      snprintf(name, sizeof(name), "_s%d.slite", s);
+      sim->WriteSerenLiteSnapshotFile(prefix + name);
     }
   }
   fclose(dfp);

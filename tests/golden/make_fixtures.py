@@ -162,6 +162,9 @@ def run_case(name, gen, over, nsteps, outdir):
                 arrs[f"s{nsteps}_su"] = np.frombuffer(f.read(), dtype=np.uint8)
             with open(os.path.join(td, f"out_s{nsteps}.sf"), "rb") as f:
                 arrs[f"s{nsteps}_sf"] = np.frombuffer(f.read(), dtype=np.uint8)
+            # and its Seren lite snapshot (SimulationIO.hpp:2264)
+            with open(os.path.join(td, f"out_s{nsteps}.slite"), "rb") as f:
+                arrs[f"s{nsteps}_slite"] = np.frombuffer(f.read(), dtype=np.uint8)
         if hyb is not None:
             for k, a in hyb.items():
                 arrs[f"star_in_{k}"] = a

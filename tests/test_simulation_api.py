@@ -75,7 +75,8 @@ def test_python_ic_run_reproduces_reference(tmp_path, monkeypatch):
     d = load_golden("box16_m4")
     N = len(d["in_m"])
     s = _box_sim(d, ic="python", Nhydro=N, run_id="PY", tsnapfirst=0.05, dt_snap=0.05,
-                 out_file_form="su", noutputstep=1000, nrestartstep=1000)
+                 out_file_form="su", noutputstep=1000, nrestartstep=1000, litesnap=1,
+                 dt_litesnap=0.02, tlitesnapfirst=0.0)
     s.PreSetupForPython()
     for k, q in enumerate("xyz"):
         s.ImportArray(d["in_r"][:, k], q)
@@ -99,6 +100,18 @@ def test_python_ic_run_reproduces_reference(tmp_path, monkeypatch):
             assert np.array_equal(a, b), (snap.filename, name)
     with open("PY.restart") as f:
         assert f.read().split() == ["su", snaps[-1].filename]
+    # lite snapshots every dt_litesnap (Simulation.cpp:527-540): single precision, N particles,
+    # the step count of the state each was written in, one per lite time passed
+    lites = sorted(f for f in os.listdir(".") if f.startswith("PY.slite."))
+    assert len(lites) == s.Noutlitesnap and len(lites) >= 3
+    steps = []
+    for name in lites:
+        b = open(name, "rb").read()
+        assert np.frombuffer(b[20:24], np.int32)[0] == 4
+        assert np.frombuffer(b[36:40], np.int32)[0] == N
+        assert len(b) == 1236 + 5 * 20 + 5 * 20 + 4 * 7 * N
+        steps.append(int(np.frombuffer(b[244:252], np.int64)[0]))
+    assert steps == sorted(steps) and steps[-1] <= 10
 
 
 @pytest.mark.gpu

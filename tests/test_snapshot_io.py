@@ -97,6 +97,40 @@ def test_su_writer_matches_reference_file(tmp_path):
     assert out.read_bytes() == ref
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libgandalf_gpu.so not built")
+def test_slite_writer_matches_reference_file(tmp_path):
+    """WriteSerenLiteSnapshotFile (SimulationIO.hpp:2264-2497): FLOAT size 4 in the header,
+    float header reals, r, m, h, rho, u as floats; byte for byte the reference's own file."""
+    d = load_golden("box16_m4")
+    n = d["meta"]["nsteps"]
+    ref = bytes(d[f"s{n}_slite"])
+    N = d["in_m"].shape[0]
+    assert np.frombuffer(ref[20:24], np.int32)[0] == 4
+    idata = np.frombuffer(ref[36:236], np.int32)
+    ilp = np.frombuffer(ref[236:636], np.int64)
+    rdata = np.frombuffer(ref[636:836], np.float32)
+    ddata = np.frombuffer(ref[836:1236], np.float64)
+    assert idata[0] == N and idata[20] == 5 and ilp[1] == n
+    assert rdata[0] == np.float32(d["meta"]["params"]["h_fac"])
+    # the arrays close the file: r (3N), m, h, rho, u as floats
+    tail = np.frombuffer(ref[len(ref) - 4 * 7 * N:], np.float32)
+    assert np.array_equal(tail[:3 * N], vec(d, f"s{n}").astype(np.float32).ravel())
+    assert np.array_equal(tail[6 * N:], d[f"s{n}_u"].astype(np.float32))
+    lib = C.CDLL(LIB)
+    f = lib.gdf_format_slite
+    f.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 6
+    f.restype = C.c_int
+    counts = np.array([ilp[0], ilp[1]], dtype=np.int64)
+    times = np.array([ddata[0], ddata[1], ddata[2]])
+    out = tmp_path / "snap.slite"
+    rc = f(str(out).encode(), N, 0, counts.ctypes.data_as(C.c_void_p), _p(times),
+           float(d["meta"]["params"]["h_fac"]), _p(vec(d, f"s{n}")),
+           _p(vec(d, f"s{n}", ("vx", "vy", "vz"))), _p(d["in_m"]), _p(d[f"s{n}_h"]),
+           _p(d[f"s{n}_rho"]), _p(d[f"s{n}_u"]))
+    assert rc == 0
+    assert out.read_bytes() == ref
+
+
 @pytest.mark.gpu
 def test_su_snapshot_roundtrip_through_device(tmp_path):
     from gandalf_amd.gpu import GpuSim
@@ -116,10 +150,13 @@ def test_su_snapshot_roundtrip_through_device(tmp_path):
     out = tmp_path / "out.su"
     s.write_snapshot(out, "su")
     assert out.read_bytes() == src.read_bytes()
-    # the column writer of the same device state gives the reference's column file
+    # the column and lite writers of the same device state give the reference's files
     col = tmp_path / "out.col"
     s.write_snapshot(col)
     assert col.read_bytes() == bytes(d[f"s{n}_column"])
+    lite = tmp_path / "out.slite"
+    s.write_snapshot(lite, "slite")
+    assert lite.read_bytes() == bytes(d[f"s{n}_slite"])
 
 
 @pytest.mark.gpu
