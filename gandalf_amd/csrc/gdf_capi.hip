@@ -483,7 +483,7 @@ int gdf_set_particles(gdf_ctx* c, int N, const double* r, const double* v, const
   for (int f = 0; f < I_NINT; f++) GDF_HIP(c, hipMemset(c->pi[f], 0, sizeof(int)*n));
   k_iota<<<nblocks(N, BS), BS, 0, c->stream>>>(N, c->iorig);
   GDF_HIP(c, hipStreamSynchronize(c->stream));
-  c->tree_valid = false; c->topo_valid = false; c->sg_valid = false; c->setup_done = false;
+  c->tree_valid = false; c->topo_valid = false; c->sg_valid = false; c->sg_union = false; c->setup_done = false;
   c->t = c->timestep = c->tlast = 0.0; c->nsteps = 0;
   return GDF_OK;
 }
@@ -529,7 +529,7 @@ int gdf_set_field(gdf_ctx* c, const char* name, const double* in) {
   GDF_HIP(c, hipGetLastError());
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   if (!strcmp(name, "r") || !strcmp(name, "m") || !strcmp(name, "h")) c->tree_valid = false;
-  c->sg_valid = false;
+  c->sg_valid = false; c->sg_union = false;
   return GDF_OK;
 }
 
@@ -719,9 +719,94 @@ int gdf_tree_arrays(gdf_ctx* c, int32_t* perm, uint64_t* keys, int32_t* left, in
   return GDF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Per-particle sets (import order) into CSR form.
+int64_t emit_sets(gdf_ctx* c, std::vector<std::vector<int>>& sets, int64_t* offsets,
+                  int32_t* ids, int64_t cap) {
+  const int N = c->N;
+  for (auto& v : sets) std::sort(v.begin(), v.end());
+  int64_t total = 0;
+  offsets[0] = 0;
+  for (int i = 0; i < N; i++) { total += (int64_t) sets[i].size(); offsets[i + 1] = total; }
+  if (!ids) return total;
+  if (total > cap) return fail(c, GDF_ERR_CAPACITY, "neighbour set output capacity too small");
+  for (int i = 0; i < N; i++) std::copy(sets[i].begin(), sets[i].end(), ids + offsets[i]);
+  return total;
+}
+
+// The scatter-gather sets the mixed path's last force pass evaluated: its candidate lists (the
+// density pass's union lists, or the force pass's own walk) decoded with the FP32 group-frame
+// test of its hydro kernel (positions relative to the group centre rounded to float, squared
+// ranges rounded to float, the squared distance with two fused multiply-adds), for every
+// target of the groups it walked.
+int64_t pass_sg_sets(gdf_ctx* c, int64_t* offsets, int32_t* ids, int64_t cap) {
+  if (c->cfg.precision != GDF_PREC_MIXED || !c->sg_valid || c->sg_masks)
+    return fail(c, GDF_ERR_INVALID, "no candidate lists of a mixed force pass to decode");
+  const CandSet& cs = c->sg_from_union ? c->cs_g : c->cs_s;
+  const int N = c->N, G = c->ngroups, nn = 2*N - 1;
+  const size_t per = (size_t) cs.capb*64;
+  std::vector<int> cidx((size_t) G*per), ncand(G), io(N), gfirst(G), gcount(G), gnode(G);
+  std::vector<double> hh(N), x(N), y(N), z(N);
+  std::vector<double4> gc(nn);
+  GDF_HIP(c, hipStreamSynchronize(c->stream));
+  GDF_HIP(c, hipMemcpy(cidx.data(), cs.idx, sizeof(int)*cidx.size(), hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(ncand.data(), cs.ncand, sizeof(int)*G, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(io.data(), c->iorig, sizeof(int)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(gfirst.data(), c->grp_first, sizeof(int)*G, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(gcount.data(), c->grp_count, sizeof(int)*G, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(gnode.data(), c->grp_node, sizeof(int)*G, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(gc.data(), c->geo_c, sizeof(double4)*nn, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(hh.data(), c->pf[F_H], sizeof(double)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(x.data(), c->pf[F_X], sizeof(double)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(y.data(), c->pf[F_Y], sizeof(double)*N, hipMemcpyDeviceToHost));
+  GDF_HIP(c, hipMemcpy(z.data(), c->pf[F_Z], sizeof(double)*N, hipMemcpyDeviceToHost));
+  const double krsqd = c->phys.kern.rangesqd;
+  const double* hb = c->phys.boxhalf;
+  const double* bs = c->phys.boxsize;
+  auto img = [&](double& d, int k) {
+    if (!c->phys.periodic) return;
+    if (d > hb[k]) d -= bs[k]; else if (d < -hb[k]) d += bs[k];
+  };
+  auto r2f = [&](double h) { return (float) (krsqd*h*h); };
+  std::vector<std::vector<int>> sets(N);
+  for (int g = 0; g < G; g++) {
+    const int nc = ncand[g];
+    if (nc <= 0 || (size_t) nc > per) continue;
+    const double4 o = gc[gnode[g]];
+    const int* lst = cidx.data() + (size_t) g*per;
+    std::vector<float> cx(nc), cy(nc), cz(nc), cr(nc);
+    for (int k = 0; k < nc; k++) {
+      const int j = lst[k];
+      double dx = x[j] - o.x, dy = y[j] - o.y, dz = z[j] - o.z;
+      img(dx, 0); img(dy, 1); img(dz, 2);
+      cx[k] = (float) dx; cy[k] = (float) dy; cz[k] = (float) dz; cr[k] = r2f(hh[j]);
+    }
+    for (int l = 0; l < gcount[g]; l++) {
+      const int i = gfirst[g] + l;
+      const float rx = (float) (x[i] - o.x), ry = (float) (y[i] - o.y), rz = (float) (z[i] - o.z);
+      const float r2i = r2f(hh[i]);
+      auto& out = sets[io[i]];
+      for (int k = 0; k < nc; k++) {
+        const float a = cx[k] - rx, b = cy[k] - ry, d = cz[k] - rz;
+        const float d2 = std::fma(d, d, std::fma(b, b, a*a));
+        if (d2 < r2i || d2 < cr[k]) out.push_back(io[lst[k]]);
+      }
+    }
+  }
+  return emit_sets(c, sets, offsets, ids, cap);
+}
+
+}  // namespace
+
+extern "C" {
+
 int64_t gdf_neighbour_sets(gdf_ctx* c, int mode, int64_t* offsets, int32_t* ids, int64_t cap) {
-  if (!c || !offsets || c->N <= 0 || (mode != 0 && mode != 1)) return GDF_ERR_INVALID;
+  if (!c || !offsets || c->N <= 0 || mode < 0 || mode > 2) return GDF_ERR_INVALID;
   if (!c->tree_valid) return fail(c, GDF_ERR_INVALID, "no tree");
+  if (mode == 2) return pass_sg_sets(c, offsets, ids, cap);
   std::vector<int> cidx, ncand;
   std::vector<unsigned long long> cmask;
   int capb = 0;

@@ -158,6 +158,12 @@ struct gdf_ctx {
   // per-group candidate sets: gather (h-rho search) and scatter-gather (forces)
   gdf::CandSet cs_g, cs_s;
   bool sg_valid = false;
+  bool sg_masks = false;        // the scatter-gather candidate set holds its masks
+  // mixed path: the last density pass's candidate lists (cs_g) also hold every scatter-gather
+  // pair of the final smoothing lengths (walked with the search radii as bounds); the force
+  // pass then reads them instead of walking the tree again (sg_from_union)
+  bool sg_union = false;
+  bool sg_from_union = false;
 
   // walk scratch
   int* stacks = nullptr;
@@ -285,7 +291,9 @@ void sph_free_lists(gdf_ctx* c);
 int sph_copy_sg(gdf_ctx* c, std::vector<int>& idx, std::vector<unsigned long long>& mask,
                 std::vector<int>& ncand, int& capb);
 int sph_density(gdf_ctx* c, int nopbc);
-int sph_build_sg(gdf_ctx* c);
+// masks: also record each target's candidate masks (the mixed path's hydro kernel otherwise
+// tests its candidates itself)
+int sph_build_sg(gdf_ctx* c, bool masks = false);
 int sph_hydro(gdf_ctx* c, int variant);
 
 // gravity (gdf_gravity.hip)

@@ -164,7 +164,7 @@ int batch_compact(gdf_ctx* c, const std::vector<int>& keep) {
   c->nseg = (int) keep.size();
   c->tree_valid = false;
   c->topo_valid = false;
-  c->sg_valid = false;
+  c->sg_valid = false; c->sg_union = false;
   if (c->nseg > 0) GDF_TRY(seg_upload(c));
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   return GDF_OK;

@@ -48,6 +48,14 @@ struct CandWalk {
   PhysParams p;
 };
 
+// The mixed gather pass's candidate radius^2 in FP32: k hs_i widened by 1e-6 of the group's
+// half-extent plus k hs_i, squared and widened by 1e-5 (r2i = (k hs_i)^2)
+__device__ __forceinline__ float gather_r2f(double r2i, const double4& gh) {
+#pragma clang fp contract(off)
+  const double rg = sqrt(r2i) + 1.0e-6*(gh.x + gh.y + gh.z + sqrt(r2i));
+  return (float) (rg*rg*(1.0 + 1.0e-5));
+}
+
 // MODE 0: search set |r_ij|^2 <= (k hs_i)^2 (FP64); 1: scatter-gather set
 // |r_ij|^2 < (k h_i)^2 or < (k h_j)^2 (FP64, the oracle's operation order); 2: the same set
 // decided in FP32 in the group-centred frame, exactly as the mixed-precision gravity walk
@@ -55,7 +63,13 @@ struct CandWalk {
 // radius widened by 1e-6 of the group's half-extent plus k hs_i (sixteen times the FP32
 // rounding of the frame's coordinates) and the square by 1e-5 (mixed h-rho pass: the extra
 // candidates lie beyond k h_i and carry zero weight).  PER: periodic nearest images.
-template <int MODE, bool PER>
+// MASKS = false (FP32 modes): the walk records the candidates only, and the pair pass that
+// reads them makes the same tests on its own staged records.  UNION (MODE 3, no masks): the
+// gather walk's candidates also include every particle whose own search radius reaches the
+// group box (hcand = the search radii hs_j = 1.05 h_j; a node's hs bound is 1.05 times its h
+// maximum), so that while no smoothing length outgrows its search radius the list holds the
+// scatter-gather set of the converged smoothing lengths as well.
+template <int MODE, bool PER, bool MASKS = true, bool UNION = false>
 __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
   constexpr bool F32 = MODE >= 2;      // group-frame FP32 tests
   constexpr bool GATHER = MODE == 0 || MODE == 3;
@@ -91,10 +105,7 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
     const float rxi = (float) (xi - gc.x), ryi = (float) (yi - gc.y), rzi = (float) (zi - gc.z);
     const float r2if = (float) r2i;
     float r2gf = 0.f;
-    if (MODE == 3 && has) {
-      const double rg = sqrt(r2i) + 1.0e-6*(gh.x + gh.y + gh.z + sqrt(r2i));
-      r2gf = (float) (rg*rg*(1.0 + 1.0e-5));
-    }
+    if (MODE == 3 && has) r2gf = gather_r2f(r2i, gh);
     int* idx = a.idx + (size_t) g*a.capb*kWave;
     unsigned long long* msk = a.mask + (size_t) g*a.capb*kWave;
     const int cap = a.capb*kWave;
@@ -121,7 +132,7 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
         const double gy = fmax(fabs(dy) - nh.y - gh.y, 0.0);
         const double gz = fmax(fabs(dz) - nh.z - gh.z, 0.0);
         const double dbox2 = gx*gx + gy*gy + gz*gz;
-        const double rr = GATHER ? kr*hG : kr*fmax(hG, nc.w);
+        const double rr = UNION ? kr*fmax(hG, 1.05*nc.w) : GATHER ? kr*hG : kr*fmax(hG, nc.w);
         if (dbox2 <= rr*rr*NODE_SLACK) {
           ef = nl.first; ec = nl.count;
           if (ec <= a.nleaf) emit = true;
@@ -151,7 +162,7 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
           const double ex = fmax(fabs(dx) - gh.x, 0.0);
           const double ey = fmax(fabs(dy) - gh.y, 0.0);
           const double ez = fmax(fabs(dz) - gh.z, 0.0);
-          const double rr = GATHER ? kr*hG : kr*fmax(hG, a.hcand[j]);
+          const double rr = (GATHER && !UNION) ? kr*hG : kr*fmax(hG, a.hcand[j]);
           if (ex*ex + ey*ey + ez*ez <= rr*rr*KEEP_SLACK) keep |= 1u << k;
         }
       }
@@ -170,6 +181,10 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
     const int nbat = (tail + kWave - 1)/kWave;
     if (tail > cap) {                // candidates beyond the set's capacity: grow and repeat
       if (lane == 0) { a.ncand[g] = tail; atomicMax(&a.flags[1], nbat); }
+      continue;
+    }
+    if (!MASKS) {
+      if (lane == 0) a.ncand[g] = tail;
       continue;
     }
     __threadfence_block();
@@ -241,7 +256,8 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
 struct DensArgs {
   int N, ngroups, capb, nopbc;
   const int* grp_node; const int* grp_first; const int* grp_count; const double4* geo_c;
-  const int* idx; const unsigned long long* mask; const int* ncand;
+  const double4* geo_h;
+  const int* idx; unsigned long long* mask; const int* ncand;
   const double4* posm;
   double* h; double* hsearch; double* rho; double* invomega; double* zeta; double* hfactor;
   double* u; double* sound; double* pressure;
@@ -305,8 +321,11 @@ __device__ void hrho_finish(const DensArgs& a, const PhysParams& p, int i, doubl
 
 // h-rho pass over one walk group per workgroup.  T = double: FP64 sums in candidate order;
 // T = float: FP32 pair terms in the group-centred frame, per-batch register sums and
-// Kahan-compensated totals (the convergence test and the final h stay FP64).
-template <typename T>
+// Kahan-compensated totals (the convergence test and the final h stay FP64).  FUSED (FP32): the
+// candidate walk recorded no masks; the first sweep tests each staged batch against the
+// target's widened radius exactly as the walk's MODE 3 would, keeps the masks for the later
+// sweeps and sums the same pairs.
+template <typename T, bool FUSED = false>
 __global__ __launch_bounds__(64) void k_density(DensArgs a) {
   __shared__ double4 sd[kWave];
   __shared__ float4 sf[kWave];
@@ -332,7 +351,10 @@ __global__ __launch_bounds__(64) void k_density(DensArgs a) {
   const double rsq = p.kern.rangesqd;
   const int nbat = (nc + kWave - 1)/kWave;
   const int* idx = a.idx + (size_t) g*a.capb*kWave;
-  const unsigned long long* msk = a.mask + (size_t) g*a.capb*kWave;
+  unsigned long long* msk = a.mask + (size_t) g*a.capb*kWave;
+  float r2gf = 0.f;
+  if (FUSED && state == 2) r2gf = gather_r2f(range2_rn(rsq, hmax), a.geo_h[a.grp_node[g]]);
+  bool first = true;
   while (__ballot(state == 2)) {
     const bool run = state == 2;
     if (run) iteration++;
@@ -357,7 +379,19 @@ __global__ __launch_bounds__(64) void k_density(DensArgs a) {
       }
       __syncthreads();
       if (run) {
-        unsigned long long mk = msk[b*kWave + lane];
+        unsigned long long mk;
+        if (FUSED && first) {
+          mk = 0;
+          #pragma unroll 16
+          for (int q = 0; q < kWave; q++) {
+            const float4 c = sf[q];
+            mk |= (unsigned long long) (dot3_f(c.x - rxi, c.y - ryi, c.z - rzi) <= r2gf) << q;
+          }
+          if (nb < kWave) mk &= (1ULL << nb) - 1ULL;
+          msk[b*kWave + lane] = mk;
+        } else {
+          mk = msk[b*kWave + lane];
+        }
         evals += __popcll(mk);
         if (sizeof(T) == 8) {
           while (mk) {
@@ -394,6 +428,7 @@ __global__ __launch_bounds__(64) void k_density(DensArgs a) {
       }
       __syncthreads();
     }
+    first = false;
     if (run) {
       double rho, invomega, zeta;
       if (sizeof(T) == 8) { rho = r_; invomega = om_; zeta = z_; }
@@ -431,6 +466,9 @@ __global__ void k_hrho_finish(DensArgs a) {
   if (i >= a.N || a.status[i] != 2) return;
   hrho_finish(a, a.p, i, a.posm[i].w, a.rho[i], a.invomega[i], a.zeta[i], a.hfactor[i]);
   a.status[i] = 0;
+  // a final h past the search radius (h from rho after the convergence test can exceed it by
+  // up to h_converge): the walk's union lists no longer bound the scatter-gather set
+  if (a.h[i] > a.hsearch[i]) atomicOr(&a.flags[3], 2);
 }
 
 // act (block timesteps): only the active particles iterate; NULL = all
@@ -466,6 +504,7 @@ struct HydroArgs {
   const int* grp_node; const int* grp_first; const int* grp_count; const double4* geo_c;
   const int* idx; const unsigned long long* mask; const int* ncand;
   const double4* posm; const double4* A; const double4* B;
+  const double* h;                  // smoothing lengths, for the fused scatter-gather tests
   const double* invomega; const double* pressure; const double* zeta;
   double* ax; double* ay; double* az; double* dudt; double* div_v;
   double* gx; double* gy; double* gz; double* gpot;    // softened neighbour gravity (VAR 1)
@@ -597,10 +636,12 @@ __global__ __launch_bounds__(64) void k_hydro(HydroArgs a) {
 // centre, velocities relative to the group's first particle), pair terms in FP32 with the
 // hardware reciprocal and square root, per-batch register sums, Kahan-compensated totals.
 // (at most 96 VGPRs: five wavefronts per SIMD instead of four, a few registers spilled)
-template <int VAR, bool BLK>
+// FUSED: the candidate walk recorded no masks; each staged batch is tested against both
+// smoothing ranges exactly as the walk's MODE 2 would, giving the same scatter-gather set.
+template <int VAR, bool BLK, bool FUSED = false>
 __global__ __launch_bounds__(64, 5) void k_hydro_mixed(HydroArgs a) {
   __shared__ float4 fP[kWave], fV[kWave], fB[kWave];
-  __shared__ float fZ[kWave];
+  __shared__ float fZ[kWave], fR[kWave];
   const int g = xcd_block(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x;
   const int nc = a.ncand[g];
@@ -627,6 +668,8 @@ __global__ __launch_bounds__(64, 5) void k_hydro_mixed(HydroArgs a) {
   const float pterm_i = (float) mB.y, snd_i = (float) mB.z, hf_i = (float) mB.w;
   const float alpha = (float) p.alpha_visc, beta = (float) p.beta_visc;
   const float invhsqd_i = invh_i*invh_i;
+  const double krsqd = p.kern.rangesqd;
+  const float r2if = (FUSED && has) ? (float) range2_rn(krsqd, a.h[i]) : 0.f;
   KahanF ax, ay, az, du, dv, gx, gy, gz, gp;
   long long npairs = 0;
   const int nbat = (nc + kWave - 1)/kWave;
@@ -644,11 +687,24 @@ __global__ __launch_bounds__(64, 5) void k_hydro_mixed(HydroArgs a) {
                              (float) (Aj.z - vref.z), (float) Aj.w);
       fB[lane] = make_float4((float) Bj.x, (float) Bj.y, (float) Bj.z, (float) Bj.w);
       if (VAR == 1) fZ[lane] = (float) a.zeta[j];
+      if (FUSED) fR[lane] = (float) range2_rn(krsqd, a.h[j]);
       if (BLK) { sJ[lane] = j; sL[lane] = blk_record(a, j); }
     }
     __syncthreads();
     if (has) {
-      unsigned long long mk = msk[b*kWave + lane];
+      unsigned long long mk;
+      if (FUSED) {
+        mk = 0;
+        #pragma unroll 8
+        for (int q = 0; q < kWave; q++) {
+          const float4 c = fP[q];
+          const float d2 = dot3_f(c.x - rxi, c.y - ryi, c.z - rzi);
+          mk |= (unsigned long long) ((d2 < r2if) | (d2 < fR[q])) << q;
+        }
+        if (nb < kWave) mk &= (1ULL << nb) - 1ULL;
+      } else {
+        mk = msk[b*kWave + lane];
+      }
       npairs += __popcll(mk);
       float bax = 0.f, bay = 0.f, baz = 0.f, bdu = 0.f, bdv = 0.f;
       float bgx = 0.f, bgy = 0.f, bgz = 0.f, bgp = 0.f;
@@ -762,15 +818,17 @@ int grow_cand(gdf_ctx* c, CandSet& cs, int need) {
 
 // glist: the groups holding targets (from group_list), or NULL to walk every group
 // (the walk's queue counters, counters[4, 12), must be zero: the list builder clears them)
+// masks = false (mixed path): the FP32 walks record candidates only (the pair pass tests them);
+// uni: the mixed gather walk's union lists (gather and scatter-gather candidates)
 int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status,
-                const int* glist) {
+                const int* glist, bool masks, bool uni = false) {
   CandWalk w;
   w.N = c->N; w.ngroups = c->ngroups; w.nleaf = c->phys.nleaf; w.capb = cs.capb;
   w.grp_node = c->grp_node; w.grp_first = c->grp_first; w.grp_count = c->grp_count;
   w.first = c->first; w.last = c->last; w.left = c->left; w.right = c->right;
   w.geo_c = c->geo_c; w.geo_h = c->geo_h;
   w.x = c->pf[F_X]; w.y = c->pf[F_Y]; w.z = c->pf[F_Z];
-  w.hcand = c->pf[F_H];
+  w.hcand = uni ? c->df[D_HSEARCH] : c->pf[F_H];
   w.htarget = mode == 0 ? c->df[D_HSEARCH] : c->pf[F_H];
   w.status = status;
   w.grp_root = c->nseg > 0 ? c->grp_root : nullptr;
@@ -787,13 +845,25 @@ int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status,
   if (w.p.periodic) {
     if (m == 0) k_walk_cand<0, true><<<grid, kWave, 0, st>>>(w);
     else if (m == 1) k_walk_cand<1, true><<<grid, kWave, 0, st>>>(w);
-    else if (m == 2) k_walk_cand<2, true><<<grid, kWave, 0, st>>>(w);
-    else k_walk_cand<3, true><<<grid, kWave, 0, st>>>(w);
+    else if (m == 2) {
+      if (masks) k_walk_cand<2, true><<<grid, kWave, 0, st>>>(w);
+      else k_walk_cand<2, true, false><<<grid, kWave, 0, st>>>(w);
+    } else {
+      if (masks) k_walk_cand<3, true><<<grid, kWave, 0, st>>>(w);
+      else if (uni) k_walk_cand<3, true, false, true><<<grid, kWave, 0, st>>>(w);
+      else k_walk_cand<3, true, false><<<grid, kWave, 0, st>>>(w);
+    }
   } else {
     if (m == 0) k_walk_cand<0, false><<<grid, kWave, 0, st>>>(w);
     else if (m == 1) k_walk_cand<1, false><<<grid, kWave, 0, st>>>(w);
-    else if (m == 2) k_walk_cand<2, false><<<grid, kWave, 0, st>>>(w);
-    else k_walk_cand<3, false><<<grid, kWave, 0, st>>>(w);
+    else if (m == 2) {
+      if (masks) k_walk_cand<2, false><<<grid, kWave, 0, st>>>(w);
+      else k_walk_cand<2, false, false><<<grid, kWave, 0, st>>>(w);
+    } else {
+      if (masks) k_walk_cand<3, false><<<grid, kWave, 0, st>>>(w);
+      else if (uni) k_walk_cand<3, false, false, true><<<grid, kWave, 0, st>>>(w);
+      else k_walk_cand<3, false, false><<<grid, kWave, 0, st>>>(w);
+    }
   }
   GDF_HIP(c, hipGetLastError());
   return GDF_OK;
@@ -827,7 +897,7 @@ int sph_density(gdf_ctx* c, int nopbc) {
   DensArgs d;
   d.N = N; d.ngroups = c->ngroups; d.nopbc = nopbc;
   d.grp_node = c->grp_node; d.grp_first = c->grp_first; d.grp_count = c->grp_count;
-  d.geo_c = c->geo_c;
+  d.geo_c = c->geo_c; d.geo_h = c->geo_h;
   d.posm = c->posm;
   d.h = c->pf[F_H]; d.hsearch = c->df[D_HSEARCH]; d.rho = c->df[D_RHO];
   d.invomega = c->df[D_INVOMEGA]; d.zeta = c->df[D_ZETA]; d.hfactor = c->df[D_HFACTOR];
@@ -837,41 +907,59 @@ int sph_density(gdf_ctx* c, int nopbc) {
   d.star_soft = c->cfg.nbody_softening;
   d.sx = c->sv[S_X]; d.sy = c->sv[S_Y]; d.sz = c->sv[S_Z]; d.sm = c->sv[S_M]; d.sh = c->sv[S_H];
   d.p = c->phys;
+  c->sg_union = false;
+  const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
+  // mixed path with periodic images: the first round walks the union lists; they stay valid for
+  // the force pass if that round settles every target (a regrowth or retry round would walk
+  // again only the unsettled groups, and a retry grows search radii past the lists' bounds)
+  const bool uni = mixed && !nopbc;
   for (int round = 0; round < 64; round++) {
     // also clears the round's flags and the walk's queue counters
     GDF_TRY(group_list(c, c->status, c->glist, c->cs_g.ncand, nullptr, nullptr, c->counters, 12));
-    GDF_TRY(launch_walk(c, 0, nopbc, c->cs_g, c->status, c->glist));
+    GDF_TRY(launch_walk(c, 0, nopbc, c->cs_g, c->status, c->glist, !mixed, uni));
     d.capb = c->cs_g.capb; d.idx = c->cs_g.idx; d.mask = c->cs_g.mask; d.ncand = c->cs_g.ncand;
-    if (c->cfg.precision == GDF_PREC_MIXED) k_density<float><<<c->ngroups, kWave, 0, s>>>(d);
+    if (mixed) k_density<float, true><<<c->ngroups, kWave, 0, s>>>(d);
     else k_density<double><<<c->ngroups, kWave, 0, s>>>(d);
     k_hrho_finish<<<nblocks(N, BS), BS, 0, s>>>(d);
     GDF_HIP(c, hipGetLastError());
     GDF_TRY(read_flags(c));
     const int* f = c->h_counters;
     if (f[2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in gather pass");
-    if (f[3]) return fail(c, GDF_ERR_NOCONV, "Problem with convergence of h-rho iteration");
+    if (f[3] & 1) return fail(c, GDF_ERR_NOCONV, "Problem with convergence of h-rho iteration");
     // groups whose candidates overflowed skipped the density kernel and keep status 1
     if (f[1] > 0) { GDF_TRY(grow_cand(c, c->cs_g, f[1])); continue; }
     if (f[0] == 0) {
       c->sg_valid = false;
-      return tree_refit_h(c);
+      GDF_TRY(tree_refit_h(c));
+      c->sg_union = uni && round == 0 && !(f[3] & 2);
+      return GDF_OK;
     }
   }
   return fail(c, GDF_ERR_CAPACITY, "h-rho neighbour search did not settle after 64 rounds");
 }
 
-int sph_build_sg(gdf_ctx* c) {
+int sph_build_sg(gdf_ctx* c, bool masks) {
+  // the mixed hydro kernel tests its candidates itself unless the masks are asked for
+  masks = masks || c->cfg.precision != GDF_PREC_MIXED;
+  c->sg_from_union = false;
+  if (!masks && c->sg_union) {      // the density pass's union lists hold the set
+    c->sg_from_union = true;
+    c->sg_valid = true;
+    c->sg_masks = false;
+    return GDF_OK;
+  }
   GDF_TRY(ensure_cand(c, c->cs_s));
   for (int round = 0; round < 8; round++) {
     const int* act = c->nlevels > 1 ? c->pi[I_ACT] : nullptr;
     int* sl = c->glist + (c->Ncap + 1);
     if (act) GDF_TRY(group_list(c, act, sl, c->cs_s.ncand, nullptr, nullptr, c->counters, 12));
     else k_zero_ints<<<1, BS, 0, c->stream>>>(4, c->counters);
-    GDF_TRY(launch_walk(c, 1, 0, c->cs_s, act, act ? sl : nullptr));
+    GDF_TRY(launch_walk(c, 1, 0, c->cs_s, act, act ? sl : nullptr, masks));
     GDF_TRY(read_flags(c));
     if (c->h_counters[2]) return fail(c, GDF_ERR_CAPACITY, "tree walk stack overflow in force pass");
     if (c->h_counters[1] > 0) { GDF_TRY(grow_cand(c, c->cs_s, c->h_counters[1])); continue; }
     c->sg_valid = true;
+    c->sg_masks = masks;
     return GDF_OK;
   }
   return fail(c, GDF_ERR_CAPACITY, "scatter-gather candidate capacity");
@@ -886,11 +974,12 @@ int sph_hydro(gdf_ctx* c, int variant) {
                                              c->df[D_INVOMEGA], c->df[D_SOUND],
                                              c->df[D_HFACTOR], c->hrecA, c->hrecB);
   HydroArgs h;
-  h.N = N; h.ngroups = c->ngroups; h.capb = c->cs_s.capb;
+  const CandSet& cs = c->sg_from_union ? c->cs_g : c->cs_s;
+  h.N = N; h.ngroups = c->ngroups; h.capb = cs.capb;
   h.grp_node = c->grp_node; h.grp_first = c->grp_first; h.grp_count = c->grp_count;
   h.geo_c = c->geo_c;
-  h.idx = c->cs_s.idx; h.mask = c->cs_s.mask; h.ncand = c->cs_s.ncand;
-  h.posm = c->posm; h.A = c->hrecA; h.B = c->hrecB;
+  h.idx = cs.idx; h.mask = cs.mask; h.ncand = cs.ncand;
+  h.posm = c->posm; h.A = c->hrecA; h.B = c->hrecB; h.h = c->pf[F_H];
   h.invomega = c->df[D_INVOMEGA]; h.pressure = c->df[D_PRESSURE]; h.zeta = c->df[D_ZETA];
   h.ax = c->df[D_AX]; h.ay = c->df[D_AY]; h.az = c->df[D_AZ];
   h.gx = c->df[D_ATX]; h.gy = c->df[D_ATY]; h.gz = c->df[D_ATZ]; h.gpot = c->df[D_GPOT];
@@ -901,20 +990,25 @@ int sph_hydro(gdf_ctx* c, int variant) {
   const bool blk = c->nlevels > 1;
   const int G = c->ngroups;
   const int kt = ktime_begin(c, s, 6);
+  const bool fused = mixed && !c->sg_masks;
   if (variant == 0) {
     if (blk) {
-      if (mixed) k_hydro_mixed<0, true><<<G, kWave, 0, s>>>(h);
+      if (fused) k_hydro_mixed<0, true, true><<<G, kWave, 0, s>>>(h);
+      else if (mixed) k_hydro_mixed<0, true><<<G, kWave, 0, s>>>(h);
       else k_hydro<0, true><<<G, kWave, 0, s>>>(h);
     } else {
-      if (mixed) k_hydro_mixed<0, false><<<G, kWave, 0, s>>>(h);
+      if (fused) k_hydro_mixed<0, false, true><<<G, kWave, 0, s>>>(h);
+      else if (mixed) k_hydro_mixed<0, false><<<G, kWave, 0, s>>>(h);
       else k_hydro<0, false><<<G, kWave, 0, s>>>(h);
     }
   } else {
     if (blk) {
-      if (mixed) k_hydro_mixed<1, true><<<G, kWave, 0, s>>>(h);
+      if (fused) k_hydro_mixed<1, true, true><<<G, kWave, 0, s>>>(h);
+      else if (mixed) k_hydro_mixed<1, true><<<G, kWave, 0, s>>>(h);
       else k_hydro<1, true><<<G, kWave, 0, s>>>(h);
     } else {
-      if (mixed) k_hydro_mixed<1, false><<<G, kWave, 0, s>>>(h);
+      if (fused) k_hydro_mixed<1, false, true><<<G, kWave, 0, s>>>(h);
+      else if (mixed) k_hydro_mixed<1, false><<<G, kWave, 0, s>>>(h);
       else k_hydro<1, false><<<G, kWave, 0, s>>>(h);
     }
   }
@@ -925,7 +1019,8 @@ int sph_hydro(gdf_ctx* c, int variant) {
 
 int sph_copy_sg(gdf_ctx* c, std::vector<int>& idx, std::vector<unsigned long long>& mask,
                 std::vector<int>& ncand, int& capb) {
-  if (!c->sg_valid) GDF_TRY(sph_build_sg(c));
+  // the mixed force pass leaves no masks: the set is walked again with them (same tests)
+  if (!c->sg_valid || !c->sg_masks) GDF_TRY(sph_build_sg(c, true));
   GDF_HIP(c, hipStreamSynchronize(c->stream));
   const CandSet& cs = c->cs_s;
   const size_t n = (size_t) c->ngroups*(size_t) cs.capb*kWave;

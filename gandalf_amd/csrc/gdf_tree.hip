@@ -1135,7 +1135,7 @@ int tree_build(gdf_ctx* c) {
   GDF_TRY(group_mac(c, c->df_alt[0] ? nullptr : c->vals));
   c->tree_valid = true;
   c->topo_valid = true;
-  c->sg_valid = false;
+  c->sg_valid = false; c->sg_union = false;
   GDF_HIP(c, hipGetLastError());
   return integ_pack_posm(c);
 }
@@ -1159,7 +1159,7 @@ int tree_restock(gdf_ctx* c) {
   GDF_TRY(integ_pack_posm(c));
   GDF_HIP(c, hipStreamSynchronize(s));
   c->tree_valid = true;
-  c->sg_valid = false;
+  c->sg_valid = false; c->sg_union = false;
   return GDF_OK;
 }
 

@@ -1,0 +1,52 @@
+"""The mixed path's force pass reads the density pass's candidate lists (walked with the search
+radii as bounds, so that they also hold every scatter-gather pair of the converged smoothing
+lengths) instead of walking the tree again.  The sets its hydro kernel evaluated, decoded from
+those lists with the kernel's FP32 tests, equal the sets of a fresh FP32 walk exactly, after the
+setup and after full steps, with and without periodic images."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_sim(d):
+    from gandalf_amd.gpu import GpuSim
+    s = GpuSim(d["meta"]["params"], precision="mixed")
+    s.set_particles(d["in_r"], d["in_v"], d["in_m"], d["in_u"])
+    return s
+
+
+def check_sets(s, what, block=False):
+    off2, ids2 = s.neighbour_sets(2)      # what the last force pass evaluated
+    if block:
+        # the setup's force pass had every particle active; the fresh walk takes the active
+        # flags, which the setup's end of step has cleared
+        s.set("active", np.ones(s.N))
+    off1, ids1 = s.neighbour_sets(1)      # a fresh walk of the tree
+    assert np.array_equal(off2, off1), what
+    assert np.array_equal(ids2, ids1), what
+    assert ids1.size > 20 * s.N
+
+
+@pytest.mark.parametrize("case,steps", [("box16_m4", 2), ("sphere4k_m4_bf", 2),
+                                        ("cloud4k_quintic_blk", 0)])
+def test_force_pass_sets_from_density_lists_are_exact(case, steps):
+    d = load_golden(case)
+    s = mixed_sim(d)
+    s.setup()
+    check_sets(s, "setup", block=d["meta"]["params"].get("Nlevels", 1) > 1)
+    for k in range(steps):
+        s.step(1)
+        check_sets(s, f"step {k + 1}")
+
+
+def test_fp64_path_has_no_pass_lists_to_decode():
+    from gandalf_amd.gpu import GdfError, GpuSim
+    d = load_golden("box16_m4")
+    s = GpuSim(d["meta"]["params"], precision="fp64")
+    s.set_particles(d["in_r"], d["in_v"], d["in_m"], d["in_u"])
+    s.setup()
+    with pytest.raises(GdfError):
+        s.neighbour_sets(2)
