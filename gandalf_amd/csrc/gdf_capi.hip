@@ -101,7 +101,7 @@ void free_particles(gdf_ctx* c) {
   for (int f = 0; f < I_NINT; f++) { hipFree(c->pi[f]); hipFree(c->pi_alt[f]); c->pi[f] = c->pi_alt[f] = nullptr; }
   void* ptrs[] = {c->iorig, c->iorig_alt, c->posm, c->hrecA, c->hrecB, c->grec, c->status,
                   c->keys, c->keys_alt, c->vals, c->vals_alt, c->sort_tmp, c->left, c->right,
-                  c->first, c->last, c->parent, c->visit, c->front, c->bbmin, c->bbmax, c->geo_c,
+                  c->first, c->last, c->parent, c->lnk, c->visit, c->front, c->bbmin, c->bbmax, c->geo_c,
                   c->geo_h, c->mp, c->gstart, c->grp_first, c->grp_count, c->grp_node, c->nsel,
                   c->sel_tmp, c->stacks, c->grav_nc,
                   c->grav_nn, c->tmpd, c->grp_root, c->stacks2, c->glist, c->grp_gmac};
@@ -109,6 +109,7 @@ void free_particles(gdf_ctx* c) {
   c->iorig = c->iorig_alt = nullptr; c->posm = c->hrecA = c->hrecB = c->grec = nullptr;
   c->status = nullptr; c->keys = c->keys_alt = nullptr; c->vals = c->vals_alt = nullptr;
   c->sort_tmp = nullptr; c->left = c->right = c->first = c->last = c->parent = c->visit = c->front = nullptr;
+  c->lnk = nullptr;
   c->bbmin = c->bbmax = nullptr; c->geo_c = c->geo_h = nullptr; c->mp = nullptr;
   c->gstart = c->grp_first = c->grp_count = c->grp_node = c->nsel = nullptr; c->sel_tmp = nullptr;
   c->stacks = nullptr;

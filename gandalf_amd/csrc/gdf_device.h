@@ -491,15 +491,13 @@ __device__ __forceinline__ void stack_put(int* lstk, int* gstk, int s, int v) {
 struct NodeLinks {
   int first, count, left, right;
 };
-__device__ __forceinline__ void load_links(int n, int N, const int* __restrict__ first,
-                                           const int* __restrict__ last,
-                                           const int* __restrict__ left,
-                                           const int* __restrict__ right, NodeLinks& l) {
+__device__ __forceinline__ void load_links(int n, int N, const int4* __restrict__ lnk,
+                                           NodeLinks& l) {
   if (n >= N - 1) {
     l.first = n - (N - 1); l.count = 1; l.left = -1; l.right = -1;
   } else {
-    const int f = first[n], e = last[n];
-    l.first = f; l.count = e - f + 1; l.left = left[n]; l.right = right[n];
+    const int4 k = lnk[n];           // (first, last, left, right)
+    l.first = k.x; l.count = k.y - k.x + 1; l.left = k.z; l.right = k.w;
   }
 }
 

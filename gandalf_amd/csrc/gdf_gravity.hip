@@ -38,7 +38,7 @@ constexpr int CCAP = 256;    // cell ring buffer (power of two)
 struct GravArgs {
   int N, ngroups, nleaf;
   const int* grp_node; const int* grp_first; const int* grp_count;
-  const int* first; const int* last; const int* left; const int* right;
+  const int4* lnk;
   const double4* geo_c; const double4* geo_h; const double* mp;
   const double4* posm; const double4* grec;     // grec: (h, zeta, hfactor, 0)
   double* atx; double* aty; double* atz; double* gpot;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(64) void k_gravity(GravArgs a) {
         // box, range and children in one round of loads
         const double4 nc = a.geo_c[node], nh = a.geo_h[node];
         NodeLinks nl;
-        load_links(node, N, a.first, a.last, a.left, a.right, nl);
+        load_links(node, N, a.lnk, nl);
         double dx = nc.x - gc.x, dy = nc.y - gc.y, dz = nc.z - gc.z;
         nearest_image<double>(a.p, dx, dy, dz);
         const double gx = fmax(fabs(dx) - nh.x - gh.x, 0.0);
@@ -704,7 +704,7 @@ GravArgs grav_args(gdf_ctx* c) {
   GravArgs a;
   a.N = c->N; a.ngroups = c->ngroups; a.nleaf = c->phys.nleaf;
   a.grp_node = c->grp_node; a.grp_first = c->grp_first; a.grp_count = c->grp_count;
-  a.first = c->first; a.last = c->last; a.left = c->left; a.right = c->right;
+  a.lnk = c->lnk;
   a.geo_c = c->geo_c; a.geo_h = c->geo_h; a.mp = c->mp;
   a.posm = c->posm; a.grec = c->grec;
   a.atx = c->df[D_ATX]; a.aty = c->df[D_ATY]; a.atz = c->df[D_ATZ]; a.gpot = c->df[D_GPOT];

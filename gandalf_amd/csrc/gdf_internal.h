@@ -143,6 +143,7 @@ struct gdf_ctx {
   void* sort_tmp = nullptr; size_t sort_tmp_bytes = 0;
   int* left = nullptr; int* right = nullptr; int* first = nullptr; int* last = nullptr;
   int* parent = nullptr; int* visit = nullptr;
+  int4* lnk = nullptr;          // internal node: (first, last, left, right) in one 16-byte record
   int* front = nullptr;                                // refit frontier [N] + count
   double* bbmin = nullptr; double* bbmax = nullptr;     // 3 per node
   double4* geo_c = nullptr; double4* geo_h = nullptr;

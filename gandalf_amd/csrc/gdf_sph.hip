@@ -36,7 +36,7 @@ constexpr int kInitBatches = 16;           // initial candidate capacity per gro
 struct CandWalk {
   int N, ngroups, nleaf, capb;
   const int* grp_node; const int* grp_first; const int* grp_count;
-  const int* first; const int* last; const int* left; const int* right;
+  const int4* lnk;
   const double4* geo_c; const double4* geo_h;
   const double* x; const double* y; const double* z; const double* hcand; const double* htarget;
   const int* status;                 // gather walks: only targets with status 1 take part
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
         // box, range and children in one round of loads
         const double4 nc = a.geo_c[node], nh = a.geo_h[node];
         NodeLinks nl;
-        load_links(node, N, a.first, a.last, a.left, a.right, nl);
+        load_links(node, N, a.lnk, nl);
         double dx = nc.x - gc.x, dy = nc.y - gc.y, dz = nc.z - gc.z;
         nearest_image<double>(pp, dx, dy, dz);
         const double gx = fmax(fabs(dx) - nh.x - gh.x, 0.0);
@@ -825,7 +825,7 @@ int launch_walk(gdf_ctx* c, int mode, int nopbc, CandSet& cs, const int* status,
   CandWalk w;
   w.N = c->N; w.ngroups = c->ngroups; w.nleaf = c->phys.nleaf; w.capb = cs.capb;
   w.grp_node = c->grp_node; w.grp_first = c->grp_first; w.grp_count = c->grp_count;
-  w.first = c->first; w.last = c->last; w.left = c->left; w.right = c->right;
+  w.lnk = c->lnk;
   w.geo_c = c->geo_c; w.geo_h = c->geo_h;
   w.x = c->pf[F_X]; w.y = c->pf[F_Y]; w.z = c->pf[F_Z];
   w.hcand = uni ? c->df[D_HSEARCH] : c->pf[F_H];

@@ -273,7 +273,7 @@ __global__ void k_grp_roots(int ngroups, const int* __restrict__ grp_first,
 // Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees, and k-d trees"
 __global__ void k_lbvh(int N, const uint64_t* __restrict__ key, int* __restrict__ left,
                        int* __restrict__ right, int* __restrict__ first, int* __restrict__ last,
-                       int* __restrict__ parent) {
+                       int* __restrict__ parent, int4* __restrict__ lnk) {
   const int i = blockIdx.x*BS + threadIdx.x;
   if (i >= N - 1) return;
   const int d = (delta(key, N, i, (long long) i + 1) - delta(key, N, i, (long long) i - 1)) >= 0 ? 1 : -1;
@@ -297,6 +297,7 @@ __global__ void k_lbvh(int N, const uint64_t* __restrict__ key, int* __restrict_
   const int L = (f == gamma) ? (N - 1 + gamma) : gamma;
   const int R = (la == gamma + 1) ? (N - 1 + gamma + 1) : gamma + 1;
   left[i] = L; right[i] = R;
+  lnk[i] = make_int4(f, la, L, R);
   parent[L] = i; parent[R] = i;
   if (i == 0) parent[0] = -1;
 }
@@ -802,6 +803,7 @@ int tree_alloc(gdf_ctx* c) {
   GDF_HIP(c, hipMalloc(&c->first, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->last, sizeof(int)*N));
   GDF_HIP(c, hipMalloc(&c->parent, sizeof(int)*nn));
+  GDF_HIP(c, hipMalloc(&c->lnk, sizeof(int4)*N));
   GDF_HIP(c, hipMalloc(&c->visit, sizeof(int)*(N + 16)));   // also 16-byte-padded flags
   // refit frontiers: phase 1 (one RB slot per chunk), chunk counts, phase 2 (N + count)
   GDF_HIP(c, hipMalloc(&c->front, sizeof(int)*(2*N + 1024 + N/1024 + 16)));
@@ -957,7 +959,7 @@ int build_body(gdf_ctx* c, hipStream_t s, bool launch) {
   // 4. topology
   if (N > 1)
     k_lbvh<<<nblocks(N - 1, BS), BS, 0, s>>>(N, c->keys, c->left, c->right, c->first, c->last,
-                                             c->parent);
+                                             c->parent, c->lnk);
   // 5. moments, bottom-up
   NodeArgs na = node_args(c);
   const double theta2 = c->phys.thetamaxsqd;
@@ -996,7 +998,7 @@ std::vector<const void*> build_key(const gdf_ctx* c) {
   }
   // every other buffer and value the launches take
   const void* other[] = {c->d_seg_off, c->seg_bnd, c->red, c->sort_tmp, c->sel_tmp, c->nsel,
-                         c->left, c->right, c->first, c->last, c->parent, c->visit, c->front,
+                         c->left, c->right, c->first, c->last, c->parent, c->lnk, c->visit, c->front,
                          c->bbmin, c->bbmax, c->geo_c, c->geo_h, c->mp, c->gstart, c->grp_first,
                          c->grp_count, c->grp_node};
   for (const void* q : other) k.push_back(q);
