@@ -32,8 +32,10 @@ sys.path.insert(0, ROOT)
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}   # MI355X dense vector peaks (FLOP/s)
 # algorithmic work per interaction (frozen accounting): density pair 45 (converged evaluation
 # only), hydro pair 75, hydro pair with the kernel-softened gravity of neighbours 135,
-# direct/monopole gravity term 22, quadrupole particle-cell term 70
-W_DENS, W_HYDRO, W_HYDRO_GRAV, W_PP, W_PC = 45, 75, 135, 22, 70
+# direct/monopole gravity term 22, quadrupole particle-cell term 70, candidate-target distance
+# test of the mixed hydro kernel 8 (three differences and a squared norm with two fused
+# multiply-adds; the kernel decides the scatter-gather sets from the density pass's lists)
+W_DENS, W_HYDRO, W_HYDRO_GRAV, W_PP, W_PC, W_TEST = 45, 75, 135, 22, 70, 8
 NLEVELS = 8          # block timestep levels of the workload
 
 
@@ -85,15 +87,17 @@ def kernel_rooflines(res, dtype):
     """The two pair kernels that dominate the step, each as algorithmic FLOPs over its own
     event-timed duration on the stream it runs on, summed over the timed steps' launches:
     k_hydro (135 FLOP per scatter-gather pair: hydro terms plus the kernel-softened gravity of
-    the pair, counted on the device) and k_gravity (22 FLOP per direct term -- walk particle
-    pairs minus the scatter-gather pairs -- and 70 per quadrupole particle-cell term).  The
-    gravity walk runs on a second stream beside the scatter-gather walk and the hydro kernel,
-    so both durations include the other's share of the device.  Returns (dominant, other)."""
+    the pair, and 8 per candidate-target distance test deciding the set, both counted on the
+    device) and k_gravity (22 FLOP per direct term -- walk particle pairs minus the
+    scatter-gather pairs -- and 70 per quadrupole particle-cell term).  The gravity walk runs
+    on a second stream beside the hydro kernel, so both durations include the other's share of
+    the device.  Returns (dominant, other)."""
     c, p = res["counts"], res["passes"]
     peak = PEAK[dtype]
     out = {}
     for name, flops, ms, nl in (
-            ("k_hydro", W_HYDRO_GRAV * c["sg_pairs"], p["k_hydro"], res["hlaunches"]),
+            ("k_hydro", W_HYDRO_GRAV * c["sg_pairs"] + W_TEST * c["sg_tests"], p["k_hydro"],
+             res["hlaunches"]),
             ("k_gravity", W_PP * (c["grav_pp"] - c["sg_pairs"]) + W_PC * c["grav_pc"],
              p["k_gravity"], res["launches"])):
         nl = max(nl, 1)
@@ -106,6 +110,7 @@ def kernel_rooflines(res, dtype):
     dom = max(out.values(), key=lambda r: r["total_ms"])
     other = [r for r in out.values() if r is not dom][0]
     loop_flops = (W_DENS * c["gather_pairs"] + W_HYDRO_GRAV * c["sg_pairs"] +
+                  W_TEST * c["sg_tests"] +
                   W_PP * (c["grav_pp"] - c["sg_pairs"]) + W_PC * c["grav_pc"])
     loop_rate = loop_flops / res["wall"]
     dom = dict(dom)

@@ -671,12 +671,13 @@ __global__ __launch_bounds__(64, 5) void k_hydro_mixed(HydroArgs a) {
   const double krsqd = p.kern.rangesqd;
   const float r2if = (FUSED && has) ? (float) range2_rn(krsqd, a.h[i]) : 0.f;
   KahanF ax, ay, az, du, dv, gx, gy, gz, gp;
-  long long npairs = 0;
+  long long npairs = 0, ntests = 0;
   const int nbat = (nc + kWave - 1)/kWave;
   const int* idx = a.idx + (size_t) g*a.capb*kWave;
   const unsigned long long* msk = a.mask + (size_t) g*a.capb*kWave;
   for (int b = 0; b < nbat; b++) {
     const int nb = (nc - b + nbat - 1)/nbat;   // candidates b, b + nbat, ...
+    if (FUSED && has) ntests += nb;
     if (lane < nb) {
       const int j = idx[b + lane*nbat];
       const double4 pj = a.posm[j], Aj = a.A[j], Bj = a.B[j];
@@ -760,6 +761,7 @@ __global__ __launch_bounds__(64, 5) void k_hydro_mixed(HydroArgs a) {
     __syncthreads();
   }
   wave_add_ll(&a.icount[1], npairs);
+  if (FUSED) wave_add_ll(&a.icount[5], ntests);
   if (!has) return;
   if (BLK) a.levelneib[i] = lmax;
   const double dvd = (double) dv.s*mB.x;

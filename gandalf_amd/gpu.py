@@ -354,7 +354,7 @@ class GpuSim:
         out = np.zeros(6, dtype=np.int64)
         self._chk(lib().gdf_interaction_counts(self._h, _p(out)))
         return dict(zip(("gather_pairs", "sg_pairs", "grav_pp", "grav_pc", "density_evals",
-                         "nodes"), (int(x) for x in out)))
+                         "sg_tests"), (int(x) for x in out)))
 
     # ---- parity taps ----
     def neighbour_sets(self, mode):

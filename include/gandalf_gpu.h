@@ -303,7 +303,8 @@ int gdf_tree_arrays(gdf_ctx* ctx, int32_t* perm, uint64_t* keys, int32_t* left,
                     int* ngroups);
 /* interaction counts of the last passes, for the roofline: {gather pairs of the converged
  * density evaluation, scatter-gather pairs, particle-particle gravity terms,
- * particle-cell gravity terms, density evaluations incl. iterations, walk nodes visited} */
+ * particle-cell gravity terms, density evaluations incl. iterations, candidate-target
+ * distance tests of the mixed hydro kernel (it decides the scatter-gather sets itself)} */
 int gdf_interaction_counts(gdf_ctx* ctx, int64_t* out6);
 /* per-group gravity interaction counts of the last gravity pass [ngroups] each */
 int gdf_gravity_group_counts(gdf_ctx* ctx, int32_t* ncell, int32_t* nnear);
