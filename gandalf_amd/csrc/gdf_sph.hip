@@ -152,27 +152,37 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
       // a bucket's members that lie beyond the largest range from the group box fail every
       // target's test below: they are dropped here, once per member, instead of being tested
       // on all 64 lanes and staged by every pair pass (about a quarter of the members of the
-      // buckets a group reaches)
-      unsigned int keep = 0;
-      if (emit) {
-        for (int k = 0; k < ec; k++) {
-          const int j = ef + k;
+      // buckets a group reaches).  The round's members are dealt one per lane, so that their
+      // loads go out together, and the kept ones are recorded in bucket order.
+      const int mcount = emit ? ec : 0;
+      const int mincl = wave_incl_scan(mcount);
+      const int mtot = __shfl(mincl, 63, 64);
+      for (int base = 0; base < mtot; base += kWave) {
+        const int s = base + lane;
+        // the lane whose bucket holds member s: the first with an inclusive count above s
+        int L = 0;
+        #pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
+          if (__shfl(mincl, L + step - 1, 64) <= s) L += step;
+        const int fL = __shfl(ef, L, 64), xL = __shfl(mincl - mcount, L, 64);
+        bool kp = false;
+        const int j = fL + (s - xL);
+        if (s < mtot) {
           double dx = a.x[j] - gc.x, dy = a.y[j] - gc.y, dz = a.z[j] - gc.z;
           nearest_image<double>(pp, dx, dy, dz);
           const double ex = fmax(fabs(dx) - gh.x, 0.0);
           const double ey = fmax(fabs(dy) - gh.y, 0.0);
           const double ez = fmax(fabs(dz) - gh.z, 0.0);
           const double rr = (GATHER && !UNION) ? kr*hG : kr*fmax(hG, a.hcand[j]);
-          if (ex*ex + ey*ey + ez*ez <= rr*rr*KEEP_SLACK) keep |= 1u << k;
+          kp = ex*ex + ey*ey + ez*ez <= rr*rr*KEEP_SLACK;
         }
+        const unsigned long long km = __ballot(kp);
+        if (kp) {
+          const int slot = tail + lanes_below(km);
+          if (slot < cap) idx[slot] = j;
+        }
+        tail += __popcll(km);
       }
-      const int kcount = __popc(keep);
-      const int incl = wave_incl_scan(kcount);
-      const int tot = __shfl(incl, 63, 64);
-      int slot = tail + incl - kcount;
-      for (unsigned int m = keep; m; m &= m - 1, slot++)
-        if (slot < cap) idx[slot] = ef + __ffs(m) - 1;
-      tail += tot;
     }
     if (overflow) {
       if (lane == 0) atomicOr(&a.flags[2], 1);
