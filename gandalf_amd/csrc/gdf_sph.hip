@@ -101,7 +101,20 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
       r2i = range2_rn(krsqd, hi);
     }
     const double hG = wave_max_d(has ? hi : 0.0);
+    // gc, gh: the group node's box, the frame of the FP32 tests.  tc, th: the box the walk
+    // tests nodes and members against, that of the group's targets alone (the refit's
+    // expressions, so it is the node's box when every member is a target); a block step's
+    // sparse groups hold a few targets each and reach far fewer buckets with it
     const double4 gc = a.geo_c[G], gh = a.geo_h[G];
+    double4 tc, th;
+    {
+#pragma clang fp contract(off)
+      const double lox = -wave_max_d(has ? -xi : -HUGE_VAL), hix = wave_max_d(has ? xi : -HUGE_VAL);
+      const double loy = -wave_max_d(has ? -yi : -HUGE_VAL), hiy = wave_max_d(has ? yi : -HUGE_VAL);
+      const double loz = -wave_max_d(has ? -zi : -HUGE_VAL), hiz = wave_max_d(has ? zi : -HUGE_VAL);
+      tc = make_double4(0.5*(lox + hix), 0.5*(loy + hiy), 0.5*(loz + hiz), 0.0);
+      th = make_double4(0.5*(hix - lox), 0.5*(hiy - loy), 0.5*(hiz - loz), 0.0);
+    }
     const float rxi = (float) (xi - gc.x), ryi = (float) (yi - gc.y), rzi = (float) (zi - gc.z);
     const float r2if = (float) r2i;
     float r2gf = 0.f;
@@ -126,11 +139,11 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
         const double4 nc = a.geo_c[node], nh = a.geo_h[node];
         NodeLinks nl;
         load_links(node, N, a.lnk, nl);
-        double dx = nc.x - gc.x, dy = nc.y - gc.y, dz = nc.z - gc.z;
+        double dx = nc.x - tc.x, dy = nc.y - tc.y, dz = nc.z - tc.z;
         nearest_image<double>(pp, dx, dy, dz);
-        const double gx = fmax(fabs(dx) - nh.x - gh.x, 0.0);
-        const double gy = fmax(fabs(dy) - nh.y - gh.y, 0.0);
-        const double gz = fmax(fabs(dz) - nh.z - gh.z, 0.0);
+        const double gx = fmax(fabs(dx) - nh.x - th.x, 0.0);
+        const double gy = fmax(fabs(dy) - nh.y - th.y, 0.0);
+        const double gz = fmax(fabs(dz) - nh.z - th.z, 0.0);
         const double dbox2 = gx*gx + gy*gy + gz*gz;
         const double rr = UNION ? kr*fmax(hG, 1.05*nc.w) : GATHER ? kr*hG : kr*fmax(hG, nc.w);
         if (dbox2 <= rr*rr*NODE_SLACK) {
@@ -168,11 +181,11 @@ __global__ __launch_bounds__(64) void k_walk_cand(CandWalk a) {
         bool kp = false;
         const int j = fL + (s - xL);
         if (s < mtot) {
-          double dx = a.x[j] - gc.x, dy = a.y[j] - gc.y, dz = a.z[j] - gc.z;
+          double dx = a.x[j] - tc.x, dy = a.y[j] - tc.y, dz = a.z[j] - tc.z;
           nearest_image<double>(pp, dx, dy, dz);
-          const double ex = fmax(fabs(dx) - gh.x, 0.0);
-          const double ey = fmax(fabs(dy) - gh.y, 0.0);
-          const double ez = fmax(fabs(dz) - gh.z, 0.0);
+          const double ex = fmax(fabs(dx) - th.x, 0.0);
+          const double ey = fmax(fabs(dy) - th.y, 0.0);
+          const double ez = fmax(fabs(dz) - th.z, 0.0);
           const double rr = (GATHER && !UNION) ? kr*hG : kr*fmax(hG, a.hcand[j]);
           kp = ex*ex + ey*ey + ez*ez <= rr*rr*KEEP_SLACK;
         }
