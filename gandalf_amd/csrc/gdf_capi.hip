@@ -741,8 +741,9 @@ int64_t emit_sets(gdf_ctx* c, std::vector<std::vector<int>>& sets, int64_t* offs
 // The scatter-gather sets the mixed path's last force pass evaluated: its candidate lists (the
 // density pass's union lists, or the force pass's own walk) decoded with the FP32 group-frame
 // test of its hydro kernel (positions relative to the group centre rounded to float, squared
-// ranges rounded to float, the squared distance with two fused multiply-adds), for every
-// target of the groups it walked.
+// ranges rounded to float, the squared distance with two fused multiply-adds), for the pass's
+// targets (its active flags in a block step, every particle otherwise): the walks bound the
+// lists by the targets' box, so they need not hold the sets of a group's other members.
 int64_t pass_sg_sets(gdf_ctx* c, int64_t* offsets, int32_t* ids, int64_t cap) {
   if (c->cfg.precision != GDF_PREC_MIXED || !c->sg_valid || c->sg_masks)
     return fail(c, GDF_ERR_INVALID, "no candidate lists of a mixed force pass to decode");
@@ -764,6 +765,12 @@ int64_t pass_sg_sets(gdf_ctx* c, int64_t* offsets, int32_t* ids, int64_t cap) {
   GDF_HIP(c, hipMemcpy(x.data(), c->pf[F_X], sizeof(double)*N, hipMemcpyDeviceToHost));
   GDF_HIP(c, hipMemcpy(y.data(), c->pf[F_Y], sizeof(double)*N, hipMemcpyDeviceToHost));
   GDF_HIP(c, hipMemcpy(z.data(), c->pf[F_Z], sizeof(double)*N, hipMemcpyDeviceToHost));
+  std::vector<int> tgt(N, 1);
+  if (!c->sg_tgt_all) {
+    if (!c->sg_tgt || c->sg_tgt_cap < N)
+      return fail(c, GDF_ERR_INVALID, "no target flags of the last force pass");
+    GDF_HIP(c, hipMemcpy(tgt.data(), c->sg_tgt, sizeof(int)*N, hipMemcpyDeviceToHost));
+  }
   const double krsqd = c->phys.kern.rangesqd;
   const double* hb = c->phys.boxhalf;
   const double* bs = c->phys.boxsize;
@@ -787,6 +794,7 @@ int64_t pass_sg_sets(gdf_ctx* c, int64_t* offsets, int32_t* ids, int64_t cap) {
     }
     for (int l = 0; l < gcount[g]; l++) {
       const int i = gfirst[g] + l;
+      if (!tgt[i]) continue;
       const float rx = (float) (x[i] - o.x), ry = (float) (y[i] - o.y), rz = (float) (z[i] - o.z);
       const float r2i = r2f(hh[i]);
       auto& out = sets[io[i]];

@@ -165,6 +165,11 @@ struct gdf_ctx {
   // pass then reads them instead of walking the tree again (sg_from_union)
   bool sg_union = false;
   bool sg_from_union = false;
+  // the last mixed block-step force pass's targets (its active flags, kept for decoding the
+  // sets it evaluated); sg_tgt_all: every particle was a target (global timesteps)
+  int* sg_tgt = nullptr;
+  int sg_tgt_cap = 0;
+  bool sg_tgt_all = true;
 
   // walk scratch
   int* stacks = nullptr;

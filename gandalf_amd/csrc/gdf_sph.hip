@@ -910,6 +910,10 @@ int sph_alloc_lists(gdf_ctx* c) {
 void sph_free_lists(gdf_ctx* c) {
   free_cand(c->cs_g);
   free_cand(c->cs_s);
+  if (c->sg_tgt) (void) hipFree(c->sg_tgt);
+  c->sg_tgt = nullptr;
+  c->sg_tgt_cap = 0;
+  c->sg_tgt_all = true;
 }
 
 int sph_density(gdf_ctx* c, int nopbc) {
@@ -1014,6 +1018,21 @@ int sph_hydro(gdf_ctx* c, int variant) {
   const bool mixed = c->cfg.precision == GDF_PREC_MIXED;
   const bool blk = c->nlevels > 1;
   const int G = c->ngroups;
+  if (mixed) {
+    // the pass's targets, for decoding the sets it evaluates (its lists cover those alone)
+    c->sg_tgt_all = !blk;
+    if (blk) {
+      if (c->sg_tgt_cap < N) {
+        if (c->sg_tgt) (void) hipFree(c->sg_tgt);
+        c->sg_tgt = nullptr;
+        c->sg_tgt_cap = 0;
+        GDF_HIP(c, hipMalloc(&c->sg_tgt, sizeof(int)*(size_t) c->Ncap));
+        c->sg_tgt_cap = c->Ncap;
+      }
+      GDF_HIP(c, hipMemcpyAsync(c->sg_tgt, c->pi[I_ACT], sizeof(int)*(size_t) N,
+                                hipMemcpyDeviceToDevice, s));
+    }
+  }
   const int kt = ktime_begin(c, s, 6);
   const bool fused = mixed && !c->sg_masks;
   if (variant == 0) {

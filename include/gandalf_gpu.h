@@ -286,9 +286,10 @@ int gdf_time(const gdf_ctx* ctx, double* t, double* timestep, int64_t* nsteps);
  * NeighbourManager::VerifyNeighbourList, NeighbourManager.h:574-684, checks the same sets) */
 /* exact neighbour sets after the last force pass, ascending ids per particle, import order.
  * mode 0 = gather {j : r2 < k^2 h_i^2}; mode 1 = scatter-gather (NeighbourManager.h:521), by a
- * fresh walk of the tree; mode 2 (mixed path) = the scatter-gather sets the last force pass
- * itself evaluated, decoded from the candidate lists it read with its FP32 tests.
- * offsets has N+1 entries.  With ids == NULL returns the total entry count; otherwise fills ids
+ * fresh walk of the tree, for the particles flagged active in a block-timestep run (every
+ * particle otherwise; the other rows are empty); mode 2 (mixed path) = the scatter-gather sets
+ * the last force pass itself evaluated, for that pass's targets, decoded from the candidate
+ * lists it read with its FP32 tests.  offsets has N+1 entries.  With ids == NULL returns the total entry count; otherwise fills ids
  * and returns the total, or GDF_ERR_CAPACITY if cap is smaller. */
 int64_t gdf_neighbour_sets(gdf_ctx* ctx, int mode, int64_t* offsets, int32_t* ids,
                            int64_t cap);

@@ -50,3 +50,39 @@ def test_fp64_path_has_no_pass_lists_to_decode():
     s.setup()
     with pytest.raises(GdfError):
         s.neighbour_sets(2)
+
+
+def test_sparse_block_step_sets_are_complete():
+    """A sparse block step walks each group's candidates with the bounding box of its targets
+    (a few of its members).  The sets the step's force pass evaluated, for its targets, equal a
+    fresh walk's with those particles flagged active, and the FP64 scatter-gather sets of the
+    oracle's brute-force search (NeighbourManager.h:521) up to pairs within 1e-5 of the range,
+    where the FP32 test may decide either way."""
+    import oracle as O
+    d = load_golden("cloud4k_quintic_blk")
+    s = mixed_sim(d)
+    s.setup()
+    for k in range(16):
+        s.step(1)
+        off2, ids2 = s.neighbour_sets(2)
+        tgt = np.diff(off2) > 0
+        if 0 < tgt.sum() < s.N // 4:
+            break
+    else:
+        pytest.fail("no sparse block step in 16 steps")
+    print(f"step {k + 1}: {int(tgt.sum())} targets", flush=True)
+    r, h = s.get("r"), s.get("h")
+    p = O.params_from_dict(d["meta"]["params"])
+    offo, idso = O.neighbour_sets(p, r, h, 1)
+    KR2 = (2.0 if p.kernel == 0 else 3.0)**2     # squared kernel range: M4 2, quintic 3
+    for i in np.flatnonzero(tgt):
+        a = set(ids2[off2[i]:off2[i + 1]].tolist())
+        b = set(idso[offo[i]:offo[i + 1]].tolist())
+        for j in a ^ b:
+            d2 = float(np.sum((r[j] - r[i])**2))
+            R2 = KR2 * max(h[i], h[j])**2
+            assert abs(d2 - R2) <= 1e-5 * R2, (i, j, d2, R2)
+    s.set("active", tgt.astype(float))
+    off1, ids1 = s.neighbour_sets(1)
+    assert np.array_equal(off2, off1)
+    assert np.array_equal(ids2, ids1)
