@@ -14,12 +14,14 @@ namespace gdf {
 // Walk groups hold at most one wavefront of targets.
 constexpr int kWave = 64;
 constexpr int kMaxGroup = 64;
-// Persistent wavefronts of the candidate walks and of the gravity walk (4 per SIMD; 8 per SIMD
-// made the candidate walks slower); walk kernels grab groups from a counter.
-constexpr int kWalkSlots = 4096;
+// Persistent wavefronts of the candidate walks (3 per SIMD: 4, 6 and 8 were slower, more walks
+// in flight contending for the caches) and of the gravity walk (4 per SIMD); walk kernels grab
+// groups from a counter.
+constexpr int kWalkSlots = 3072;
 constexpr int kGravSlots = 4096;
-// Per-slot stack capacity of the tree walks (node indices).
+// Per-slot stack capacity of the tree walks (node indices); the stack regions serve both walks.
 constexpr int kStackCap = 8192;
+constexpr int kStackSlots = kWalkSlots > kGravSlots ? kWalkSlots : kGravSlots;
 
 // Per-particle persistent fields, permuted with the particles at every tree build.
 // F_TLAST, F_PDT, F_DTNEXT: start time, length and next length of the particle's block step

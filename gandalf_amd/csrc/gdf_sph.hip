@@ -900,10 +900,10 @@ int sph_alloc_lists(gdf_ctx* c) {
   GDF_HIP(c, hipMalloc(&c->status, sizeof(int)*(size_t) c->Ncap));
   GDF_HIP(c, hipMalloc(&c->hrecA, sizeof(double4)*(size_t) c->Ncap));
   GDF_HIP(c, hipMalloc(&c->hrecB, sizeof(double4)*(size_t) c->Ncap));
-  GDF_HIP(c, hipMalloc(&c->stacks, sizeof(int)*(size_t) kWalkSlots*kStackCap));
+  GDF_HIP(c, hipMalloc(&c->stacks, sizeof(int)*(size_t) kStackSlots*kStackCap));
   GDF_HIP(c, hipMalloc(&c->glist, sizeof(int)*3*((size_t) c->Ncap + 1)));
   if (c->phys.self_gravity && c->phys.hydro_forces && !c->phys.nbody)
-    GDF_HIP(c, hipMalloc(&c->stacks2, sizeof(int)*(size_t) kWalkSlots*kStackCap));
+    GDF_HIP(c, hipMalloc(&c->stacks2, sizeof(int)*(size_t) kStackSlots*kStackCap));
   return GDF_OK;
 }
 
